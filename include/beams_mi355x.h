@@ -1,0 +1,191 @@
+/*
+ * beams_mi355x.h — C ABI of the MI355X (gfx950) lighted volume renderer.
+ *
+ * Drop-in replacement for the hot path of Beams (goodbadwolf/pilot): the
+ * transmittance-map build and the shadowed camera-ray march behind
+ * beams::rendering::LightedVolumeRenderer and MapperLightedVolume.  Each entry
+ * point names the reference interface it replaces (file:line in the pilot
+ * repository).
+ *
+ * Conventions
+ *  - Every call returns BEAMS_OK (0) or a BEAMS_ERR_* status; no exception or
+ *    signal crosses this boundary.  beams_last_error() gives the message of the
+ *    last failing call on a handle.
+ *  - One handle owns one HIP stream and its device buffers.  A handle is not
+ *    thread-safe; separate handles may be used from separate threads.
+ *  - Buffers are caller-owned.  `mem` says whether a pointer is host memory
+ *    (BEAMS_MEM_HOST: copied in/out synchronously) or device memory on the
+ *    handle's device (BEAMS_MEM_DEVICE: used in place, stream-ordered).
+ *  - Arrays are x-fastest, matching vtkm point/cell ordering.
+ */
+#ifndef BEAMS_MI355X_H
+#define BEAMS_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct beams_renderer beams_renderer;
+
+/* status codes */
+#define BEAMS_OK 0
+#define BEAMS_ERR_INVALID 1     /* bad argument: vtkm::cont::ErrorBadValue in the reference */
+#define BEAMS_ERR_UNSUPPORTED 2 /* unsupported dtype / association / mode                   */
+#define BEAMS_ERR_STATE 3       /* call order (e.g. render before set_field)               */
+#define BEAMS_ERR_HIP 4         /* HIP runtime failure                                     */
+#define BEAMS_ERR_NO_DEVICE 5   /* no gfx950 device visible                                */
+
+/* element types */
+#define BEAMS_F32 0
+#define BEAMS_F16 1
+#define BEAMS_BF16 2
+#define BEAMS_F64 3
+
+/* memory kinds */
+#define BEAMS_MEM_HOST 0
+#define BEAMS_MEM_DEVICE 1
+
+/* field association (vtkm::cont::Field::Association) */
+#define BEAMS_ASSOC_POINTS 0
+#define BEAMS_ASSOC_CELLS 1
+
+/* light kinds; the reference has point lights only (PointLight.h:15-30) */
+#define BEAMS_LIGHT_POINT 0
+#define BEAMS_LIGHT_DIRECTIONAL 1
+
+/* shadow-map construction */
+#define BEAMS_SHADOW_REF_MARCH 0 /* per-vertex light march, TransmittanceMap.h:239-340 */
+
+/* phases reported by beams_get_timings (Beams.cxx:30-34 Phase1..5Time) */
+#define BEAMS_PHASE_MAP 0       /* phase 1: transmittance map build           */
+#define BEAMS_PHASE_EXCHANGE 1  /* phase 2: non-local hit exchange (bricks)   */
+#define BEAMS_PHASE_REMAP 2     /* phase 3: seeded map re-march (bricks)      */
+#define BEAMS_PHASE_MARCH 3     /* phase 4: CalcRayStart + Sampler            */
+#define BEAMS_PHASE_COMPOSITE 4 /* phase 5: brick composite + background      */
+#define BEAMS_NUM_PHASES 5
+
+/* Pinhole camera for beams_render_image.  Replaces the vtkm::rendering::Camera ->
+ * raytracing::Camera::CreateRays step (MapperLightedVolume.cxx:141-152). */
+typedef struct beams_camera {
+  double eye[3];
+  double look_at[3];
+  double up[3];
+  double fov_y_deg;  /* vertical field of view                            */
+  double near_dist;  /* initial Ray::MinDistance (0 for a fresh canvas)  */
+  double far_dist;   /* initial Ray::MaxDistance (<= 0: unbounded)       */
+} beams_camera;
+
+/* ---- lifetime: LightedVolumeRenderer::LightedVolumeRenderer (LightedVolumeRenderer.cxx:752-759)
+ * Defaults follow the reference: sample distance unset (|extent|/200 at render),
+ * shadow map on, 16^3 map cells, early termination at alpha >= 1. */
+int beams_create(beams_renderer** out);
+int beams_destroy(beams_renderer* h);
+const char* beams_last_error(const beams_renderer* h);
+/* Wait for all work queued on the handle's stream. */
+int beams_synchronize(beams_renderer* h);
+
+/* ---- LightedVolumeRenderer::SetData (LightedVolumeRenderer.cxx:766-779) -------------------
+ * values: scalars, dims[0]*dims[1]*dims[2] of them for points, (dims-1) products for cells.
+ * dims: point dimensions of the structured grid (CellSetStructured<3>).
+ * coords: three rectilinear coordinate arrays (ArrayHandleCartesianProduct), host memory,
+ *   dims[i] increasing values each; or NULL for uniform coordinates given by origin and
+ *   spacing (held as x_i = float(origin) + float(spacing) * i, what ArrayHandleCounting holds).
+ * dtype: BEAMS_F32, BEAMS_F16 or BEAMS_BF16 storage; arithmetic is FP32. */
+int beams_set_field(beams_renderer* h, const void* values, int32_t dtype, int32_t assoc,
+                    const int64_t dims[3], const float* const coords[3], const double origin[3],
+                    const double spacing[3], int32_t mem);
+
+/* scalarRange argument of SetData (vtkm::Range). */
+int beams_set_scalar_range(beams_renderer* h, double min_value, double max_value);
+
+/* ---- LightedVolumeRenderer::SetColorMap (LightedVolumeRenderer.cxx:761-764) --------------
+ * n RGBA float entries (ArrayHandle<Vec4f_32>), host memory. */
+int beams_set_colormap(beams_renderer* h, const float* rgba, int64_t n);
+
+/* ---- LightedVolumeRenderer::ClearLights + AddLight (LightedVolumeRenderer.cxx:835-846) ---
+ * Point light: xyz = position.  Directional (extension): xyz = direction of travel.
+ * The light colour used is color * intensity, as Lights::AddLight stores it. */
+int beams_set_light(beams_renderer* h, int32_t kind, const float xyz[3], const float color[3],
+                    float intensity);
+
+/* ---- SetUseShadowMap / SetShadowMapSize (LightedVolumeRenderer.h:59-62) ------------------
+ * map_cells: cells per axis of the transmittance map (the map has cells+1 vertices).
+ * mode: BEAMS_SHADOW_REF_MARCH.  map_dtype: BEAMS_F32 or BEAMS_F16 storage. */
+int beams_set_shadow(beams_renderer* h, int32_t enabled, const int64_t map_cells[3],
+                     int32_t mode, int32_t map_dtype);
+
+/* ---- LightedVolumeRenderer::SetSampleDistance (LightedVolumeRenderer.cxx:1374-1379) ------
+ * Non-positive distances are rejected (BEAMS_ERR_INVALID), as the reference throws. */
+int beams_set_sample_distance(beams_renderer* h, float distance);
+
+/* Early-termination threshold on accumulated alpha; the reference uses 1.0
+ * (LightedVolumeRenderer.cxx:491).  Extension. */
+int beams_set_alpha_stop(beams_renderer* h, float alpha);
+
+/* ---- LightedVolumeRenderer::Render(Ray<Float32>&) (LightedVolumeRenderer.cxx:807-825,910-1372)
+ * The parity entry point: builds the transmittance map, then runs CalcRayStart and the
+ * sampler over n rays given as the Ray<Float32> SoA.  min_d/max_d/dist are in/out as in
+ * the reference (MinDistance, MaxDistance, Distance); rgba is Buffers[0], four
+ * interleaved floats per ray, read and accumulated in place. */
+int beams_render_rays(beams_renderer* h, int64_t n, const float* const origin[3],
+                      const float* const dir[3], float* min_d, float* max_d, float* dist,
+                      float* rgba, int32_t mem);
+
+/* ---- MapperLightedVolume::RenderCells (MapperLightedVolume.cxx:114-193) ------------------
+ * Full frame for a width x height image: camera rays, map build, march, optional
+ * background blend (background may be NULL: none; else RGBA blended under the image as
+ * CanvasRayTracer::BlendBackground does).  rgba: width*height*4 floats; depth:
+ * width*height floats (the ray entry distance, +inf where the ray misses), may be NULL. */
+int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width,
+                       int32_t height, const float* background, float* rgba, float* depth,
+                       int32_t mem);
+
+/* Copy of the last transmittance (opacity) map, (cells+1)^3 FP32 values; the debugging
+ * dump SaveTransmittance (TransmittanceMap.h:527-537) writes the same field. */
+int beams_get_map(beams_renderer* h, float* out, int32_t mem);
+
+/* Per-phase device times of the last render in milliseconds (BEAMS_NUM_PHASES values),
+ * replacing the Phase1..5Time globals (Beams.cxx:30-34). */
+int beams_get_timings(beams_renderer* h, float* phase_ms, int32_t n);
+
+/* ---- DataSetFactory / SceneFactory inputs (DataSetFactory.h:15, SceneFactory.h:13) -------
+ * Seeded synthetic scenes for the benchmark configurations. */
+typedef struct beams_field_recipe {
+  uint64_t seed;
+  int32_t nblobs;        /* isotropic Gaussian blobs                                   */
+  int32_t noise_octaves; /* lattice value-noise octaves (0: none)                      */
+  double noise_amp;      /* amplitude of octave 0                                      */
+  double noise_freq;     /* base lattice frequency                                     */
+  double time;           /* time step: blobs drift, noise phase advances               */
+  double drift_speed;    /* blob speed in box lengths per time step                    */
+  double noise_phase_rate; /* lattice offset per time step                             */
+} beams_field_recipe;
+
+/* Field on the unit cube sampled at dims points per axis, normalised to [0,1].
+ * out: dims product values of dtype (BEAMS_F32/F16/BF16/F64) in `mem` memory.
+ * Host and device generation give identical values. */
+int beams_gen_field(const beams_field_recipe* r, const int64_t dims[3], int32_t dtype,
+                    void* out, int32_t mem);
+
+/* Piecewise-linear RGBA table of n entries between npoints seeded control points
+ * (x = 0 and x = 1 included, RGBA uniform in [0,1]).  rgba32 and/or rgba64 may be NULL. */
+int beams_gen_colormap(uint64_t seed, int32_t npoints, int64_t n, float* rgba32, double* rgba64);
+
+/* Orbit camera looking at the centre of bounds (xmin,xmax,ymin,ymax,zmin,zmax) from a
+ * distance of one box diagonal, at azimuth/elevation in degrees, up = +y. */
+int beams_gen_orbit_camera(const double bounds[6], double azimuth_deg, double elevation_deg,
+                           double fov_y_deg, beams_camera* out);
+
+/* The FP32 rays beams_render_image generates for cam at width x height, as SoA
+ * (host memory, width*height each, pixel index = row * width + column, row 0 on top);
+ * dir64/org64 (may be NULL) receive the same rays before rounding to FP32. */
+int beams_gen_camera_rays(const beams_camera* cam, int32_t width, int32_t height,
+                          float* const origin[3], float* const dir[3], double* const origin64[3],
+                          double* const dir64[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,693 @@
+// C-ABI layer: a handle holding the renderer state of
+// beams::rendering::LightedVolumeRenderer (LightedVolumeRenderer.h:20-101) with
+// device-resident inputs, and the frame driver of RenderOnDevice
+// (LightedVolumeRenderer.cxx:910-1372) over the HIP kernels.
+#include "../../include/beams_mi355x.h"
+#include "camera.hpp"
+#include "gen.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace beams;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n)
+  {
+    if (n <= bytes && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(n, 256));
+    if (e == hipSuccess) bytes = std::max<size_t>(n, 256);
+    return e;
+  }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+size_t dtype_size(int32_t dt)
+{
+  switch (dt) {
+    case BEAMS_F32: return 4;
+    case BEAMS_F16:
+    case BEAMS_BF16: return 2;
+    case BEAMS_F64: return 8;
+    default: return 0;
+  }
+}
+
+}  // namespace
+
+struct beams_renderer {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // SetData
+  DevBuf field;
+  int32_t dtype = -1, assoc = 0;
+  int64_t dims[3] = { 0, 0, 0 };
+  std::vector<float> hx[3];
+  DevBuf coords;
+  double bounds[6] = { 0, 0, 0, 0, 0, 0 };
+  bool has_field = false;
+  double smin = 0.0, smax = 1.0;
+  bool has_range = false;
+  // SetColorMap
+  std::vector<float> htf;
+  DevBuf tf;
+  // AddLight (light 0 only)
+  int32_t light_kind = BEAMS_LIGHT_POINT;
+  float light[3] = { 0, 0, 0 }, light_color[3] = { 1, 1, 1 };
+  bool has_light = false;
+  // shadow map
+  bool use_shadow = true;
+  int64_t map_cells[3] = { 16, 16, 16 };
+  int32_t shadow_mode = BEAMS_SHADOW_REF_MARCH;
+  int32_t map_dtype = BEAMS_F32;
+  DevBuf map;
+  bool map_valid = false;
+  // march
+  float sample_distance = -1.0f;
+  float alpha_stop = 1.0f;
+  DevBuf rays, image;
+  hipEvent_t ev[BEAMS_NUM_PHASES + 1] = {};
+  bool timed[BEAMS_NUM_PHASES] = {};
+
+  int fail(int code, const std::string& msg)
+  {
+    err = msg;
+    return code;
+  }
+};
+
+#define HIP_TRY(h, expr)                                                                   \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return (h)->fail(BEAMS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHECK_HANDLE(h) \
+  if (!(h)) return BEAMS_ERR_INVALID
+
+namespace {
+
+hipMemcpyKind copy_kind(int32_t mem, bool to_device)
+{
+  if (mem == BEAMS_MEM_DEVICE) return hipMemcpyDeviceToDevice;
+  return to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+}
+
+// GridParams for the handle's block (SetData state).
+GridParams grid_params(const beams_renderer* h)
+{
+  GridParams g{};
+  g.field = h->field.p;
+  g.dtype = h->dtype;
+  g.assoc = h->assoc;
+  for (int d = 0; d < 3; ++d) g.n[d] = h->dims[d];
+  const int64_t sx = (h->assoc == 1) ? h->dims[0] - 1 : h->dims[0];
+  const int64_t sy = (h->assoc == 1) ? h->dims[1] - 1 : h->dims[1];
+  g.field_stride[0] = 1;
+  g.field_stride[1] = sx;
+  g.field_stride[2] = sx * sy;
+  const float* base = (const float*)h->coords.p;
+  g.x[0] = base;
+  g.x[1] = base + h->dims[0];
+  g.x[2] = base + h->dims[0] + h->dims[1];
+  for (int d = 0; d < 3; ++d) {
+    const std::vector<float>& x = h->hx[d];
+    g.minp[d] = x.front();
+    g.maxp[d] = x.back();
+    g.gx0[d] = x.front();
+    g.ginv[d] = (float)((double)(h->dims[d] - 1) / ((double)x.back() - (double)x.front()));
+  }
+  std::memcpy(g.bounds, h->bounds, sizeof(g.bounds));
+  // Math.h:12-20 BoundsMagnitude<Float32>
+  const float e0 = (float)(h->bounds[1] - h->bounds[0]);
+  const float e1 = (float)(h->bounds[3] - h->bounds[2]);
+  const float e2 = (float)(h->bounds[5] - h->bounds[4]);
+  g.ext_mag = std::sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+  return g;
+}
+
+// CreateDataSetForOpacityMap + GenerateEstimator set-up (TransmittanceMap.h:550-621).
+MapParams map_params(const beams_renderer* h, const GridParams& g)
+{
+  MapParams m{};
+  m.map = h->map.p;
+  m.dtype = h->map_dtype;
+  m.light_kind = h->light_kind;
+  float size[3];
+  for (int d = 0; d < 3; ++d) {
+    m.mp[d] = h->map_cells[d] + 1;
+    m.morg[d] = (float)h->bounds[2 * d];
+    size[d] = (float)(h->bounds[2 * d + 1] - h->bounds[2 * d]);
+    m.msp[d] = size[d] / (float)h->map_cells[d];
+    m.minv[d] = 1.0f / m.msp[d];
+    m.light[d] = h->light[d];
+    m.light_color[d] = h->light_color[d];
+  }
+  m.step = std::sqrt(size[0] * size[0] + size[1] * size[1] + size[2] * size[2]) / 128.0f;
+  m.dir_far = 4.0f * g.ext_mag;
+  return m;
+}
+
+ShadeParams shade_params(beams_renderer* h, const GridParams& g)
+{
+  ShadeParams s{};
+  s.tf = (const float*)h->tf.p;
+  s.ntf = (int32_t)(h->htf.size() / 4);
+  float ma = 0.0f;
+  for (int32_t i = 0; i < s.ntf; ++i) ma = std::fmax(ma, h->htf[4 * i + 3]);
+  s.max_density = ma;
+  const float mn = (float)h->smin, mx = (float)h->smax;
+  s.min_scalar = mn;
+  s.inv_delta = ((mx - mn) != 0.0f) ? 1.0f / (mx - mn) : mn;
+  // Render() stores the default once (LightedVolumeRenderer.cxx:815-819)
+  if (h->sample_distance <= 0.0f) h->sample_distance = g.ext_mag / 200.0f;
+  s.sample_dist = h->sample_distance;
+  s.mesh_eps = g.ext_mag * 0.0001f;  // LightedVolumeRenderer.cxx:1261
+  s.alpha_stop = h->alpha_stop;
+  s.use_map = h->use_shadow ? 1 : 0;
+  s.max_steps = (int64_t)(2.0f * g.ext_mag / s.sample_dist) + 64;
+  return s;
+}
+
+int check_ready(beams_renderer* h)
+{
+  if (!h->has_field) return h->fail(BEAMS_ERR_STATE, "no field: call beams_set_field first");
+  if (h->htf.empty()) return h->fail(BEAMS_ERR_STATE, "no colormap: call beams_set_colormap first");
+  if (!h->has_range) return h->fail(BEAMS_ERR_STATE, "no scalar range: call beams_set_scalar_range");
+  if (h->dtype == BEAMS_F64) return h->fail(BEAMS_ERR_UNSUPPORTED, "FP64 field storage is not rendered");
+  if (h->use_shadow && h->assoc == 0) {
+    if (!h->has_light) return h->fail(BEAMS_ERR_STATE, "shadow map needs a light: call beams_set_light");
+    float ma = 0.0f;
+    for (size_t i = 3; i < h->htf.size(); i += 4) ma = std::fmax(ma, h->htf[i]);
+    // The reference divides by the maximum alpha (TransmittanceMap.h:329) and produces NaN.
+    if (!(ma > 0.0f)) return h->fail(BEAMS_ERR_INVALID, "shadow map needs a colormap with nonzero alpha");
+  }
+  return BEAMS_OK;
+}
+
+int record(beams_renderer* h, int slot)
+{
+  HIP_TRY(h, hipEventRecord(h->ev[slot], h->stream));
+  return BEAMS_OK;
+}
+
+// Phase 1 (and 3, which equals phase 1 for a single block): build the map.
+int build_map(beams_renderer* h, const GridParams& g, const ShadeParams& s)
+{
+  const int64_t nv = (h->map_cells[0] + 1) * (h->map_cells[1] + 1) * (h->map_cells[2] + 1);
+  HIP_TRY(h, h->map.ensure((size_t)nv * dtype_size(h->map_dtype)));
+  MapParams m = map_params(h, g);
+  HIP_TRY(h, launch_tmap_ref(g, m, s, nullptr, h->stream));
+  h->map_valid = true;
+  return BEAMS_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+extern "C" {
+
+int beams_create(beams_renderer** out)
+{
+  if (!out) return BEAMS_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return BEAMS_ERR_NO_DEVICE;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return BEAMS_ERR_NO_DEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return BEAMS_ERR_NO_DEVICE;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return BEAMS_ERR_NO_DEVICE;
+  auto* h = new beams_renderer();
+  h->device = dev;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return BEAMS_ERR_HIP;
+  }
+  for (auto& e : h->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete h;
+      return BEAMS_ERR_HIP;
+    }
+  *out = h;
+  return BEAMS_OK;
+}
+
+int beams_destroy(beams_renderer* h)
+{
+  CHECK_HANDLE(h);
+  (void)hipStreamSynchronize(h->stream);
+  h->field.release();
+  h->coords.release();
+  h->tf.release();
+  h->map.release();
+  h->rays.release();
+  h->image.release();
+  for (auto& e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(h->stream);
+  delete h;
+  return BEAMS_OK;
+}
+
+const char* beams_last_error(const beams_renderer* h) { return h ? h->err.c_str() : "null handle"; }
+
+int beams_synchronize(beams_renderer* h)
+{
+  CHECK_HANDLE(h);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return BEAMS_OK;
+}
+
+int beams_set_field(beams_renderer* h, const void* values, int32_t dtype, int32_t assoc,
+                    const int64_t dims[3], const float* const coords[3], const double origin[3],
+                    const double spacing[3], int32_t mem)
+{
+  CHECK_HANDLE(h);
+  if (!values || !dims) return h->fail(BEAMS_ERR_INVALID, "null field values or dims");
+  if (dtype != BEAMS_F32 && dtype != BEAMS_F16 && dtype != BEAMS_BF16 && dtype != BEAMS_F64)
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "unsupported field dtype");
+  if (assoc != BEAMS_ASSOC_POINTS && assoc != BEAMS_ASSOC_CELLS)
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "field must be associated with points or cells");
+  if (mem != BEAMS_MEM_HOST && mem != BEAMS_MEM_DEVICE) return h->fail(BEAMS_ERR_INVALID, "bad mem kind");
+  for (int d = 0; d < 3; ++d)
+    if (dims[d] < 2 || dims[d] > (1 << 20)) return h->fail(BEAMS_ERR_INVALID, "point dims must be in [2, 2^20]");
+  if (!coords && (!origin || !spacing)) return h->fail(BEAMS_ERR_INVALID, "need coords or origin+spacing");
+  std::vector<float> x[3];
+  for (int d = 0; d < 3; ++d) {
+    x[d].resize((size_t)dims[d]);
+    if (coords) {
+      if (!coords[d]) return h->fail(BEAMS_ERR_INVALID, "null coordinate array");
+      std::memcpy(x[d].data(), coords[d], sizeof(float) * (size_t)dims[d]);
+    } else {
+      const float o = (float)origin[d], sp = (float)spacing[d];
+      for (int64_t i = 0; i < dims[d]; ++i) x[d][(size_t)i] = o + sp * (float)i;
+    }
+    for (int64_t i = 1; i < dims[d]; ++i)
+      if (!(x[d][(size_t)i] > x[d][(size_t)i - 1]))
+        return h->fail(BEAMS_ERR_INVALID, "coordinates must be strictly increasing");
+  }
+  const int64_t nvals = (assoc == BEAMS_ASSOC_CELLS) ? (dims[0] - 1) * (dims[1] - 1) * (dims[2] - 1)
+                                                     : dims[0] * dims[1] * dims[2];
+  const size_t bytes = (size_t)nvals * dtype_size(dtype);
+  HIP_TRY(h, h->field.ensure(bytes));
+  HIP_TRY(h, hipMemcpyAsync(h->field.p, values, bytes, copy_kind(mem, true), h->stream));
+  const size_t nc = (size_t)(dims[0] + dims[1] + dims[2]);
+  std::vector<float> flat;
+  flat.reserve(nc);
+  for (int d = 0; d < 3; ++d) flat.insert(flat.end(), x[d].begin(), x[d].end());
+  HIP_TRY(h, h->coords.ensure(sizeof(float) * nc));
+  HIP_TRY(h, hipMemcpyAsync(h->coords.p, flat.data(), sizeof(float) * nc, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  for (int d = 0; d < 3; ++d) {
+    h->hx[d] = std::move(x[d]);
+    h->dims[d] = dims[d];
+    double lo = h->hx[d].front(), hi = h->hx[d].back();
+    h->bounds[2 * d] = lo;
+    h->bounds[2 * d + 1] = hi;
+  }
+  h->dtype = dtype;
+  h->assoc = assoc;
+  h->has_field = true;
+  h->map_valid = false;
+  return BEAMS_OK;
+}
+
+int beams_set_scalar_range(beams_renderer* h, double min_value, double max_value)
+{
+  CHECK_HANDLE(h);
+  if (!std::isfinite(min_value) || !std::isfinite(max_value) || max_value < min_value)
+    return h->fail(BEAMS_ERR_INVALID, "scalar range must be finite with min <= max");
+  h->smin = min_value;
+  h->smax = max_value;
+  h->has_range = true;
+  return BEAMS_OK;
+}
+
+int beams_set_colormap(beams_renderer* h, const float* rgba, int64_t n)
+{
+  CHECK_HANDLE(h);
+  if (!rgba || n < 1 || n > 8192) return h->fail(BEAMS_ERR_INVALID, "colormap needs 1..8192 RGBA entries");
+  h->htf.assign(rgba, rgba + 4 * n);
+  HIP_TRY(h, h->tf.ensure(sizeof(float) * 4 * (size_t)n));
+  HIP_TRY(h, hipMemcpyAsync(h->tf.p, h->htf.data(), sizeof(float) * 4 * (size_t)n, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return BEAMS_OK;
+}
+
+int beams_set_light(beams_renderer* h, int32_t kind, const float xyz[3], const float color[3],
+                    float intensity)
+{
+  CHECK_HANDLE(h);
+  if (!xyz || !color) return h->fail(BEAMS_ERR_INVALID, "null light vector");
+  if (kind != BEAMS_LIGHT_POINT && kind != BEAMS_LIGHT_DIRECTIONAL)
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "unknown light kind");
+  for (int d = 0; d < 3; ++d)
+    if (!std::isfinite(xyz[d]) || !std::isfinite(color[d] * intensity))
+      return h->fail(BEAMS_ERR_INVALID, "light position and colour must be finite");
+  for (int d = 0; d < 3; ++d) {
+    h->light[d] = xyz[d];
+    h->light_color[d] = color[d] * intensity;  // Lights::AddLight(radius, pos, Color * Intensity)
+  }
+  if (kind == BEAMS_LIGHT_DIRECTIONAL) {
+    const float n2 = xyz[0] * xyz[0] + xyz[1] * xyz[1] + xyz[2] * xyz[2];
+    if (!(n2 > 0.0f)) return h->fail(BEAMS_ERR_INVALID, "directional light needs a nonzero direction");
+    const float rs = 1.0f / std::sqrt(n2);
+    for (int d = 0; d < 3; ++d) h->light[d] = xyz[d] * rs;
+  }
+  h->light_kind = kind;
+  h->has_light = true;
+  return BEAMS_OK;
+}
+
+int beams_set_shadow(beams_renderer* h, int32_t enabled, const int64_t map_cells[3], int32_t mode,
+                     int32_t map_dtype)
+{
+  CHECK_HANDLE(h);
+  if (mode != BEAMS_SHADOW_REF_MARCH) return h->fail(BEAMS_ERR_UNSUPPORTED, "unknown shadow mode");
+  if (map_dtype != BEAMS_F32 && map_dtype != BEAMS_F16)
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "map storage must be F32 or F16");
+  if (map_cells)
+    for (int d = 0; d < 3; ++d) {
+      if (map_cells[d] < 1 || map_cells[d] > 4096) return h->fail(BEAMS_ERR_INVALID, "map cells must be in [1, 4096]");
+      h->map_cells[d] = map_cells[d];
+    }
+  h->use_shadow = enabled != 0;
+  h->shadow_mode = mode;
+  h->map_dtype = map_dtype;
+  h->map_valid = false;
+  return BEAMS_OK;
+}
+
+int beams_set_sample_distance(beams_renderer* h, float distance)
+{
+  CHECK_HANDLE(h);
+  if (!(distance > 0.0f)) return h->fail(BEAMS_ERR_INVALID, "Sample distance must be positive.");
+  h->sample_distance = distance;
+  return BEAMS_OK;
+}
+
+int beams_set_alpha_stop(beams_renderer* h, float alpha)
+{
+  CHECK_HANDLE(h);
+  if (!(alpha > 0.0f) || alpha > 1.0f) return h->fail(BEAMS_ERR_INVALID, "alpha stop must be in (0, 1]");
+  h->alpha_stop = alpha;
+  return BEAMS_OK;
+}
+
+int beams_render_rays(beams_renderer* h, int64_t n, const float* const origin[3],
+                      const float* const dir[3], float* min_d, float* max_d, float* dist,
+                      float* rgba, int32_t mem)
+{
+  CHECK_HANDLE(h);
+  if (n < 0) return h->fail(BEAMS_ERR_INVALID, "negative ray count");
+  if (!origin || !dir || !min_d || !max_d || !dist || !rgba)
+    return h->fail(BEAMS_ERR_INVALID, "null ray array");
+  for (int d = 0; d < 3; ++d)
+    if (!origin[d] || !dir[d]) return h->fail(BEAMS_ERR_INVALID, "null ray array");
+  if (mem != BEAMS_MEM_HOST && mem != BEAMS_MEM_DEVICE) return h->fail(BEAMS_ERR_INVALID, "bad mem kind");
+  int rc = check_ready(h);
+  if (rc) return rc;
+  GridParams g = grid_params(h);
+  ShadeParams s = shade_params(h, g);
+  for (bool& t : h->timed) t = false;
+  if ((rc = record(h, 0))) return rc;
+  if (h->use_shadow && h->assoc == 0) {
+    if ((rc = build_map(h, g, s))) return rc;
+    h->timed[BEAMS_PHASE_MAP] = true;
+  }
+  if ((rc = record(h, 1))) return rc;
+  RaySoA r{};
+  r.n = n;
+  if (mem == BEAMS_MEM_DEVICE) {
+    for (int d = 0; d < 3; ++d) {
+      r.org[d] = origin[d];
+      r.dir[d] = dir[d];
+    }
+    r.min_d = min_d;
+    r.max_d = max_d;
+    r.dist = dist;
+    r.rgba = rgba;
+  } else {
+    const size_t fb = sizeof(float) * (size_t)n;
+    HIP_TRY(h, h->rays.ensure(fb * 13 + 64));
+    float* base = (float*)h->rays.p;
+    float* arr[9];
+    for (int k = 0; k < 9; ++k) arr[k] = base + (size_t)k * (size_t)n;
+    float* drgba = base + (size_t)9 * (size_t)n;
+    for (int d = 0; d < 3; ++d) {
+      HIP_TRY(h, hipMemcpyAsync(arr[d], origin[d], fb, hipMemcpyHostToDevice, h->stream));
+      HIP_TRY(h, hipMemcpyAsync(arr[3 + d], dir[d], fb, hipMemcpyHostToDevice, h->stream));
+      r.org[d] = arr[d];
+      r.dir[d] = arr[3 + d];
+    }
+    HIP_TRY(h, hipMemcpyAsync(arr[6], min_d, fb, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(arr[7], max_d, fb, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(arr[8], dist, fb, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(drgba, rgba, 4 * fb, hipMemcpyHostToDevice, h->stream));
+    r.min_d = arr[6];
+    r.max_d = arr[7];
+    r.dist = arr[8];
+    r.rgba = drgba;
+  }
+  if ((rc = record(h, 2))) return rc;
+  MapParams m = map_params(h, g);
+  HIP_TRY(h, launch_march_rays(g, m, s, r, h->stream));
+  if ((rc = record(h, 3))) return rc;
+  h->timed[BEAMS_PHASE_MARCH] = true;
+  if (mem == BEAMS_MEM_HOST) {
+    const size_t fb = sizeof(float) * (size_t)n;
+    HIP_TRY(h, hipMemcpyAsync(min_d, r.min_d, fb, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(max_d, r.max_d, fb, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(dist, r.dist, fb, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(rgba, r.rgba, 4 * fb, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+  }
+  return BEAMS_OK;
+}
+
+int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width, int32_t height,
+                       const float* background, float* rgba, float* depth, int32_t mem)
+{
+  CHECK_HANDLE(h);
+  if (!cam || width < 1 || height < 1 || width > 32768 || height > 32768)
+    return h->fail(BEAMS_ERR_INVALID, "bad camera or image size");
+  if (mem != BEAMS_MEM_HOST && mem != BEAMS_MEM_DEVICE) return h->fail(BEAMS_ERR_INVALID, "bad mem kind");
+  if (!rgba) return h->fail(BEAMS_ERR_INVALID, "null rgba output");
+  int rc = check_ready(h);
+  if (rc) return rc;
+  CameraParams c{};
+  {
+    double f[3], len = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      f[d] = cam->look_at[d] - cam->eye[d];
+      len += f[d] * f[d];
+    }
+    len = std::sqrt(len);
+    if (!(len > 0.0)) return h->fail(BEAMS_ERR_INVALID, "camera eye equals look_at");
+    for (int d = 0; d < 3; ++d) f[d] /= len;
+    double r[3] = { f[1] * cam->up[2] - f[2] * cam->up[1], f[2] * cam->up[0] - f[0] * cam->up[2],
+                    f[0] * cam->up[1] - f[1] * cam->up[0] };
+    double rl = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    if (!(rl > 0.0)) return h->fail(BEAMS_ERR_INVALID, "camera up is parallel to the view direction");
+    for (int d = 0; d < 3; ++d) r[d] /= rl;
+    double u[3] = { r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0] };
+    for (int d = 0; d < 3; ++d) {
+      c.eye[d] = cam->eye[d];
+      c.fwd[d] = f[d];
+      c.right[d] = r[d];
+      c.up[d] = u[d];
+    }
+    c.tan_half_fov = std::tan(cam->fov_y_deg * M_PI / 360.0);
+    c.aspect = (double)width / (double)height;
+    c.width = width;
+    c.height = height;
+    c.near_d = (float)cam->near_dist;
+    c.far_d = cam->far_dist > 0.0 ? (float)cam->far_dist : INFINITY;
+  }
+  GridParams g = grid_params(h);
+  ShadeParams s = shade_params(h, g);
+  for (bool& t : h->timed) t = false;
+  if ((rc = record(h, 0))) return rc;
+  if (h->use_shadow && h->assoc == 0) {
+    if ((rc = build_map(h, g, s))) return rc;
+    h->timed[BEAMS_PHASE_MAP] = true;
+  }
+  if ((rc = record(h, 1))) return rc;
+  if ((rc = record(h, 2))) return rc;
+  const size_t npix = (size_t)width * (size_t)height;
+  ImageOut o{};
+  if (mem == BEAMS_MEM_DEVICE) {
+    o.rgba = rgba;
+    o.depth = depth;
+  } else {
+    HIP_TRY(h, h->image.ensure(sizeof(float) * npix * 5));
+    o.rgba = (float*)h->image.p;
+    o.depth = depth ? (float*)h->image.p + 4 * npix : nullptr;
+  }
+  o.use_bg = background ? 1 : 0;
+  for (int k = 0; k < 4; ++k) o.bg[k] = background ? background[k] : 0.0f;
+  MapParams m = map_params(h, g);
+  HIP_TRY(h, launch_march_image(g, m, s, c, o, h->stream));
+  if ((rc = record(h, 3))) return rc;
+  h->timed[BEAMS_PHASE_MARCH] = true;
+  if (mem == BEAMS_MEM_HOST) {
+    HIP_TRY(h, hipMemcpyAsync(rgba, o.rgba, sizeof(float) * 4 * npix, hipMemcpyDeviceToHost, h->stream));
+    if (depth) HIP_TRY(h, hipMemcpyAsync(depth, o.depth, sizeof(float) * npix, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+  }
+  return BEAMS_OK;
+}
+
+int beams_get_map(beams_renderer* h, float* out, int32_t mem)
+{
+  CHECK_HANDLE(h);
+  if (!out) return h->fail(BEAMS_ERR_INVALID, "null output");
+  if (!h->map_valid) return h->fail(BEAMS_ERR_STATE, "no map: render with shadows first");
+  const int64_t nv = (h->map_cells[0] + 1) * (h->map_cells[1] + 1) * (h->map_cells[2] + 1);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  if (h->map_dtype == BEAMS_F32) {
+    HIP_TRY(h, hipMemcpy(out, h->map.p, sizeof(float) * (size_t)nv, copy_kind(mem, false)));
+  } else {
+    std::vector<uint16_t> tmp((size_t)nv);
+    HIP_TRY(h, hipMemcpy(tmp.data(), h->map.p, 2 * (size_t)nv, hipMemcpyDeviceToHost));
+    std::vector<float> f((size_t)nv);
+    for (size_t i = 0; i < f.size(); ++i) f[i] = f16_to_f32(tmp[i]);
+    HIP_TRY(h, hipMemcpy(out, f.data(), sizeof(float) * (size_t)nv,
+                         mem == BEAMS_MEM_DEVICE ? hipMemcpyHostToDevice : hipMemcpyHostToHost));
+  }
+  return BEAMS_OK;
+}
+
+int beams_get_timings(beams_renderer* h, float* phase_ms, int32_t n)
+{
+  CHECK_HANDLE(h);
+  if (!phase_ms || n < 1) return h->fail(BEAMS_ERR_INVALID, "null timings output");
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  float t[BEAMS_NUM_PHASES] = { 0, 0, 0, 0, 0 };
+  if (h->timed[BEAMS_PHASE_MAP]) HIP_TRY(h, hipEventElapsedTime(&t[BEAMS_PHASE_MAP], h->ev[0], h->ev[1]));
+  if (h->timed[BEAMS_PHASE_MARCH]) HIP_TRY(h, hipEventElapsedTime(&t[BEAMS_PHASE_MARCH], h->ev[2], h->ev[3]));
+  for (int i = 0; i < n && i < BEAMS_NUM_PHASES; ++i) phase_ms[i] = t[i];
+  return BEAMS_OK;
+}
+
+int beams_gen_field(const beams_field_recipe* r, const int64_t dims[3], int32_t dtype, void* out,
+                    int32_t mem)
+{
+  if (!r || !dims || !out) return BEAMS_ERR_INVALID;
+  if (r->nblobs < 0 || r->nblobs > 4096 || r->noise_octaves < 0 || r->noise_octaves > 16)
+    return BEAMS_ERR_INVALID;
+  for (int d = 0; d < 3; ++d)
+    if (dims[d] < 2) return BEAMS_ERR_INVALID;
+  if (dtype_size(dtype) == 0) return BEAMS_ERR_UNSUPPORTED;
+  FieldRecipe fr{ r->seed, r->nblobs, r->noise_octaves, r->noise_amp, r->noise_freq, r->time,
+                  r->drift_speed, r->noise_phase_rate };
+  std::string err;
+  if (mem == BEAMS_MEM_DEVICE) {
+    hipStream_t st;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return BEAMS_ERR_HIP;
+    int rc = gen_field(fr, dims, dtype, out, true, st, err);
+    (void)hipStreamDestroy(st);
+    return rc;
+  }
+  return gen_field(fr, dims, dtype, out, false, nullptr, err);
+}
+
+int beams_gen_colormap(uint64_t seed, int32_t npoints, int64_t n, float* rgba32, double* rgba64)
+{
+  if (npoints < 2 || n < 1) return BEAMS_ERR_INVALID;
+  gen_colormap(seed, npoints, n, rgba32, rgba64);
+  return BEAMS_OK;
+}
+
+int beams_gen_orbit_camera(const double bounds[6], double azimuth_deg, double elevation_deg,
+                           double fov_y_deg, beams_camera* out)
+{
+  if (!bounds || !out) return BEAMS_ERR_INVALID;
+  double c[3], diag2 = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    c[d] = 0.5 * (bounds[2 * d] + bounds[2 * d + 1]);
+    const double e = bounds[2 * d + 1] - bounds[2 * d];
+    diag2 += e * e;
+  }
+  const double dist = std::sqrt(diag2);
+  const double az = azimuth_deg * M_PI / 180.0, el = elevation_deg * M_PI / 180.0;
+  const double dir[3] = { std::cos(el) * std::sin(az), std::sin(el), std::cos(el) * std::cos(az) };
+  for (int d = 0; d < 3; ++d) {
+    out->eye[d] = c[d] + dist * dir[d];
+    out->look_at[d] = c[d];
+  }
+  out->up[0] = 0.0;
+  out->up[1] = 1.0;
+  out->up[2] = 0.0;
+  out->fov_y_deg = fov_y_deg;
+  out->near_dist = 0.0;
+  out->far_dist = 0.0;
+  return BEAMS_OK;
+}
+
+int beams_gen_camera_rays(const beams_camera* cam, int32_t width, int32_t height,
+                          float* const origin[3], float* const dir[3], double* const origin64[3],
+                          double* const dir64[3])
+{
+  if (!cam || width < 1 || height < 1 || !origin || !dir) return BEAMS_ERR_INVALID;
+  CameraParams c{};
+  double f[3], len = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    f[d] = cam->look_at[d] - cam->eye[d];
+    len += f[d] * f[d];
+  }
+  len = std::sqrt(len);
+  if (!(len > 0.0)) return BEAMS_ERR_INVALID;
+  for (int d = 0; d < 3; ++d) f[d] /= len;
+  double r[3] = { f[1] * cam->up[2] - f[2] * cam->up[1], f[2] * cam->up[0] - f[0] * cam->up[2],
+                  f[0] * cam->up[1] - f[1] * cam->up[0] };
+  double rl = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  if (!(rl > 0.0)) return BEAMS_ERR_INVALID;
+  for (int d = 0; d < 3; ++d) r[d] /= rl;
+  double u[3] = { r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0] };
+  for (int d = 0; d < 3; ++d) {
+    c.eye[d] = cam->eye[d];
+    c.fwd[d] = f[d];
+    c.right[d] = r[d];
+    c.up[d] = u[d];
+  }
+  c.tan_half_fov = std::tan(cam->fov_y_deg * M_PI / 360.0);
+  c.aspect = (double)width / (double)height;
+  c.width = width;
+  c.height = height;
+  for (int32_t py = 0; py < height; ++py)
+    for (int32_t px = 0; px < width; ++px) {
+      const size_t i = (size_t)py * (size_t)width + (size_t)px;
+      float o[3], dd[3];
+      double o64[3], d64[3];
+      camera_ray(c, px, py, o, dd, o64, d64);
+      for (int d = 0; d < 3; ++d) {
+        origin[d][i] = o[d];
+        dir[d][i] = dd[d];
+        if (origin64 && origin64[d]) origin64[d][i] = o64[d];
+        if (dir64 && dir64[d]) dir64[d][i] = d64[d];
+      }
+    }
+  return BEAMS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// Internal interface between the C-ABI layer (abi.hip) and the HIP kernels.
+// Parameter blocks are plain structs passed by value to the kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace beams {
+
+enum Dtype : int32_t { kF32 = 0, kF16 = 1, kBF16 = 2, kF64 = 3 };
+
+// One block's structured grid: rectilinear coordinates + scalars.
+struct GridParams {
+  const void* field;   // scalars, x fastest
+  int32_t dtype;       // kF32 / kF16 / kBF16
+  int32_t assoc;       // 0 points, 1 cells
+  int64_t n[3];        // point dims
+  int64_t field_stride[3];  // element strides of the stored array (x, y, z)
+  const float* x[3];   // coordinate arrays (device)
+  float minp[3], maxp[3];
+  float gx0[3], ginv[3];    // cell-guess helpers: x[0] and (n-1)/(x[n-1]-x[0])
+  double bounds[6];         // FP64 bounds of the coordinates
+  float ext_mag;            // |extent| in FP32 (BoundsMagnitude)
+};
+
+// Transmittance (opacity) map over the block bounds.
+struct MapParams {
+  void* map;           // (S+1)^3 opacities
+  int32_t dtype;       // kF32 / kF16
+  int32_t light_kind;  // 0 point, 1 directional
+  int64_t mp[3];       // map point dims
+  float morg[3], msp[3], minv[3];
+  float step;          // |size| / 128
+  float light[3];      // position (point) or travel direction (directional)
+  float light_color[3];
+  float dir_far;       // directional-light origin distance
+};
+
+struct ShadeParams {
+  const float* tf;     // ntf x RGBA (device)
+  int32_t ntf;
+  float max_density;   // max alpha of the TF
+  float min_scalar, inv_delta;
+  float sample_dist, mesh_eps, alpha_stop;
+  int32_t use_map;
+  int64_t max_steps;   // bound on samples per camera ray (2 diagonals / step + 64)
+};
+
+// Bound on light-march steps per map vertex: a segment inside the bounds is at most one
+// diagonal, i.e. 128 steps of |size| / 128.
+constexpr int64_t kMapMaxSteps = 1024;
+
+struct RaySoA {
+  const float* org[3];
+  const float* dir[3];
+  float* min_d;
+  float* max_d;
+  float* dist;
+  float* rgba;
+  int64_t n;
+};
+
+struct CameraParams {
+  double eye[3];
+  double fwd[3], right[3], up[3];
+  double tan_half_fov, aspect;
+  int32_t width, height;
+  float near_d, far_d;
+};
+
+struct ImageOut {
+  float* rgba;   // W*H*4
+  float* depth;  // W*H (may be null)
+  float bg[4];   // background blended under the image when use_bg
+  int32_t use_bg;
+};
+
+// ---- launchers (return hipError_t) ------------------------------------------------------
+hipError_t launch_tmap_ref(const GridParams& g, const MapParams& m, const ShadeParams& s,
+                           const float* seed, hipStream_t st);
+hipError_t launch_march_rays(const GridParams& g, const MapParams& m, const ShadeParams& s,
+                             const RaySoA& r, hipStream_t st);
+hipError_t launch_march_image(const GridParams& g, const MapParams& m, const ShadeParams& s,
+                              const CameraParams& c, const ImageOut& o, hipStream_t st);
+
+// field generation on the device
+struct FieldGenParams;
+hipError_t launch_field_minmax(const void* tables, int64_t n0, int64_t n1, int64_t n2,
+                               double* block_minmax, int32_t nblocks, hipStream_t st);
+
+}  // namespace beams

@@ -1,0 +1,203 @@
+// Device functions of the exact (reference-order FP32) path.
+//
+// Each function follows one reference routine operation by operation; the file
+// is compiled with -ffp-contract=off, so products and sums round separately as in
+// the reference's CPU build, and fused multiply-adds appear only where the
+// reference's own code (lcl's lerp) uses them.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace beams {
+
+// ---- storage loads --------------------------------------------------------------------
+__device__ __forceinline__ float load_scalar(const void* p, int32_t dtype, int64_t i)
+{
+  if (dtype == kF16) return f16_to_f32(((const uint16_t*)p)[i]);
+  if (dtype == kBF16) return bf16_to_f32(((const uint16_t*)p)[i]);
+  return ((const float*)p)[i];
+}
+
+template <int DT> __device__ __forceinline__ float load_t(const void* p, int64_t i);
+template <> __device__ __forceinline__ float load_t<kF32>(const void* p, int64_t i)
+{
+  return ((const float*)p)[i];
+}
+template <> __device__ __forceinline__ float load_t<kF16>(const void* p, int64_t i)
+{
+  return f16_to_f32(((const uint16_t*)p)[i]);
+}
+template <> __device__ __forceinline__ float load_t<kBF16>(const void* p, int64_t i)
+{
+  return bf16_to_f32(((const uint16_t*)p)[i]);
+}
+
+template <int DT> __device__ __forceinline__ void store_t(void* p, int64_t i, float v);
+template <> __device__ __forceinline__ void store_t<kF32>(void* p, int64_t i, float v)
+{
+  ((float*)p)[i] = v;
+}
+template <> __device__ __forceinline__ void store_t<kF16>(void* p, int64_t i, float v)
+{
+  ((uint16_t*)p)[i] = f32_to_f16(v);
+}
+
+__device__ __forceinline__ float lerp_lcl(float a, float b, float t)
+{
+  return __builtin_fmaf(t, b, __builtin_fmaf(-t, a, a));
+}
+
+__device__ __forceinline__ float interp_hex(const float* v, const float* pc)
+{
+  float a = lerp_lcl(v[0], v[1], pc[0]);
+  float b = lerp_lcl(v[3], v[2], pc[0]);
+  float c = lerp_lcl(v[4], v[5], pc[0]);
+  float d = lerp_lcl(v[7], v[6], pc[0]);
+  float lo = lerp_lcl(a, b, pc[1]);
+  float hi = lerp_lcl(c, d, pc[1]);
+  return lerp_lcl(lo, hi, pc[2]);
+}
+
+__device__ __forceinline__ float dot3(const float* a, const float* b)
+{
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// Intersections.cxx:8-49 (the slab terms promote to FP64 through vtkm::Bounds).
+__device__ __forceinline__ bool segment_aabb(const float* s, const float* e, const double* bb,
+                                             float teps, float& tmin_o, float& tmax_o)
+{
+  float d[3] = { e[0] - s[0], e[1] - s[1], e[2] - s[2] };
+  float len = __builtin_sqrtf(dot3(d, d));
+  d[0] = d[0] / len;
+  d[1] = d[1] / len;
+  d[2] = d[2] / len;
+  float tmin = 0.0f, tmax = __builtin_inff();
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float t0 = (float)((bb[2 * a] - (double)s[a]) / (double)d[a]);
+    float t1 = (float)((bb[2 * a + 1] - (double)s[a]) / (double)d[a]);
+    tmin = fmaxf(tmin, fminf(t0, t1));
+    tmax = fminf(tmax, fmaxf(t0, t1));
+  }
+  if (tmin < 0.0f) tmin = teps;
+  if (tmax > len) tmax = len;
+  tmin_o = tmin;
+  tmax_o = tmax;
+  return tmin <= tmax;
+}
+
+__device__ __forceinline__ bool inside(const GridParams& g, const float* p)
+{
+  bool in = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (p[d] < g.minp[d] || p[d] > g.maxp[d]) in = false;
+  return in;
+}
+
+// RectilinearMeshOracle.h:154-176: smallest c in [0, C-1] with v <= x[c+1].  The binary
+// search is replaced by a uniform-spacing guess corrected against the coordinates, which
+// yields the same unique cell for monotone coordinates.
+__device__ __forceinline__ int64_t oracle_cell(const float* x, int64_t ncell, float v, float x0,
+                                               float ginv)
+{
+  float gf = (v - x0) * ginv;
+  int64_t c = (gf > 0.0f) ? (int64_t)gf : 0;
+  c = c < ncell - 1 ? c : ncell - 1;
+  while (c > 0 && v <= x[c]) --c;
+  while (c < ncell - 1 && v > x[c + 1]) ++c;
+  return c;
+}
+
+// LightedVolumeRenderer.cxx:123-164: unique c with x[c] <= v < x[c+1] (v != max).
+__device__ __forceinline__ int64_t walk_cell(const float* x, int64_t ncell, float v, float x0,
+                                             float ginv)
+{
+  float gf = (v - x0) * ginv;
+  int64_t c = (gf > 0.0f) ? (int64_t)gf : 0;
+  c = c < ncell - 1 ? c : ncell - 1;
+  while (c > 0 && v < x[c]) --c;
+  while (c < ncell - 1 && v >= x[c + 1]) ++c;
+  return c;
+}
+
+// TransmittanceMap.h:93-112, 166-229: trilinear map opacity at p.
+template <int MDT>
+__device__ __forceinline__ float map_opacity(const MapParams& m, const float* p)
+{
+  float pc[3];
+  int64_t cell[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float t = (p[d] - m.morg[d]) * m.minv[d];
+    if (t < 0.0f) t = 0.0f;
+    if (t >= (float)(m.mp[d] - 1)) t = (float)(m.mp[d] - 2);
+    cell[d] = (int64_t)t;
+    float lo = m.morg[d] + m.msp[d] * (float)cell[d];
+    float hi = m.morg[d] + m.msp[d] * (float)(cell[d] + 1);
+    pc[d] = (p[d] - lo) / (hi - lo);
+  }
+  const int64_t sx = m.mp[0], sxy = m.mp[0] * m.mp[1];
+  const int64_t i0 = (cell[2] * m.mp[1] + cell[1]) * sx + cell[0];
+  float v[8];
+  v[0] = load_t<MDT>(m.map, i0);
+  v[1] = load_t<MDT>(m.map, i0 + 1);
+  v[2] = load_t<MDT>(m.map, i0 + 1 + sx);
+  v[3] = load_t<MDT>(m.map, i0 + sx);
+  v[4] = load_t<MDT>(m.map, i0 + sxy);
+  v[5] = load_t<MDT>(m.map, i0 + sxy + 1);
+  v[6] = load_t<MDT>(m.map, i0 + sxy + 1 + sx);
+  v[7] = load_t<MDT>(m.map, i0 + sxy + sx);
+  return interp_hex(v, pc);
+}
+
+// ArrayPortalUniformPointCoordinates::Get on the map grid.
+__device__ __forceinline__ void map_vertex(const MapParams& m, int64_t id, float* v)
+{
+  const int64_t i = id % m.mp[0];
+  const int64_t j = (id / m.mp[0]) % m.mp[1];
+  const int64_t k = id / (m.mp[0] * m.mp[1]);
+  v[0] = m.morg[0] + m.msp[0] * (float)i;
+  v[1] = m.morg[1] + m.msp[1] * (float)j;
+  v[2] = m.morg[2] + m.msp[2] * (float)k;
+}
+
+__device__ __forceinline__ void light_origin(const MapParams& m, const float* v, float* o)
+{
+  if (m.light_kind == 1) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) o[d] = v[d] - m.light[d] * m.dir_far;
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) o[d] = m.light[d];
+  }
+}
+
+// LightedVolumeRenderer.cxx:666-717
+__device__ __forceinline__ float rcp_safe(float f)
+{
+  return 1.0f / ((__builtin_fabsf(f) < 1e-8f) ? 1e-8f : f);
+}
+
+__device__ __forceinline__ void calc_ray_start(const GridParams& g, const float* org,
+                                               const float* dir, float& minD, float& dist,
+                                               float& maxD)
+{
+  const float xminb = (float)g.bounds[0], xmaxb = (float)g.bounds[1];
+  const float yminb = (float)g.bounds[2], ymaxb = (float)g.bounds[3];
+  const float zminb = (float)g.bounds[4], zmaxb = (float)g.bounds[5];
+  const float ix = rcp_safe(dir[0]), iy = rcp_safe(dir[1]), iz = rcp_safe(dir[2]);
+  const float ox = org[0] * ix, oy = org[1] * iy, oz = org[2] * iz;
+  const float xmin = xminb * ix - ox, ymin = yminb * iy - oy, zmin = zminb * iz - oz;
+  const float xmax = xmaxb * ix - ox, ymax = ymaxb * iy - oy, zmax = zmaxb * iz - oz;
+  minD = fmaxf(fmaxf(fmaxf(fminf(ymin, ymax), fminf(xmin, xmax)), fminf(zmin, zmax)), minD);
+  const float ex = fminf(fminf(fmaxf(ymin, ymax), fmaxf(xmin, xmax)), fmaxf(zmin, zmax));
+  maxD = fminf(maxD, ex);
+  if (maxD < minD)
+    minD = -1.0f;
+  else
+    dist = minD;
+}
+
+}  // namespace beams
