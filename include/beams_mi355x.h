@@ -150,6 +150,19 @@ int beams_get_map(beams_renderer* h, float* out, int32_t mem);
  * replacing the Phase1..5Time globals (Beams.cxx:30-34). */
 int beams_get_timings(beams_renderer* h, float* phase_ms, int32_t n);
 
+/* Mean per-phase device times over the frames rendered since the last reset (at most
+ * the last 256 frames), and their number; the Profiler's per-frame collection
+ * (Profiler.cxx:24-74) in one call.  beams_reset_timings starts a new window. */
+int beams_get_timing_stats(beams_renderer* h, float* avg_ms, int32_t n, int32_t* nframes);
+int beams_reset_timings(beams_renderer* h);
+
+/* Optional device counters of the next renders: composited camera-ray samples (the
+ * iterations of the Sampler loop that composite, LightedVolumeRenderer.cxx:404-493)
+ * and light-march steps inside the volume (TransmittanceMap.h:299-331) of the last
+ * frame.  Extension; counting adds one atomic per ray. */
+int beams_set_sample_counting(beams_renderer* h, int32_t enabled);
+int beams_get_sample_counts(beams_renderer* h, int64_t* camera_samples, int64_t* light_steps);
+
 /* ---- DataSetFactory / SceneFactory inputs (DataSetFactory.h:15, SceneFactory.h:13) -------
  * Seeded synthetic scenes for the benchmark configurations. */
 typedef struct beams_field_recipe {

@@ -52,6 +52,7 @@ typedef struct OrcRays {
   void* max_distance; /* in/out */
   void* distance;     /* out    */
   void* rgba;         /* in/out, 4 interleaved values per ray */
+  void* nsamples;     /* optional out: int64 composited samples per ray */
 } OrcRays;
 
 #define ORC_DECLARE(SUF)                                                                   \
@@ -60,6 +61,8 @@ typedef struct OrcRays {
   int64_t orc_find_cell_##SUF(const OrcScene* s, const void* point, void* pcoords);        \
   double orc_interp_hex_##SUF(const void* values8, const void* pcoords);                   \
   int orc_tmap_build_##SUF(const OrcScene* s, const void* seed, void* map_out, int nthr);  \
+  int orc_tmap_build_range_##SUF(const OrcScene* s, void* map_out, int64_t begin,          \
+                                 int64_t end, int nthr);                                   \
   int orc_map_estimate_##SUF(const OrcScene* s, const void* map, const void* point,        \
                              void* rgb_out, void* opacity_out);                            \
   int orc_calc_ray_start_##SUF(const OrcScene* s, OrcRays* rays, int nthr);                \

@@ -43,6 +43,7 @@ class OrcRays(C.Structure):
         ("max_distance", C.c_void_p),
         ("distance", C.c_void_p),
         ("rgba", C.c_void_p),
+        ("nsamples", C.c_void_p),
     ]
 
 
@@ -71,6 +72,7 @@ def lib():
                 "orc_find_cell": [VP, VP, VP],
                 "orc_interp_hex": [VP, VP],
                 "orc_tmap_build": [VP, VP, VP, I32],
+                "orc_tmap_build_range": [VP, VP, I64, I64, I32],
                 "orc_map_estimate": [VP, VP, VP, VP, VP],
                 "orc_calc_ray_start": [VP, VP, I32],
                 "orc_sample_rays": [VP, VP, VP, I32],
@@ -151,6 +153,7 @@ class RayBuf:
         self.max_d = np.full(n, np.inf, dt) if max_d is None else np.array(max_d, dt)
         self.dist = np.zeros(n, dt)
         self.rgba = np.zeros((n, 4), dt) if rgba is None else np.array(rgba, dt).reshape(n, 4)
+        self.nsamples = np.zeros(n, np.int64)
         r = OrcRays()
         r.n = n
         r.origin = (C.c_void_p * 3)(*[self.origin[i].ctypes.data for i in range(3)])
@@ -159,6 +162,7 @@ class RayBuf:
         r.max_distance = self.max_d.ctypes.data
         r.distance = self.dist.ctypes.data
         r.rgba = self.rgba.ctypes.data
+        r.nsamples = self.nsamples.ctypes.data
         self.r = r
 
 
@@ -167,6 +171,13 @@ def tmap_build(vol, seed=None, nthreads=8):
     sd = None if seed is None else np.ascontiguousarray(seed, _dt(vol.twin))
     getattr(lib(), f"orc_tmap_build_{vol.twin}")(C.byref(vol.s), None if sd is None else sd.ctypes.data,
                                                  out.ctypes.data, nthreads)
+    return out
+
+
+def tmap_build_range(vol, out, begin, end, nthreads=8):
+    """Map vertices [begin, end) into out (a full-size map array)."""
+    getattr(lib(), f"orc_tmap_build_range_{vol.twin}")(C.byref(vol.s), out.ctypes.data, int(begin),
+                                                       int(end), nthreads)
     return out
 
 

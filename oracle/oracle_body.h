@@ -301,6 +301,22 @@ int FN(orc_tmap_build)(const OrcScene* s, const void* seed, void* map_out, int n
   return 0;
 }
 
+/* The map vertices [begin, end) only (a slab of the map, for timing samples). */
+int FN(orc_tmap_build_range)(const OrcScene* s, void* map_out, int64_t begin, int64_t end,
+                             int nthr)
+{
+  FN(Geo) g;
+  FN(geo_init)(&g, s);
+  R* out = (R*)map_out;
+  const R* field = (const R*)s->field;
+  const int64_t nf = FN(nfield)(s);
+  (void)nthr;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthr > 0 ? nthr : 1)
+  for (int64_t id = begin; id < end; ++id)
+    out[id] = FN(march_vertex)(&g, field, nf, s->light_kind, id, (R)0);
+  return 0;
+}
+
 /* ---- TransmittanceMap.h:93-112,166-229 ------------------------------------------------ */
 static R FN(map_opacity)(const FN(Geo) * g, const R* map, const R* p)
 {
@@ -418,11 +434,12 @@ static inline void FN(locate_cell)(const FN(Geo) * g, int64_t* cell, const R* p,
 }
 
 /* ---- LightedVolumeRenderer.cxx:345-498 (Sampler, point field) -------------------------- */
-static void FN(sample_point)(const FN(Geo) * g, const R* field, const R* map, int use_map,
+static int64_t FN(sample_point)(const FN(Geo) * g, const R* field, const R* map, int use_map,
                              R sampleDist, R meshEps, R alphaStop, const R* org, const R* dir,
                              R minD, R maxD, R* color)
 {
-  if (minD == (R)-1) return;
+  if (minD == (R)-1) return 0;
+  int64_t composited = 0;
   R dist = minD + meshEps;
   R p[3];
   /* step bound (never reached by a finite ray): 2 diagonals / step + 64 */
@@ -494,6 +511,7 @@ static void FN(sample_point)(const FN(Geo) * g, const R* field, const R* map, in
       color[1] = color[1] + sc[1] * sc[3];
       color[2] = color[2] + sc[2] * sc[3];
       color[3] = sc[3] + color[3];
+      ++composited;
     }
     dist += sampleDist;
     for (int d = 0; d < 3; ++d) p[d] = p[d] + sampleDist * dir[d];
@@ -502,13 +520,15 @@ static void FN(sample_point)(const FN(Geo) * g, const R* field, const R* map, in
     tz = (p[2] - bl[2]) * invsp[2];
     if (color[3] >= alphaStop) break;
   }
+  return composited;
 }
 
 /* ---- LightedVolumeRenderer.cxx:540-641 (SamplerCellAssoc) ------------------------------ */
-static void FN(sample_cell)(const FN(Geo) * g, const R* field, R sampleDist, R meshEps,
+static int64_t FN(sample_cell)(const FN(Geo) * g, const R* field, R sampleDist, R meshEps,
                             R alphaStop, const R* org, const R* dir, R minD, R maxD, R* color)
 {
-  if (minD == (R)-1) return;
+  if (minD == (R)-1) return 0;
+  int64_t composited = 0;
   R dist = minD + meshEps;
   R p[3];
   /* step bound (never reached by a finite ray): 2 diagonals / step + 64 */
@@ -552,6 +572,7 @@ static void FN(sample_cell)(const FN(Geo) * g, const R* field, R sampleDist, R m
     color[1] = color[1] + scol[1] * a;
     color[2] = color[2] + scol[2] * a;
     color[3] = a + color[3];
+    ++composited;
     dist += sampleDist;
     for (int d = 0; d < 3; ++d) p[d] = p[d] + sampleDist * dir[d];
     if (color[3] >= alphaStop) break;
@@ -560,6 +581,7 @@ static void FN(sample_cell)(const FN(Geo) * g, const R* field, R sampleDist, R m
     tz = (p[2] - bl[2]) * invsp[2];
   }
   for (int c = 0; c < 4; ++c) color[c] = FN(vmin)(color[c], (R)1);
+  return composited;
 }
 
 static R FN(sample_distance)(const FN(Geo) * g, const OrcScene* s)
@@ -590,10 +612,12 @@ int FN(orc_sample_rays)(const OrcScene* s, const void* map, OrcRays* rays, int n
       d[c] = ((const R*)rays->dir[c])[r];
     }
     R* col = rgba + 4 * r;
+    int64_t ns;
     if (s->assoc == 1)
-      FN(sample_cell)(&g, field, sd, eps, stop, o, d, mn[r], mx[r], col);
+      ns = FN(sample_cell)(&g, field, sd, eps, stop, o, d, mn[r], mx[r], col);
     else
-      FN(sample_point)(&g, field, (const R*)map, use_map, sd, eps, stop, o, d, mn[r], mx[r], col);
+      ns = FN(sample_point)(&g, field, (const R*)map, use_map, sd, eps, stop, o, d, mn[r], mx[r], col);
+    if (rays->nsamples) ((int64_t*)rays->nsamples)[r] = ns;
   }
   return 0;
 }

@@ -84,6 +84,10 @@ SIGNATURES = {
     "beams_render_image": (C.c_int, [_P, C.POINTER(Camera), _I32, _I32, _PF, _P, _P, _I32]),
     "beams_get_map": (C.c_int, [_P, _P, _I32]),
     "beams_get_timings": (C.c_int, [_P, _PF, _I32]),
+    "beams_get_timing_stats": (C.c_int, [_P, _PF, _I32, C.POINTER(C.c_int32)]),
+    "beams_reset_timings": (C.c_int, [_P]),
+    "beams_set_sample_counting": (C.c_int, [_P, _I32]),
+    "beams_get_sample_counts": (C.c_int, [_P, _PI64, _PI64]),
     "beams_gen_field": (C.c_int, [C.POINTER(FieldRecipe), _PI64, _I32, _P, _I32]),
     "beams_gen_colormap": (C.c_int, [C.c_uint64, _I32, _I64, _PF, _PD]),
     "beams_gen_orbit_camera": (C.c_int, [_PD, _D, _D, _D, C.POINTER(Camera)]),
@@ -294,13 +298,31 @@ class LightedVolumeRenderer:
         self._c(lib().beams_get_timings(self._h, _f32p(t), len(PHASES)))
         return dict(zip(PHASES, t.tolist()))
 
+    def GetTimingStats(self):
+        t = np.zeros(len(PHASES), np.float32)
+        n = C.c_int32()
+        self._c(lib().beams_get_timing_stats(self._h, _f32p(t), len(PHASES), C.byref(n)))
+        return dict(zip(PHASES, t.tolist())), n.value
+
+    def ResetTimings(self):
+        self._c(lib().beams_reset_timings(self._h))
+
+    def SetSampleCounting(self, enabled):
+        self._c(lib().beams_set_sample_counting(self._h, int(bool(enabled))))
+
+    def GetSampleCounts(self):
+        a = C.c_int64()
+        b = C.c_int64()
+        self._c(lib().beams_get_sample_counts(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def Synchronize(self):
         self._c(lib().beams_synchronize(self._h))
 
 
 # ---- scene inputs (DataSetFactory / SceneFactory side) -----------------------------------------
 
-def gen_field(dims, seed, nblobs=16, noise_octaves=0, noise_amp=0.0, noise_freq=4.0,
+def gen_field(dims, seed=1, nblobs=16, noise_octaves=0, noise_amp=0.0, noise_freq=4.0,
               time=0.0, drift_speed=0.0, noise_phase_rate=0.0, dtype=F32, out=None):
     """Blob + value-noise field normalised to [0,1].  out may be a device tensor."""
     r = FieldRecipe(seed, nblobs, noise_octaves, noise_amp, noise_freq, time, drift_speed,

@@ -83,8 +83,19 @@ struct beams_renderer {
   float sample_distance = -1.0f;
   float alpha_stop = 1.0f;
   DevBuf rays, image;
-  hipEvent_t ev[BEAMS_NUM_PHASES + 1] = {};
-  bool timed[BEAMS_NUM_PHASES] = {};
+  // Per-frame phase timing: events [2p] / [2p+1] bracket phase p.  A ring of frames
+  // holds the frames rendered since the last reset.
+  struct Frame {
+    hipEvent_t ev[2 * BEAMS_NUM_PHASES] = {};
+    bool timed[BEAMS_NUM_PHASES] = {};
+  };
+  static constexpr int kMaxFrames = 256;
+  std::vector<Frame> frames;
+  int64_t total_frames = 0;  // since the last reset
+  int cur = -1;
+  // optional sample counters: [0] camera-ray samples, [1] light-march steps
+  bool counting = false;
+  DevBuf counters;
 
   int fail(int code, const std::string& msg)
   {
@@ -184,6 +195,7 @@ ShadeParams shade_params(beams_renderer* h, const GridParams& g)
   s.alpha_stop = h->alpha_stop;
   s.use_map = h->use_shadow ? 1 : 0;
   s.max_steps = (int64_t)(2.0f * g.ext_mag / s.sample_dist) + 64;
+  s.counters = h->counting ? (unsigned long long*)h->counters.p : nullptr;
   return s;
 }
 
@@ -203,9 +215,28 @@ int check_ready(beams_renderer* h)
   return BEAMS_OK;
 }
 
-int record(beams_renderer* h, int slot)
+int frame_begin(beams_renderer* h)
 {
-  HIP_TRY(h, hipEventRecord(h->ev[slot], h->stream));
+  const int slot = (int)(h->total_frames % beams_renderer::kMaxFrames);
+  if ((int)h->frames.size() <= slot) {
+    h->frames.emplace_back();
+    for (auto& e : h->frames.back().ev) HIP_TRY(h, hipEventCreate(&e));
+  }
+  h->cur = slot;
+  ++h->total_frames;
+  for (bool& t : h->frames[slot].timed) t = false;
+  if (h->counting) {
+    HIP_TRY(h, h->counters.ensure(2 * sizeof(unsigned long long)));
+    HIP_TRY(h, launch_zero_u64((unsigned long long*)h->counters.p, 2, h->stream));
+  }
+  return BEAMS_OK;
+}
+
+int phase_mark(beams_renderer* h, int phase, bool end)
+{
+  auto& f = h->frames[h->cur];
+  HIP_TRY(h, hipEventRecord(f.ev[2 * phase + (end ? 1 : 0)], h->stream));
+  if (end) f.timed[phase] = true;
   return BEAMS_OK;
 }
 
@@ -242,11 +273,6 @@ int beams_create(beams_renderer** out)
     delete h;
     return BEAMS_ERR_HIP;
   }
-  for (auto& e : h->ev)
-    if (hipEventCreate(&e) != hipSuccess) {
-      delete h;
-      return BEAMS_ERR_HIP;
-    }
   *out = h;
   return BEAMS_OK;
 }
@@ -261,8 +287,10 @@ int beams_destroy(beams_renderer* h)
   h->map.release();
   h->rays.release();
   h->image.release();
-  for (auto& e : h->ev)
-    if (e) (void)hipEventDestroy(e);
+  for (auto& f : h->frames)
+    for (auto& e : f.ev)
+      if (e) (void)hipEventDestroy(e);
+  h->counters.release();
   (void)hipStreamDestroy(h->stream);
   delete h;
   return BEAMS_OK;
@@ -426,15 +454,14 @@ int beams_render_rays(beams_renderer* h, int64_t n, const float* const origin[3]
   if (mem != BEAMS_MEM_HOST && mem != BEAMS_MEM_DEVICE) return h->fail(BEAMS_ERR_INVALID, "bad mem kind");
   int rc = check_ready(h);
   if (rc) return rc;
+  if ((rc = frame_begin(h))) return rc;
   GridParams g = grid_params(h);
   ShadeParams s = shade_params(h, g);
-  for (bool& t : h->timed) t = false;
-  if ((rc = record(h, 0))) return rc;
   if (h->use_shadow && h->assoc == 0) {
+    if ((rc = phase_mark(h, BEAMS_PHASE_MAP, false))) return rc;
     if ((rc = build_map(h, g, s))) return rc;
-    h->timed[BEAMS_PHASE_MAP] = true;
+    if ((rc = phase_mark(h, BEAMS_PHASE_MAP, true))) return rc;
   }
-  if ((rc = record(h, 1))) return rc;
   RaySoA r{};
   r.n = n;
   if (mem == BEAMS_MEM_DEVICE) {
@@ -468,11 +495,10 @@ int beams_render_rays(beams_renderer* h, int64_t n, const float* const origin[3]
     r.dist = arr[8];
     r.rgba = drgba;
   }
-  if ((rc = record(h, 2))) return rc;
+  if ((rc = phase_mark(h, BEAMS_PHASE_MARCH, false))) return rc;
   MapParams m = map_params(h, g);
   HIP_TRY(h, launch_march_rays(g, m, s, r, h->stream));
-  if ((rc = record(h, 3))) return rc;
-  h->timed[BEAMS_PHASE_MARCH] = true;
+  if ((rc = phase_mark(h, BEAMS_PHASE_MARCH, true))) return rc;
   if (mem == BEAMS_MEM_HOST) {
     const size_t fb = sizeof(float) * (size_t)n;
     HIP_TRY(h, hipMemcpyAsync(min_d, r.min_d, fb, hipMemcpyDeviceToHost, h->stream));
@@ -523,16 +549,14 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
     c.near_d = (float)cam->near_dist;
     c.far_d = cam->far_dist > 0.0 ? (float)cam->far_dist : INFINITY;
   }
+  if ((rc = frame_begin(h))) return rc;
   GridParams g = grid_params(h);
   ShadeParams s = shade_params(h, g);
-  for (bool& t : h->timed) t = false;
-  if ((rc = record(h, 0))) return rc;
   if (h->use_shadow && h->assoc == 0) {
+    if ((rc = phase_mark(h, BEAMS_PHASE_MAP, false))) return rc;
     if ((rc = build_map(h, g, s))) return rc;
-    h->timed[BEAMS_PHASE_MAP] = true;
+    if ((rc = phase_mark(h, BEAMS_PHASE_MAP, true))) return rc;
   }
-  if ((rc = record(h, 1))) return rc;
-  if ((rc = record(h, 2))) return rc;
   const size_t npix = (size_t)width * (size_t)height;
   ImageOut o{};
   if (mem == BEAMS_MEM_DEVICE) {
@@ -546,9 +570,9 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
   o.use_bg = background ? 1 : 0;
   for (int k = 0; k < 4; ++k) o.bg[k] = background ? background[k] : 0.0f;
   MapParams m = map_params(h, g);
+  if ((rc = phase_mark(h, BEAMS_PHASE_MARCH, false))) return rc;
   HIP_TRY(h, launch_march_image(g, m, s, c, o, h->stream));
-  if ((rc = record(h, 3))) return rc;
-  h->timed[BEAMS_PHASE_MARCH] = true;
+  if ((rc = phase_mark(h, BEAMS_PHASE_MARCH, true))) return rc;
   if (mem == BEAMS_MEM_HOST) {
     HIP_TRY(h, hipMemcpyAsync(rgba, o.rgba, sizeof(float) * 4 * npix, hipMemcpyDeviceToHost, h->stream));
     if (depth) HIP_TRY(h, hipMemcpyAsync(depth, o.depth, sizeof(float) * npix, hipMemcpyDeviceToHost, h->stream));
@@ -581,11 +605,66 @@ int beams_get_timings(beams_renderer* h, float* phase_ms, int32_t n)
 {
   CHECK_HANDLE(h);
   if (!phase_ms || n < 1) return h->fail(BEAMS_ERR_INVALID, "null timings output");
+  if (h->cur < 0) return h->fail(BEAMS_ERR_STATE, "nothing rendered yet");
   HIP_TRY(h, hipStreamSynchronize(h->stream));
-  float t[BEAMS_NUM_PHASES] = { 0, 0, 0, 0, 0 };
-  if (h->timed[BEAMS_PHASE_MAP]) HIP_TRY(h, hipEventElapsedTime(&t[BEAMS_PHASE_MAP], h->ev[0], h->ev[1]));
-  if (h->timed[BEAMS_PHASE_MARCH]) HIP_TRY(h, hipEventElapsedTime(&t[BEAMS_PHASE_MARCH], h->ev[2], h->ev[3]));
-  for (int i = 0; i < n && i < BEAMS_NUM_PHASES; ++i) phase_ms[i] = t[i];
+  const auto& f = h->frames[h->cur];
+  for (int p = 0; p < n && p < BEAMS_NUM_PHASES; ++p) {
+    float t = 0.0f;
+    if (f.timed[p]) HIP_TRY(h, hipEventElapsedTime(&t, f.ev[2 * p], f.ev[2 * p + 1]));
+    phase_ms[p] = t;
+  }
+  return BEAMS_OK;
+}
+
+int beams_get_timing_stats(beams_renderer* h, float* avg_ms, int32_t n, int32_t* nframes)
+{
+  CHECK_HANDLE(h);
+  if (!avg_ms || n < 1) return h->fail(BEAMS_ERR_INVALID, "null timings output");
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  const int64_t used = std::min<int64_t>(h->total_frames, beams_renderer::kMaxFrames);
+  for (int p = 0; p < n && p < BEAMS_NUM_PHASES; ++p) {
+    double sum = 0.0;
+    int cnt = 0;
+    for (int64_t i = 0; i < used; ++i) {
+      const auto& f = h->frames[(size_t)i];
+      if (!f.timed[p]) continue;
+      float t = 0.0f;
+      HIP_TRY(h, hipEventElapsedTime(&t, f.ev[2 * p], f.ev[2 * p + 1]));
+      sum += t;
+      ++cnt;
+    }
+    avg_ms[p] = cnt ? (float)(sum / cnt) : 0.0f;
+  }
+  if (nframes) *nframes = (int32_t)used;
+  return BEAMS_OK;
+}
+
+int beams_reset_timings(beams_renderer* h)
+{
+  CHECK_HANDLE(h);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->total_frames = 0;
+  h->cur = -1;
+  return BEAMS_OK;
+}
+
+int beams_set_sample_counting(beams_renderer* h, int32_t enabled)
+{
+  CHECK_HANDLE(h);
+  h->counting = enabled != 0;
+  if (h->counting) HIP_TRY(h, h->counters.ensure(2 * sizeof(unsigned long long)));
+  return BEAMS_OK;
+}
+
+int beams_get_sample_counts(beams_renderer* h, int64_t* camera_samples, int64_t* light_steps)
+{
+  CHECK_HANDLE(h);
+  if (!h->counting) return h->fail(BEAMS_ERR_STATE, "sample counting is off");
+  unsigned long long c[2] = { 0, 0 };
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  HIP_TRY(h, hipMemcpy(c, h->counters.p, sizeof(c), hipMemcpyDeviceToHost));
+  if (camera_samples) *camera_samples = (int64_t)c[0];
+  if (light_steps) *light_steps = (int64_t)c[1];
   return BEAMS_OK;
 }
 

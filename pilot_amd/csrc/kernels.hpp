@@ -43,6 +43,7 @@ struct ShadeParams {
   float sample_dist, mesh_eps, alpha_stop;
   int32_t use_map;
   int64_t max_steps;   // bound on samples per camera ray (2 diagonals / step + 64)
+  unsigned long long* counters;  // optional: [0] composited camera-ray samples, [1] light steps
 };
 
 // Bound on light-march steps per map vertex: a segment inside the bounds is at most one
@@ -81,6 +82,8 @@ hipError_t launch_march_rays(const GridParams& g, const MapParams& m, const Shad
                              const RaySoA& r, hipStream_t st);
 hipError_t launch_march_image(const GridParams& g, const MapParams& m, const ShadeParams& s,
                               const CameraParams& c, const ImageOut& o, hipStream_t st);
+
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
 
 // field generation on the device
 struct FieldGenParams;

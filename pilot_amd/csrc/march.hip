@@ -63,6 +63,7 @@ __global__ void __launch_bounds__(kBlock) k_tmap_ref(GridParams g, MapParams m, 
   const int64_t c0 = g.n[0] - 1, c1 = g.n[1] - 1, c2 = g.n[2] - 1;
   const int64_t sx = g.field_stride[0], sy = g.field_stride[1], sz = g.field_stride[2];
   float op = seed ? seed[id] : 0.0f;
+  int64_t taken = 0;
   for (int64_t it = 0; it < kMapMaxSteps; ++it) {
     float df[3] = { p[0] - start[0], p[1] - start[1], p[2] - start[2] };
     if (!(dot3(df, df) <= len2)) break;
@@ -70,6 +71,7 @@ __global__ void __launch_bounds__(kBlock) k_tmap_ref(GridParams g, MapParams m, 
     p[1] = p[1] + m.step * dir[1];
     p[2] = p[2] + m.step * dir[2];
     if (!inside(g, p)) break;
+    ++taken;
     const int64_t ci = oracle_cell(x[0], c0, p[0], g.gx0[0], g.ginv[0]);
     const int64_t cj = oracle_cell(x[1], c1, p[1], g.gx0[1], g.ginv[1]);
     const int64_t ck = oracle_cell(x[2], c2, p[2], g.gx0[2], g.ginv[2]);
@@ -96,17 +98,19 @@ __global__ void __launch_bounds__(kBlock) k_tmap_ref(GridParams g, MapParams m, 
     if (op == 1.0f) break;  // every later step leaves an opacity of exactly 1 unchanged
   }
   store_t<MDT>(m.map, id, op);
+  if (s.counters && taken) atomicAdd(&s.counters[1], (unsigned long long)taken);
 }
 
 // ---- LightedVolumeRenderer.cxx:345-498 ----------------------------------------------------
 template <int FDT, int MDT, bool USE_MAP>
-__device__ __forceinline__ void sample_point(const GridParams& g, const MapParams& m,
+__device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapParams& m,
                                              const ShadeParams& s, const float4* tf,
                                              const float* const* x, const float* org,
                                              const float* dir, float minD, float maxD,
                                              float* color)
 {
-  if (minD == -1.0f) return;
+  if (minD == -1.0f) return 0;
+  int64_t composited = 0;
   float dist = minD + s.mesh_eps;
   float p[3];
   int64_t steps = 0;  // bounded by s.max_steps, which no finite in-box ray reaches
@@ -190,6 +194,7 @@ __device__ __forceinline__ void sample_point(const GridParams& g, const MapParam
       color[1] = color[1] + c.y * c.w;
       color[2] = color[2] + c.z * c.w;
       color[3] = c.w + color[3];
+      ++composited;
     }
     dist += sd;
     p[0] = p[0] + sd * dir[0];
@@ -200,16 +205,18 @@ __device__ __forceinline__ void sample_point(const GridParams& g, const MapParam
     tz = (p[2] - bl[2]) * invsp[2];
     if (color[3] >= s.alpha_stop) break;
   }
+  return composited;
 }
 
 // ---- LightedVolumeRenderer.cxx:540-641 (cell-associated scalars, no shadow term) --------
 template <int FDT>
-__device__ __forceinline__ void sample_cell(const GridParams& g, const ShadeParams& s,
+__device__ __forceinline__ int64_t sample_cell(const GridParams& g, const ShadeParams& s,
                                             const float4* tf, const float* const* x,
                                             const float* org, const float* dir, float minD,
                                             float maxD, float* color)
 {
-  if (minD == -1.0f) return;
+  if (minD == -1.0f) return 0;
+  int64_t composited = 0;
   float dist = minD + s.mesh_eps;
   float p[3];
   int64_t steps = 0;
@@ -264,6 +271,7 @@ __device__ __forceinline__ void sample_cell(const GridParams& g, const ShadePara
     color[1] = color[1] + sc.y * a;
     color[2] = color[2] + sc.z * a;
     color[3] = a + color[3];
+    ++composited;
     dist += sd;
     p[0] = p[0] + sd * dir[0];
     p[1] = p[1] + sd * dir[1];
@@ -277,6 +285,7 @@ __device__ __forceinline__ void sample_cell(const GridParams& g, const ShadePara
   color[1] = fminf(color[1], 1.0f);
   color[2] = fminf(color[2], 1.0f);
   color[3] = fminf(color[3], 1.0f);
+  return composited;
 }
 
 __device__ __forceinline__ void stage_tf(const ShadeParams& s, float4* tf)
@@ -291,10 +300,10 @@ __device__ __forceinline__ void shade(const GridParams& g, const MapParams& m,
                                       const float* const* x, const float* org, const float* dir,
                                       float minD, float maxD, float* color)
 {
-  if (g.assoc == 1)
-    sample_cell<FDT>(g, s, tf, x, org, dir, minD, maxD, color);
-  else
-    sample_point<FDT, MDT, USE_MAP>(g, m, s, tf, x, org, dir, minD, maxD, color);
+  const int64_t n = (g.assoc == 1)
+                      ? sample_cell<FDT>(g, s, tf, x, org, dir, minD, maxD, color)
+                      : sample_point<FDT, MDT, USE_MAP>(g, m, s, tf, x, org, dir, minD, maxD, color);
+  if (s.counters && n) atomicAdd(&s.counters[0], (unsigned long long)n);
 }
 
 // Camera rays given as the reference's Ray<Float32> SoA.
@@ -364,6 +373,11 @@ __global__ void __launch_bounds__(kBlock) k_march_image(GridParams g, MapParams 
   if (out.depth) out.depth[pix] = (minD == -1.0f) ? __builtin_inff() : minD;
 }
 
+__global__ void k_zero_u64(unsigned long long* p, int n)
+{
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 size_t lds_bytes_coords(const GridParams& g) { return sizeof(float) * (g.n[0] + g.n[1] + g.n[2]); }
 
 }  // namespace
@@ -383,6 +397,12 @@ size_t lds_bytes_coords(const GridParams& g) { return sizeof(float) * (g.n[0] + 
     case kF16: { constexpr int MDT = kF16; BODY; } break;   \
     default: return hipErrorInvalidValue;            \
   }
+
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_tmap_ref(const GridParams& g, const MapParams& m, const ShadeParams& s,
                            const float* seed, hipStream_t st)

@@ -82,13 +82,23 @@ def _light_outside(rng):
 
 def make_scene(name, n, seed, nblobs, noise_octaves=0, noise_amp=0.0, light="point",
                map_cells=None, width=512, height=512, field_dtype=B.F32, map_dtype=B.F32,
-               alpha_stop=1.0, time=0.0, drift_speed=0.0, noise_phase_rate=0.0, fov=60.0):
+               alpha_stop=1.0, time=0.0, drift_speed=0.0, noise_phase_rate=0.0, fov=60.0,
+               device=False):
+    """device=True generates the field straight into HBM (a devmem.DeviceArray) and keeps
+    no host copy (field32/field64 are None)."""
     dims = (n, n, n)
     rng = np.random.default_rng(seed)
-    f64 = B.gen_field(dims, seed, nblobs, noise_octaves, noise_amp, 4.0, time, drift_speed,
-                      noise_phase_rate, dtype=B.F64)
-    store = B.gen_field(dims, seed, nblobs, noise_octaves, noise_amp, 4.0, time, drift_speed,
-                        noise_phase_rate, dtype=field_dtype)
+    gen = dict(seed=seed, nblobs=nblobs, noise_octaves=noise_octaves, noise_amp=noise_amp,
+               noise_freq=4.0, time=time, drift_speed=drift_speed, noise_phase_rate=noise_phase_rate)
+    if device:
+        from .devmem import DeviceArray
+
+        npdt = {B.F32: np.float32, B.F16: np.float16, B.BF16: np.uint16}[field_dtype]
+        store = B.gen_field(dims, dtype=field_dtype, out=DeviceArray(n * n * n, npdt), **gen)
+        f64 = None
+    else:
+        f64 = B.gen_field(dims, dtype=B.F64, **gen)
+        store = B.gen_field(dims, dtype=field_dtype, **gen)
     tf32, tf64 = B.gen_colormap(seed + 1, 6, 256)
     c32, c64 = unit_coords(n)
     if light == "directional":
@@ -103,7 +113,8 @@ def make_scene(name, n, seed, nblobs, noise_octaves=0, noise_amp=0.0, light="poi
     cam = B.orbit_camera(bounds, az, el, fov)
     mc = tuple(map_cells) if map_cells is not None else (n - 1, n - 1, n - 1)
     return Scene(name=name, dims=dims, coords=[c32, c32, c32], coords64=[c64, c64, c64],
-                 field_store=store, field_dtype=field_dtype, field32=_decode(store, field_dtype),
+                 field_store=store, field_dtype=field_dtype,
+                 field32=None if device else _decode(store, field_dtype),
                  field64=f64, tf32=tf32, tf64=tf64, scalar_range=(0.0, 1.0), light_kind=kind,
                  light_pos=pos, light_color=np.float32([2.0, 2.0, 2.0]), map_cells=mc,
                  sample_distance=float(np.float32(0.5 / (n - 1))), alpha_stop=alpha_stop,

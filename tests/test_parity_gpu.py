@@ -68,6 +68,24 @@ def test_plain_small_full_image(oracle_mod, light):
     assert rays.rgba[:, 3].max() > 0.5
 
 
+def test_sample_counters_match_oracle(oracle_mod):
+    """The GPU's composited-sample counter equals the oracle's per-ray sample counts."""
+    sc = scenes.make_scene("count", 40, seed=17, nblobs=12, width=128, height=96)
+    o, d = B.camera_rays(sc.camera, sc.width, sc.height)
+    with B.LightedVolumeRenderer() as r:
+        scenes.configure(r, sc)
+        r.SetSampleCounting(True)
+        r.Render(B.Rays(o, d))
+        cam, light = r.GetSampleCounts()
+        img, _ = r.RenderImage(sc.camera, sc.width, sc.height)
+        cam2, light2 = r.GetSampleCounts()
+    O = oracle_mod
+    rb = O.RayBuf(o, d, "f32")
+    O.render_rays(O.volume_from_scene(sc, "f32"), rb)
+    assert cam == int(rb.nsamples.sum()) == cam2
+    assert light == light2 and light > 0
+
+
 def test_alpha_stop_099(oracle_mod):
     sc = scenes.plain_small(seed=3, alpha_stop=0.99)
     o, d = B.camera_rays(sc.camera, sc.width, sc.height)
