@@ -1,0 +1,351 @@
+"""Pins for the oracle (CPU): each check is fixed by geometry or a closed form, not by
+re-typing the oracle's own formula, and is chosen so that a plausible slip (a
+dropped term, a swapped corner, a wrong sign or index) fails it.
+
+The reference ships no tests, fixtures or golden vectors for this path, so these
+checks (with the committed fixtures in tests/golden) are what pin the oracle.
+"""
+
+import numpy as np
+import pytest
+
+
+def _vol(O, n=9, field=None, tf=None, twin="f64", coords=None, **kw):
+    if coords is None:
+        c = np.arange(n, dtype=np.float64) / (n - 1)
+        coords = [c, c, c]
+    dims = [len(c) for c in coords]
+    if field is None:
+        field = np.zeros(int(np.prod(dims)))
+    if tf is None:
+        tf = np.tile([1.0, 1.0, 1.0, 0.5], (256, 1))
+    return O.Volume(dims, coords, field, tf, twin=twin, **kw)
+
+
+# ---- SegmentAABB (Intersections.cxx:8-49) -----------------------------------------------
+
+def test_segment_aabb_matches_dense_sampling(oracle_mod):
+    O = oracle_mod
+    rng = np.random.default_rng(1)
+    for _ in range(300):
+        lo = rng.uniform(-1, 0.5, 3)
+        hi = lo + rng.uniform(0.2, 1.5, 3)
+        box = np.array([lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]])
+        s = rng.uniform(-2, 3, 3)
+        e = rng.uniform(-2, 3, 3)
+        hit, t0, t1 = O.segment_aabb(s, e, box, twin="f64")
+        L = np.linalg.norm(e - s)
+        ts = np.linspace(0, L, 20001)
+        pts = s[None, :] + ts[:, None] * ((e - s) / L)[None, :]
+        inside = np.all((pts >= lo) & (pts <= hi), axis=1)
+        if inside.any():
+            assert hit
+            # the reference replaces an interval start below zero by tEps only when tMin < 0,
+            # which cannot happen (tMin starts at 0), so the sampled interval is [t0, t1]
+            assert abs(ts[inside][0] - t0) <= 2 * L / 20000 + 1e-12
+            assert abs(ts[inside][-1] - t1) <= 2 * L / 20000 + 1e-12
+        else:
+            assert (not hit) or (t1 - t0) < 2 * L / 20000
+
+
+def test_segment_aabb_clamps_to_segment(oracle_mod):
+    O = oracle_mod
+    box = [0, 1, 0, 1, 0, 1]
+    hit, t0, t1 = O.segment_aabb([0.5, 0.5, 0.5], [0.5, 0.5, 3.0], box, twin="f64")
+    assert hit and t0 == 0.0 and abs(t1 - 0.5) < 1e-15  # starts inside: t0 = 0
+    hit, t0, t1 = O.segment_aabb([0.5, 0.5, -1.0], [0.5, 0.5, 0.25], box, twin="f64")
+    assert hit and abs(t0 - 1.0) < 1e-15 and abs(t1 - 1.25) < 1e-15  # ends inside: tMax = length
+    hit, _, _ = O.segment_aabb([2.0, 2.0, 2.0], [3.0, 2.0, 2.0], box, twin="f64")
+    assert not hit
+
+
+# ---- FindCell + hex interpolation ------------------------------------------------------------
+
+def test_interp_hex_corner_order(oracle_mod):
+    """Corner k of the hexahedron sits at VTK order 0:(0,0,0) 1:(1,0,0) 2:(1,1,0) 3:(0,1,0),
+    4-7 the same at z=1."""
+    O = oracle_mod
+    corners = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    for k, pc in enumerate(corners):
+        v = np.zeros(8)
+        v[k] = 1.0
+        for tw in ("f32", "f64"):
+            assert O.interp_hex(v, pc, tw) == 1.0
+            for j, pc2 in enumerate(corners):
+                if j != k:
+                    assert O.interp_hex(v, pc2, tw) == 0.0
+    # interior point: the trilinear weights
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        v = rng.uniform(-1, 1, 8)
+        x, y, z = rng.uniform(0, 1, 3)
+        w = [(1 - x) * (1 - y) * (1 - z), x * (1 - y) * (1 - z), x * y * (1 - z), (1 - x) * y * (1 - z),
+             (1 - x) * (1 - y) * z, x * (1 - y) * z, x * y * z, (1 - x) * y * z]
+        assert abs(O.interp_hex(v, [x, y, z], "f64") - float(np.dot(w, v))) < 1e-14
+
+
+@pytest.mark.parametrize("nonuniform", [False, True])
+def test_find_cell_linear_field_exact(oracle_mod, nonuniform):
+    """A linear field is reproduced exactly by trilinear interpolation, on uniform and on
+    truly rectilinear (random spacing) coordinates; the cell contains the point."""
+    O = oracle_mod
+    rng = np.random.default_rng(3)
+    coords = []
+    for n in (7, 9, 11):
+        if nonuniform:
+            c = np.concatenate([[0.0], np.cumsum(rng.uniform(0.2, 1.0, n - 1))])
+            c = c / c[-1]
+        else:
+            c = np.arange(n) / (n - 1)
+        coords.append(c)
+    X, Y, Z = np.meshgrid(coords[0], coords[1], coords[2], indexing="ij")
+    f = (0.3 * X - 1.7 * Y + 2.1 * Z + 0.25).transpose(2, 1, 0).ravel()  # x fastest
+    vol = _vol(O, field=f, coords=coords)
+    for _ in range(200):
+        p = rng.uniform(0, 1, 3)
+        cid, pc = O.find_cell(vol, p)
+        assert cid >= 0
+        cd = [len(c) - 1 for c in coords]
+        i, j, k = cid % cd[0], (cid // cd[0]) % cd[1], cid // (cd[0] * cd[1])
+        for d, ix in enumerate((i, j, k)):
+            assert coords[d][ix] <= p[d] <= coords[d][ix + 1]
+            assert -1e-12 <= pc[d] <= 1 + 1e-12
+        nx, ny = len(coords[0]), len(coords[1])
+        base = (k * ny + j) * nx + i
+        idx = [base, base + 1, base + 1 + nx, base + nx, base + nx * ny, base + nx * ny + 1,
+               base + nx * ny + 1 + nx, base + nx * ny + nx]
+        val = O.interp_hex(f[idx], pc, "f64")
+        assert abs(val - (0.3 * p[0] - 1.7 * p[1] + 2.1 * p[2] + 0.25)) < 1e-12
+    # outside -> -1; a point on an interior plane goes to the lower cell
+    assert O.find_cell(vol, [1.2, 0.5, 0.5])[0] == -1
+    c1 = coords[0][3]
+    cid, pc = O.find_cell(vol, [c1, 0.5, 0.5])
+    assert cid % (len(coords[0]) - 1) == 2 and abs(pc[0] - 1.0) < 1e-15
+
+
+# ---- TransmittanceMapGenerator (TransmittanceMap.h:239-340) --------------------------------
+
+def _light_geometry(vol_dims_cells, light, n_cells):
+    """Exact per-vertex segment length inside the unit box and number of steps."""
+    S = n_cells
+    g = np.arange(S + 1) / S
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    V = np.stack([X, Y, Z], -1).transpose(2, 1, 0, 3).reshape(-1, 3)
+    return V
+
+
+def test_map_single_alpha_level_saturates(oracle_mod):
+    """One alpha level in the TF: alpha / max alpha = 1, so every vertex that takes at least
+    one step inside the volume has opacity exactly 1, the others 0."""
+    O = oracle_mod
+    n = 9
+    f = np.random.default_rng(0).uniform(0, 1, n ** 3)
+    tf = np.tile([0.2, 0.4, 0.6, 0.37], (256, 1))
+    for tw in ("f32", "f64"):
+        vol = _vol(O, n, f, tf, tw, light_pos=(2.5, 1.8, -1.2), map_cells=(6, 6, 6))
+        m = O.tmap_build(vol).reshape(7, 7, 7)
+        assert set(np.unique(m)).issubset({0.0, 1.0})
+        # interior vertices are 1/6 from every face, farther than one step (sqrt(3)/128),
+        # so their first step is inside
+        assert np.all(m[1:-1, 1:-1, 1:-1] == 1.0)
+        # on the face the light enters through, a vertex whose first step leaves the box
+        # (here the y = 0 edge of z = 0, the light being above y = 1) stays at 0
+        assert np.all(m[0, 0, :] == 0.0)
+
+
+def test_map_constant_field_closed_form(oracle_mod):
+    """Constant field, two-level TF: opacity = 1 - (1 - r)^K with K the number of steps of
+    length |extent|/128 taken inside the box, K = floor(L / step) + 1 (the step past the
+    vertex included) minus that last step when it leaves the box."""
+    O = oracle_mod
+    n, S = 9, 8
+    r = 0.25
+    tf = np.tile([1.0, 1.0, 1.0, 0.8], (256, 1))
+    v = 0.5
+    tf[int(v * 255)] = [1, 1, 1, 0.8 * r]  # the constant scalar's entry: alpha/max = r
+    light = np.array([2.7, -1.9, 1.4])
+    vol = _vol(O, n, np.full(n ** 3, v), tf, "f64", light_pos=light, map_cells=(S, S, S))
+    m = O.tmap_build(vol)
+    step = np.sqrt(3.0) / 128
+    g = np.arange(S + 1) / S
+    ok = total = 0
+    for k in range(S + 1):
+        for j in range(S + 1):
+            for i in range(S + 1):
+                p = np.array([g[i], g[j], g[k]])
+                d = p - light
+                Lseg = np.linalg.norm(d)
+                d /= Lseg
+                # entry distance of the light ray into the unit box
+                with np.errstate(divide="ignore"):
+                    t0 = np.where(d != 0, (0 - light) / d, -np.inf)
+                    t1 = np.where(d != 0, (1 - light) / d, np.inf)
+                tin = np.max(np.minimum(t0, t1))
+                L = Lseg - tin
+                K = int(np.floor(L / step)) + 1
+                last = light + (tin + K * step) * d
+                if np.any(last < 0) or np.any(last > 1):
+                    K -= 1
+                expect = 1 - (1 - r) ** K
+                got = m[(k * (S + 1) + j) * (S + 1) + i]
+                total += 1
+                # K is exact away from the step boundary; allow a one-step slip where L/step
+                # sits within 1e-9 of an integer
+                if abs(got - expect) < 1e-12:
+                    ok += 1
+                else:
+                    frac = L / step - np.floor(L / step)
+                    assert min(frac, 1 - frac) < 1e-6, (i, j, k, got, expect)
+    # the slips are the entry-face vertices, where L = 0 exactly
+    assert ok / total > 0.95
+
+
+def test_map_twins_agree_on_smooth_input(oracle_mod):
+    """FP32 and FP64 twins agree closely where no TF index sits on a boundary."""
+    from pilot_amd import scenes
+
+    O = oracle_mod
+    sc = scenes.make_scene("t", 24, seed=5, nblobs=8)
+    m32 = O.tmap_build(O.volume_from_scene(sc, "f32"))
+    m64 = O.tmap_build(O.volume_from_scene(sc, "f64"))
+    err = np.abs(m32 - m64)
+    assert np.quantile(err, 0.99) < 1e-4
+    assert err.max() < 5e-2
+
+
+# ---- Sampler (LightedVolumeRenderer.cxx:294-499) + CalcRayStart -------------------------------
+
+def _rays_into_box(n, rng, twin):
+    org = np.stack([rng.uniform(-1, 2, n), rng.uniform(-1, 2, n), np.full(n, -1.5)])
+    tgt = rng.uniform(0.2, 0.8, (3, n))
+    d = tgt - org
+    d /= np.linalg.norm(d, axis=0)
+    return org, d
+
+
+def test_sampler_zero_alpha_is_black(oracle_mod):
+    O = oracle_mod
+    rng = np.random.default_rng(2)
+    tf = np.tile([0.9, 0.5, 0.1, 0.0], (256, 1))
+    for tw in ("f32", "f64"):
+        vol = _vol(O, 9, rng.uniform(0, 1, 729), tf, tw, use_shadow=False, sample_distance=0.01)
+        o, d = _rays_into_box(500, rng, tw)
+        rb = O.RayBuf(o, d, tw)
+        O.render_rays(vol, rb)
+        assert np.all(rb.rgba == 0)
+
+
+def test_sampler_constant_closed_form(oracle_mod):
+    """Constant field and TF, no shadow: A_n = 1 - (1 - a)^n, RGB = clamp(c) A_n, with n the
+    number of samples between the slab entry (+ mesh epsilon) and exit."""
+    O = oracle_mod
+    rng = np.random.default_rng(4)
+    a = 0.05
+    c = np.array([0.3, 0.6, 1.4])  # blue is clamped to 1
+    tf = np.tile(np.r_[c, a], (256, 1))
+    sd = 0.013
+    vol = _vol(O, 9, np.full(729, 0.5), tf, "f64", use_shadow=False, sample_distance=sd)
+    o, d = _rays_into_box(400, rng, "f64")
+    rb = O.RayBuf(o, d, "f64")
+    O.render_rays(vol, rb)
+    eps = np.sqrt(3.0) * 1e-4
+    with np.errstate(divide="ignore"):
+        t0 = np.where(d != 0, (0 - o) / d, -np.inf)
+        t1 = np.where(d != 0, (1 - o) / d, np.inf)
+    tin = np.max(np.minimum(t0, t1), axis=0)
+    tout = np.min(np.maximum(t0, t1), axis=0)
+    nexp = np.floor((tout - tin - eps) / sd).astype(int) + 1
+    A = 1 - (1 - a) ** nexp
+    ok = np.abs(rb.rgba[:, 3] - A) < 1e-12
+    assert ok.mean() > 0.97
+    assert np.allclose(rb.rgba[ok, :3], np.minimum(c, 1)[None, :] * A[ok, None], atol=1e-12)
+    assert np.array_equal(rb.nsamples[ok], nexp[ok])
+
+
+def test_sampler_shadow_scales_rgb(oracle_mod):
+    """A constant map opacity O scales each sample's colour by (1 - O) * light colour before
+    the clamp; alpha is unchanged."""
+    O = oracle_mod
+    rng = np.random.default_rng(5)
+    c = np.array([0.2, 0.3, 0.45])
+    tf = np.tile(np.r_[c, 0.1], (256, 1))
+    lc = (1.5, 2.0, 2.5)
+    vol = _vol(O, 9, np.full(729, 0.5), tf, "f64", sample_distance=0.02, light_color=lc,
+               map_cells=(4, 4, 4))
+    o, d = _rays_into_box(200, rng, "f64")
+    ref = O.RayBuf(o, d, "f64")
+    O.sample_rays(_vol(O, 9, np.full(729, 0.5), tf, "f64", use_shadow=False, sample_distance=0.02),
+                  None, ref)
+    for Ov in (0.0, 0.3, 0.9):
+        rb = O.RayBuf(o, d, "f64")
+        O.sample_rays(vol, np.full(125, Ov), rb)
+        scale = np.minimum(c * (1 - Ov) * np.array(lc), 1.0) / c
+        assert np.allclose(rb.rgba[:, 3], ref.rgba[:, 3], atol=1e-15)
+        assert np.allclose(rb.rgba[:, :3], ref.rgba[:, :3] * scale[None, :], atol=1e-12)
+
+
+def test_sampler_opaque_entry_stops_after_one_sample(oracle_mod):
+    O = oracle_mod
+    rng = np.random.default_rng(6)
+    tf = np.tile([0.25, 0.5, 0.75, 1.0], (256, 1))
+    vol = _vol(O, 9, np.full(729, 0.5), tf, "f64", use_shadow=False, sample_distance=0.01)
+    o, d = _rays_into_box(100, rng, "f64")
+    rb = O.RayBuf(o, d, "f64")
+    O.render_rays(vol, rb)
+    assert np.all(rb.nsamples == 1)
+    assert np.allclose(rb.rgba, [[0.25, 0.5, 0.75, 1.0]])
+
+
+def test_calc_ray_start_slab(oracle_mod):
+    """Entry/exit against the block bounds (FP64 twin vs the slab formula), misses flagged
+    with MinDistance = -1, axis-parallel rays through rcp_safe."""
+    O = oracle_mod
+    rng = np.random.default_rng(7)
+    n = 2000
+    o = rng.uniform(-2, 3, (3, n))
+    d = rng.normal(size=(3, n))
+    d[:, :100] = 0
+    d[rng.integers(0, 3, 100), np.arange(100)] = 1.0
+    d /= np.linalg.norm(d, axis=0)
+    vol = _vol(O, 5)
+    rb = O.RayBuf(o, d, "f64")
+    O.render_rays(_vol(O, 5, use_shadow=False, tf=np.zeros((4, 4))), rb)
+    dd = np.where(np.abs(d) < 1e-8, 1e-8, d)
+    t0, t1 = (0 - o) / dd, (1 - o) / dd
+    tin = np.maximum(np.max(np.minimum(t0, t1), axis=0), 0.0)
+    tout = np.min(np.maximum(t0, t1), axis=0)
+    hit = tout >= tin
+    assert np.array_equal(rb.min_d != -1, hit)
+    assert np.allclose(rb.min_d[hit], tin[hit], rtol=1e-12, atol=1e-9)
+    assert np.allclose(rb.max_d[hit], tout[hit], rtol=1e-12, atol=1e-9)
+    assert np.array_equal(rb.dist[hit], rb.min_d[hit])
+    del vol
+
+
+# ---- GlobalComposite (MapperLightedVolume.cxx:215-303) --------------------------------------
+
+def test_composite_depth_order_and_over(oracle_mod):
+    O = oracle_mod
+    rng = np.random.default_rng(8)
+    nb, npix = 5, 64
+    bounds = []
+    for b in range(nb):
+        lo = rng.uniform(0, 2, 3)
+        bounds.append([lo[0], lo[0] + 1, lo[1], lo[1] + 1, lo[2], lo[2] + 1])
+    imgs = []
+    for _ in range(nb):
+        a = rng.uniform(0, 1, npix)
+        rgb = rng.uniform(0, 1, (npix, 3)) * a[:, None]
+        imgs.append(np.c_[rgb, a])
+    cam = np.array([-3.0, 0.5, 4.0])
+    out = O.composite(np.array(bounds).ravel(), cam, imgs, "f64")
+    centres = np.array([[(b[0] + b[1]) / 2, (b[2] + b[3]) / 2, (b[4] + b[5]) / 2] for b in bounds])
+    order = np.argsort(np.linalg.norm(centres - cam, axis=1))
+    acc = imgs[order[0]].copy()
+    for b in order[1:]:
+        acc = acc + imgs[b] * (1 - acc[:, 3:4])
+    assert np.allclose(out, acc, atol=1e-14)
+    # input order does not matter, only the depth order
+    perm = rng.permutation(nb)
+    out2 = O.composite(np.array(bounds)[perm].ravel(), cam, [imgs[p] for p in perm], "f64")
+    assert np.allclose(out2, out, atol=1e-14)
