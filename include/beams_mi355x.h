@@ -97,6 +97,27 @@ int beams_set_field(beams_renderer* h, const void* values, int32_t dtype, int32_
                     const int64_t dims[3], const float* const coords[3], const double origin[3],
                     const double spacing[3], int32_t mem);
 
+/* ---- BoundsMap (BoundsMap.cxx:77-127, one block per rank) -> bricks on this device ---------
+ * Cuts the field of beams_set_field into nbricks blocks, each rendered as the reference
+ * renders one MPI rank's block: point_ranges holds 6 inclusive point-index bounds per brick
+ * (i0,i1,j0,j1,k0,k1; neighbours share their boundary plane), each brick is stored with
+ * `ghost` extra point layers (clipped at the field's edge; outside the brick's bounds), and
+ * map_cells gives 3 map cells per brick (NULL: the brick's own cell counts).  A frame then
+ * runs, for all bricks at once: local maps (phase 1), the non-local hit exchange and
+ * composite (TransmittanceMap.h:342-513, LightedVolumeRenderer.cxx:968-1235, phase 2), the
+ * seeded re-march (phase 3), the per-brick march (phase 4) and the depth-ordered over
+ * composite (MapperLightedVolume.cxx:215-303, phase 5).  nbricks = 0 returns to one block;
+ * beams_set_field drops the bricks. */
+int beams_set_bricks(beams_renderer* h, int32_t nbricks, const int64_t* point_ranges,
+                     int32_t ghost, const int64_t* map_cells);
+
+/* Copy of brick b's map: which = 0 the phase-1 local map, 1 the final (seeded) map;
+ * (cells+1)^3 FP32 values. */
+int beams_get_brick_map(beams_renderer* h, int32_t brick, int32_t which, float* out);
+
+/* Workgroups of the persistent brick launches (a schedule knob; default 2048). */
+int beams_set_persistent_groups(beams_renderer* h, int32_t groups);
+
 /* scalarRange argument of SetData (vtkm::Range). */
 int beams_set_scalar_range(beams_renderer* h, double min_value, double max_value);
 
@@ -185,6 +206,12 @@ int beams_gen_field(const beams_field_recipe* r, const int64_t dims[3], int32_t 
 /* Piecewise-linear RGBA table of n entries between npoints seeded control points
  * (x = 0 and x = 1 included, RGBA uniform in [0,1]).  rgba32 and/or rgba64 may be NULL. */
 int beams_gen_colormap(uint64_t seed, int32_t npoints, int64_t n, float* rgba32, double* rgba64);
+
+/* Seeded kd decomposition of a grid of dims points into nbricks bricks whose cell extents
+ * per axis lie in [min_cells, max_cells]; point_ranges receives 6 values per brick in the
+ * layout of beams_set_bricks. */
+int beams_gen_kd_bricks(uint64_t seed, const int64_t dims[3], int32_t nbricks, int64_t min_cells,
+                        int64_t max_cells, int64_t* point_ranges);
 
 /* Orbit camera looking at the centre of bounds (xmin,xmax,ymin,ymax,zmin,zmax) from a
  * distance of one box diagonal, at azimuth/elevation in degrees, up = +y. */

@@ -42,6 +42,7 @@ typedef struct OrcScene {
   int64_t map_cells[3];   /* SetShadowMapSize (cells per axis)                  */
   double sample_distance; /* <= 0 selects |extent| / 200                        */
   double alpha_stop;      /* early-termination threshold (reference: 1.0)       */
+  double light_far;       /* directional light: origin distance (<= 0: 4|extent|) */
 } OrcScene;
 
 typedef struct OrcRays {

@@ -31,6 +31,7 @@ class OrcScene(C.Structure):
         ("map_cells", C.c_int64 * 3),
         ("sample_distance", C.c_double),
         ("alpha_stop", C.c_double),
+        ("light_far", C.c_double),
     ]
 
 
@@ -96,7 +97,8 @@ class Volume:
 
     def __init__(self, dims, coords, field, tf, scalar_range=(0.0, 1.0), light_kind=0,
                  light_pos=(2.0, 2.0, 2.0), light_color=(1.0, 1.0, 1.0), map_cells=(16, 16, 16),
-                 use_shadow=True, sample_distance=-1.0, alpha_stop=1.0, assoc=0, twin="f32"):
+                 use_shadow=True, sample_distance=-1.0, alpha_stop=1.0, assoc=0, twin="f32",
+                 light_far=0.0):
         dt = _dt(twin)
         self.twin = twin
         self.coords = [np.ascontiguousarray(c, dt) for c in coords]
@@ -117,6 +119,7 @@ class Volume:
         s.map_cells = (C.c_int64 * 3)(*[int(v) for v in map_cells])
         s.sample_distance = float(sample_distance)
         s.alpha_stop = float(alpha_stop)
+        s.light_far = float(light_far)
         self.s = s
         self.map_cells = tuple(int(v) for v in map_cells)
 
@@ -247,3 +250,54 @@ def composite(block_bounds, cam_pos, images, twin="f32"):
     getattr(lib(), f"orc_composite_{twin}")(len(imgs), bb.ctypes.data, cp.ctypes.data, ptrs,
                                             C.c_int64(npix), out.ctypes.data)
     return out
+
+
+# ---- many blocks (the reference's one block per MPI rank) ------------------------------------
+
+def brick_volumes(sc, ranges, twin="f32", map_cells=None):
+    """One Volume per brick of a pilot_amd.scenes.Scene: the owned points of the brick
+    (ranges: inclusive point bounds per axis), its slice of the coordinates, map cells =
+    brick cells unless given, and one directional-light distance for all bricks."""
+    nx, ny, nz = sc.dims
+    field = (sc.field32 if twin == "f32" else sc.field64).reshape(nz, ny, nx)
+    coords = sc.coords if twin == "f32" else sc.coords64
+    tf = sc.tf32 if twin == "f32" else sc.tf64
+    ext = np.float32(np.sqrt(sum(np.float32(float(c[-1]) - float(c[0])) ** 2 for c in sc.coords)))
+    light_far = float(np.float32(4.0) * ext) if twin == "f32" else 4.0 * float(
+        np.sqrt(sum((c[-1] - c[0]) ** 2 for c in sc.coords64)))
+    sd = float(sc.sample_distance) if twin == "f32" else 0.5 / (sc.dims[0] - 1)
+    vols = []
+    for b, r in enumerate(np.asarray(ranges).reshape(-1, 6)):
+        lo, hi = r[0::2], r[1::2]
+        cs = [coords[d][lo[d]:hi[d] + 1] for d in range(3)]
+        f = np.ascontiguousarray(field[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]).ravel()
+        mc = tuple(int(v) for v in (hi - lo)) if map_cells is None else tuple(map_cells[b])
+        vols.append(Volume(hi - lo + 1, cs, f, tf, sc.scalar_range, sc.light_kind,
+                           [float(v) for v in sc.light_pos], [float(v) for v in sc.light_color],
+                           mc, True, sd, sc.alpha_stop, twin=twin, light_far=light_far))
+    return vols
+
+
+def render_bricks(vols, origin, direction, cam_pos, nthreads=8, map_round=None):
+    """RenderOnDevice on every block (phase-1 maps, non-local seed, seeded re-march,
+    CalcRayStart + sampler over all rays) and GlobalComposite.  map_round, if given, is
+    applied to every map as it is stored (e.g. FP16 storage).  Returns
+    (image, local maps, final maps, partial images)."""
+    twin = vols[0].twin
+    L = lib()
+    rnd = map_round if map_round is not None else (lambda m: m)
+    local = [rnd(tmap_build(v, nthreads=nthreads)) for v in vols]
+    arr = (OrcScene * len(vols))(*[v.s for v in vols])
+    mptr = (C.c_void_p * len(vols))(*[m.ctypes.data for m in local])
+    final, partial, bounds = [], [], []
+    for b, v in enumerate(vols):
+        seed = np.zeros(v.nverts, _dt(twin))
+        getattr(L, f"orc_nonlocal_seed_{twin}")(len(vols), arr, mptr, b, seed.ctypes.data, nthreads)
+        final.append(rnd(tmap_build(v, seed, nthreads)))
+        rb = RayBuf(origin, direction, twin)
+        sample_rays(v, final[-1], rb, nthreads)
+        partial.append(rb.rgba)
+        bounds += [float(v.coords[0][0]), float(v.coords[0][-1]), float(v.coords[1][0]),
+                   float(v.coords[1][-1]), float(v.coords[2][0]), float(v.coords[2][-1])]
+    img = composite(np.array(bounds), np.asarray(cam_pos, np.float64), partial, twin)
+    return img, local, final, partial

@@ -104,7 +104,7 @@ static void FN(geo_init)(FN(Geo) * g, const OrcScene* s)
     R rs = (R)1 / SQRT_(FN(dot3)(g->light, g->light));
     for (int d = 0; d < 3; ++d) g->light[d] = g->light[d] * rs;
   }
-  g->dir_far = (R)4 * g->ext_mag;
+  g->dir_far = s->light_far > 0 ? (R)s->light_far : (R)4 * g->ext_mag;
 }
 
 /* ---- Intersections.cxx:8-49 ------------------------------------------------------ */

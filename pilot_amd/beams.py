@@ -74,6 +74,9 @@ SIGNATURES = {
     "beams_last_error": (C.c_char_p, [_P]),
     "beams_synchronize": (C.c_int, [_P]),
     "beams_set_field": (C.c_int, [_P, _P, _I32, _I32, _PI64, C.POINTER(_PF), _PD, _PD, _I32]),
+    "beams_set_bricks": (C.c_int, [_P, _I32, _PI64, _I32, _PI64]),
+    "beams_get_brick_map": (C.c_int, [_P, _I32, _I32, _PF]),
+    "beams_set_persistent_groups": (C.c_int, [_P, _I32]),
     "beams_set_scalar_range": (C.c_int, [_P, _D, _D]),
     "beams_set_colormap": (C.c_int, [_P, _PF, _I64]),
     "beams_set_light": (C.c_int, [_P, _I32, _PF, _PF, _F]),
@@ -90,6 +93,7 @@ SIGNATURES = {
     "beams_get_sample_counts": (C.c_int, [_P, _PI64, _PI64]),
     "beams_gen_field": (C.c_int, [C.POINTER(FieldRecipe), _PI64, _I32, _P, _I32]),
     "beams_gen_colormap": (C.c_int, [C.c_uint64, _I32, _I64, _PF, _PD]),
+    "beams_gen_kd_bricks": (C.c_int, [C.c_uint64, _PI64, _I32, _I64, _I64, _PI64]),
     "beams_gen_orbit_camera": (C.c_int, [_PD, _D, _D, _D, C.POINTER(Camera)]),
     "beams_gen_camera_rays": (C.c_int, [C.POINTER(Camera), _I32, _I32, C.POINTER(_PF), C.POINTER(_PF),
                                         C.POINTER(_PD), C.POINTER(_PD)]),
@@ -298,6 +302,24 @@ class LightedVolumeRenderer:
         self._c(lib().beams_get_timings(self._h, _f32p(t), len(PHASES)))
         return dict(zip(PHASES, t.tolist()))
 
+    # ---- BoundsMap -> bricks
+    def SetBricks(self, point_ranges, ghost=1, map_cells=None):
+        r = np.ascontiguousarray(point_ranges, dtype=np.int64).reshape(-1, 6)
+        mc = None if map_cells is None else np.ascontiguousarray(map_cells, dtype=np.int64).reshape(-1, 3)
+        self._c(lib().beams_set_bricks(self._h, r.shape[0], r.ctypes.data_as(_PI64), int(ghost),
+                                       None if mc is None else mc.ctypes.data_as(_PI64)))
+
+    def ClearBricks(self):
+        self._c(lib().beams_set_bricks(self._h, 0, None, 0, None))
+
+    def GetBrickMap(self, brick, nverts, final=True):
+        out = np.zeros(nverts, np.float32)
+        self._c(lib().beams_get_brick_map(self._h, int(brick), 1 if final else 0, _f32p(out)))
+        return out
+
+    def SetPersistentGroups(self, groups):
+        self._c(lib().beams_set_persistent_groups(self._h, int(groups)))
+
     def GetTimingStats(self):
         t = np.zeros(len(PHASES), np.float32)
         n = C.c_int32()
@@ -340,6 +362,14 @@ def gen_colormap(seed, npoints=6, n=256):
     t64 = np.zeros((n, 4), np.float64)
     _check(lib().beams_gen_colormap(seed, npoints, n, _f32p(t32), _f64p(t64)))
     return t32, t64
+
+
+def kd_bricks(dims, nbricks, seed, min_cells=32, max_cells=160):
+    """Seeded kd bricks: (nbricks, 6) inclusive point ranges (i0,i1,j0,j1,k0,k1)."""
+    out = np.zeros((nbricks, 6), np.int64)
+    _check(lib().beams_gen_kd_bricks(seed, _i64(dims), nbricks, min_cells, max_cells,
+                                     out.ctypes.data_as(_PI64)))
+    return out
 
 
 def orbit_camera(bounds, azimuth, elevation, fov=60.0):

@@ -3,6 +3,7 @@
 // device-resident inputs, and the frame driver of RenderOnDevice
 // (LightedVolumeRenderer.cxx:910-1372) over the HIP kernels.
 #include "../../include/beams_mi355x.h"
+#include "bricks.hpp"
 #include "camera.hpp"
 #include "gen.hpp"
 #include "kernels.hpp"
@@ -96,6 +97,21 @@ struct beams_renderer {
   // optional sample counters: [0] camera-ray samples, [1] light-march steps
   bool counting = false;
   DevBuf counters;
+  // Bricks: one reference block (MPI rank) per brick.  Empty = the field is one block.
+  struct Brick {
+    int64_t lo[3], hi[3];      // owned point ranges, inclusive (neighbours share planes)
+    int64_t slo[3], sdims[3];  // stored region: owned points + ghost layers
+    int64_t cells[3];          // map cells
+    DevBuf field;
+    int64_t nverts = 0, seed_off = 0;
+  };
+  std::vector<Brick> bricks;
+  int32_t ghost = 0;
+  DevBuf bmap_local, bmap_final, bseeds, btables, bpartial, bqueue;
+  std::vector<unsigned char> btab_host;  // tables of the last frame (reused while unchanged)
+  int32_t n_map_items = 0, n_px_items = 0, max_coords = 0;
+  int32_t persistent_groups = 2048;
+  bool overflow_checked = true;
 
   int fail(int code, const std::string& msg)
   {
@@ -199,6 +215,39 @@ ShadeParams shade_params(beams_renderer* h, const GridParams& g)
   return s;
 }
 
+// Pinhole basis of a beams_camera (FP64); false for a degenerate camera.
+bool camera_params(const beams_camera* cam, int32_t width, int32_t height, CameraParams& c)
+{
+  double f[3], len = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    f[d] = cam->look_at[d] - cam->eye[d];
+    len += f[d] * f[d];
+  }
+  len = std::sqrt(len);
+  if (!(len > 0.0)) return false;
+  for (int d = 0; d < 3; ++d) f[d] /= len;
+  double r[3] = { f[1] * cam->up[2] - f[2] * cam->up[1], f[2] * cam->up[0] - f[0] * cam->up[2],
+                  f[0] * cam->up[1] - f[1] * cam->up[0] };
+  const double rl = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  if (!(rl > 0.0)) return false;
+  for (int d = 0; d < 3; ++d) r[d] /= rl;
+  const double u[3] = { r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2],
+                        r[0] * f[1] - r[1] * f[0] };
+  for (int d = 0; d < 3; ++d) {
+    c.eye[d] = cam->eye[d];
+    c.fwd[d] = f[d];
+    c.right[d] = r[d];
+    c.up[d] = u[d];
+  }
+  c.tan_half_fov = std::tan(cam->fov_y_deg * M_PI / 360.0);
+  c.aspect = (double)width / (double)height;
+  c.width = width;
+  c.height = height;
+  c.near_d = (float)cam->near_dist;
+  c.far_d = cam->far_dist > 0.0 ? (float)cam->far_dist : INFINITY;
+  return true;
+}
+
 int check_ready(beams_renderer* h)
 {
   if (!h->has_field) return h->fail(BEAMS_ERR_STATE, "no field: call beams_set_field first");
@@ -248,6 +297,268 @@ int build_map(beams_renderer* h, const GridParams& g, const ShadeParams& s)
   MapParams m = map_params(h, g);
   HIP_TRY(h, launch_tmap_ref(g, m, s, nullptr, h->stream));
   h->map_valid = true;
+  return BEAMS_OK;
+}
+
+// ---- bricks ---------------------------------------------------------------------------------
+
+// Screen rectangle (x0, y0, w, h) that contains every pixel whose camera ray can meet the
+// box: the bounding box of the projected corners plus 2 pixels, or the whole image when a
+// corner is not in front of the camera.
+void footprint(const CameraParams& c, const double* bb, int32_t* fp)
+{
+  double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+  bool front = true;
+  for (int k = 0; k < 8; ++k) {
+    const double p[3] = { bb[(k & 1) ? 1 : 0], bb[(k & 2) ? 3 : 2], bb[(k & 4) ? 5 : 4] };
+    double v[3], z = 0.0, xr = 0.0, yu = 0.0;
+    for (int d = 0; d < 3; ++d) v[d] = p[d] - c.eye[d];
+    for (int d = 0; d < 3; ++d) {
+      z += v[d] * c.fwd[d];
+      xr += v[d] * c.right[d];
+      yu += v[d] * c.up[d];
+    }
+    if (!(z > 1e-6)) {
+      front = false;
+      break;
+    }
+    const double px = (xr / (z * c.tan_half_fov * c.aspect) + 1.0) * 0.5 * c.width - 0.5;
+    const double py = (1.0 - yu / (z * c.tan_half_fov)) * 0.5 * c.height - 0.5;
+    xmin = std::min(xmin, px);
+    xmax = std::max(xmax, px);
+    ymin = std::min(ymin, py);
+    ymax = std::max(ymax, py);
+  }
+  int64_t x0 = 0, x1 = c.width - 1, y0 = 0, y1 = c.height - 1;
+  if (front) {
+    x0 = std::max<int64_t>(0, (int64_t)std::floor(xmin) - 2);
+    x1 = std::min<int64_t>(c.width - 1, (int64_t)std::ceil(xmax) + 2);
+    y0 = std::max<int64_t>(0, (int64_t)std::floor(ymin) - 2);
+    y1 = std::min<int64_t>(c.height - 1, (int64_t)std::ceil(ymax) + 2);
+  }
+  fp[0] = (int32_t)x0;
+  fp[1] = (int32_t)y0;
+  fp[2] = x1 >= x0 ? (int32_t)(x1 - x0 + 1) : 0;
+  fp[3] = y1 >= y0 ? (int32_t)(y1 - y0 + 1) : 0;
+}
+
+template <typename T>
+size_t append(std::vector<unsigned char>& buf, const T* data, size_t n)
+{
+  const size_t off = (buf.size() + 255) & ~size_t(255);
+  buf.resize(off + sizeof(T) * n);
+  if (n) std::memcpy(buf.data() + off, data, sizeof(T) * n);
+  return off;
+}
+
+// Brick descriptors, work lists and depth order for camera c (host), uploaded when changed.
+int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int64_t& total_verts,
+                   size_t& partial_pixels)
+{
+  const GridParams whole = grid_params(h);
+  const float user_sd = h->sample_distance;  // each rank defaults to its own |extent| / 200
+  ShadeParams s0 = shade_params(h, whole);    // TF, range, counters; per-brick fields below
+  h->sample_distance = user_sd;
+  const size_t esz = dtype_size(h->dtype);
+  const size_t msz = dtype_size(h->map_dtype);
+  const int nb = (int)h->bricks.size();
+  std::vector<BrickDesc> desc((size_t)nb);
+  std::vector<int32_t> mi_brick, px_brick, px_tile, order((size_t)nb);
+  std::vector<int64_t> mi_first;
+  std::vector<float> depth((size_t)nb);
+  total_verts = 0;
+  partial_pixels = 0;
+  int32_t max_coords = 0;
+  const float* xdev = (const float*)h->coords.p;
+  const int64_t xoff[3] = { 0, h->dims[0], h->dims[0] + h->dims[1] };
+  for (int b = 0; b < nb; ++b) {
+    const auto& br = h->bricks[(size_t)b];
+    BrickDesc& d = desc[(size_t)b];
+    GridParams& g = d.g;
+    g.dtype = h->dtype;
+    g.assoc = 0;
+    const int64_t sx = br.sdims[0], sxy = br.sdims[0] * br.sdims[1];
+    g.field = (const unsigned char*)br.field.p +
+              esz * (size_t)((br.lo[2] - br.slo[2]) * sxy + (br.lo[1] - br.slo[1]) * sx + (br.lo[0] - br.slo[0]));
+    g.field_stride[0] = 1;
+    g.field_stride[1] = sx;
+    g.field_stride[2] = sxy;
+    for (int k = 0; k < 3; ++k) {
+      const std::vector<float>& x = h->hx[k];
+      g.n[k] = br.hi[k] - br.lo[k] + 1;
+      g.x[k] = xdev + xoff[k] + br.lo[k];
+      g.minp[k] = x[(size_t)br.lo[k]];
+      g.maxp[k] = x[(size_t)br.hi[k]];
+      g.gx0[k] = g.minp[k];
+      g.ginv[k] = (float)((double)(g.n[k] - 1) / ((double)g.maxp[k] - (double)g.minp[k]));
+      g.bounds[2 * k] = g.minp[k];
+      g.bounds[2 * k + 1] = g.maxp[k];
+    }
+    const float e0 = (float)(g.bounds[1] - g.bounds[0]), e1 = (float)(g.bounds[3] - g.bounds[2]),
+                e2 = (float)(g.bounds[5] - g.bounds[4]);
+    g.ext_mag = std::sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+    max_coords = std::max<int32_t>(max_coords, (int32_t)(g.n[0] + g.n[1] + g.n[2]));
+    // map over the brick bounds (CreateDataSetForOpacityMap per rank)
+    MapParams& m = d.m;
+    m.dtype = h->map_dtype;
+    m.light_kind = h->light_kind;
+    float size[3];
+    for (int k = 0; k < 3; ++k) {
+      m.mp[k] = br.cells[k] + 1;
+      m.morg[k] = (float)g.bounds[2 * k];
+      size[k] = (float)(g.bounds[2 * k + 1] - g.bounds[2 * k]);
+      m.msp[k] = size[k] / (float)br.cells[k];
+      m.minv[k] = 1.0f / m.msp[k];
+      m.light[k] = h->light[k];
+      m.light_color[k] = h->light_color[k];
+    }
+    m.step = std::sqrt(size[0] * size[0] + size[1] * size[1] + size[2] * size[2]) / 128.0f;
+    m.dir_far = 4.0f * whole.ext_mag;  // one light origin for all bricks
+    m.map = (unsigned char*)h->bmap_local.p + msz * (size_t)br.seed_off;
+    d.map_final = (unsigned char*)h->bmap_final.p + msz * (size_t)br.seed_off;
+    // per-rank Render() state: its own extent sets the mesh epsilon (and the default step)
+    ShadeParams& s = d.s;
+    s = s0;
+    s.sample_dist = user_sd > 0.0f ? user_sd : g.ext_mag / 200.0f;
+    s.mesh_eps = g.ext_mag * 0.0001f;
+    s.max_steps = (int64_t)(2.0f * g.ext_mag / s.sample_dist) + 64;
+    d.nverts = br.nverts;
+    d.seed_off = br.seed_off;
+    for (int k = 0; k < 3; ++k) {
+      const float pad = 1e-4f * (float)(g.bounds[2 * k + 1] - g.bounds[2 * k]) + 1e-6f;
+      d.box32[2 * k] = (float)g.bounds[2 * k] - pad;
+      d.box32[2 * k + 1] = (float)g.bounds[2 * k + 1] + pad;
+    }
+    footprint(c, g.bounds, d.fp);
+    d.img_off = (int64_t)partial_pixels;
+    partial_pixels += (size_t)d.fp[2] * (size_t)d.fp[3];
+    for (int64_t v = 0; v < br.nverts; v += 256) {
+      mi_brick.push_back(b);
+      mi_first.push_back(v);
+    }
+    const int tiles = ((d.fp[2] + 15) / 16) * ((d.fp[3] + 15) / 16);
+    for (int t = 0; t < tiles; ++t) {
+      px_brick.push_back(b);
+      px_tile.push_back(t);
+    }
+    total_verts += br.nverts;
+    // MapperLightedVolume.cxx:222-236: |camera position - block centre| in FP32
+    const float cx = (float)((g.bounds[0] + g.bounds[1]) * 0.5);
+    const float cy = (float)((g.bounds[2] + g.bounds[3]) * 0.5);
+    const float cz = (float)((g.bounds[4] + g.bounds[5]) * 0.5);
+    const float dx = (float)c.eye[0] - cx, dy = (float)c.eye[1] - cy, dz = (float)c.eye[2] - cz;
+    depth[(size_t)b] = std::sqrt(dx * dx + dy * dy + dz * dz);
+    order[(size_t)b] = b;
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t b) { return depth[(size_t)a] < depth[(size_t)b]; });
+  HIP_TRY(h, h->bpartial.ensure(std::max<size_t>(partial_pixels, 1) * 20));
+  std::vector<unsigned char> tab;
+  const size_t o_desc = append(tab, desc.data(), desc.size());
+  const size_t o_mib = append(tab, mi_brick.data(), mi_brick.size());
+  const size_t o_mif = append(tab, mi_first.data(), mi_first.size());
+  const size_t o_pxb = append(tab, px_brick.data(), px_brick.size());
+  const size_t o_pxt = append(tab, px_tile.data(), px_tile.size());
+  const size_t o_ord = append(tab, order.data(), order.size());
+  if (tab != h->btab_host) {
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    HIP_TRY(h, h->btables.ensure(tab.size()));
+    HIP_TRY(h, hipMemcpy(h->btables.p, tab.data(), tab.size(), hipMemcpyHostToDevice));
+    h->btab_host.swap(tab);
+  }
+  unsigned char* base = (unsigned char*)h->btables.p;
+  f = BrickFrame{};
+  f.bricks = (const BrickDesc*)(base + o_desc);
+  f.nbricks = nb;
+  f.map_item_brick = (const int32_t*)(base + o_mib);
+  f.map_item_first = (const int64_t*)(base + o_mif);
+  f.n_map_items = (int32_t)mi_brick.size();
+  f.px_item_brick = (const int32_t*)(base + o_pxb);
+  f.px_item_tile = (const int32_t*)(base + o_pxt);
+  f.n_px_items = (int32_t)px_brick.size();
+  f.depth_order = (const int32_t*)(base + o_ord);
+  f.queue = (unsigned int*)h->bqueue.p;
+  f.overflow = (unsigned int*)h->bqueue.p + 4;
+  f.seeds = (float*)h->bseeds.p;
+  f.partial = (float4*)h->bpartial.p;
+  f.partial_depth = (float*)((float4*)h->bpartial.p + partial_pixels);
+  f.max_coords = max_coords;
+  f.persistent_groups = h->persistent_groups;
+  h->n_map_items = f.n_map_items;
+  h->n_px_items = f.n_px_items;
+  h->max_coords = max_coords;
+  return BEAMS_OK;
+}
+
+int check_overflow(beams_renderer* h)
+{
+  if (h->bricks.empty() || h->overflow_checked) return BEAMS_OK;
+  unsigned int flag = 0;
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  HIP_TRY(h, hipMemcpy(&flag, (unsigned int*)h->bqueue.p + 4, sizeof(flag), hipMemcpyDeviceToHost));
+  h->overflow_checked = true;
+  if (flag)
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "a light path crossed more bricks than the exchange keeps (64)");
+  return BEAMS_OK;
+}
+
+// RenderOnDevice phases 1-4 for every brick + GlobalComposite + BlendBackground.
+int render_bricks(beams_renderer* h, const CameraParams& c, const float* background, float* rgba,
+                  float* depth, int32_t mem)
+{
+  int rc = check_overflow(h);
+  if (rc) return rc;
+  const size_t msz = dtype_size(h->map_dtype);
+  int64_t nv_total = 0;
+  for (const auto& br : h->bricks) nv_total += br.nverts;
+  HIP_TRY(h, h->bmap_local.ensure((size_t)nv_total * msz));
+  HIP_TRY(h, h->bmap_final.ensure((size_t)nv_total * msz));
+  HIP_TRY(h, h->bseeds.ensure((size_t)nv_total * sizeof(float)));
+  HIP_TRY(h, h->bqueue.ensure(8 * sizeof(unsigned int)));
+  if ((rc = frame_begin(h))) return rc;
+  BrickFrame f;
+  int64_t total_verts = 0;
+  size_t partial_pixels = 0;
+  if ((rc = prepare_bricks(h, c, f, total_verts, partial_pixels))) return rc;
+  const int32_t ntf = (int32_t)(h->htf.size() / 4);
+  const bool shadows = h->use_shadow;
+  HIP_TRY(h, launch_zero_u32(f.queue, 5, h->stream));
+  if (shadows) {
+    if ((rc = phase_mark(h, BEAMS_PHASE_MAP, false))) return rc;
+    HIP_TRY(h, launch_bricks_map(f, h->dtype, h->map_dtype, ntf, false, h->stream));
+    if ((rc = phase_mark(h, BEAMS_PHASE_MAP, true))) return rc;
+    if ((rc = phase_mark(h, BEAMS_PHASE_EXCHANGE, false))) return rc;
+    HIP_TRY(h, launch_bricks_exchange(f, h->map_dtype, total_verts, h->stream));
+    if ((rc = phase_mark(h, BEAMS_PHASE_EXCHANGE, true))) return rc;
+    if ((rc = phase_mark(h, BEAMS_PHASE_REMAP, false))) return rc;
+    HIP_TRY(h, launch_bricks_map(f, h->dtype, h->map_dtype, ntf, true, h->stream));
+    if ((rc = phase_mark(h, BEAMS_PHASE_REMAP, true))) return rc;
+    h->map_valid = true;
+    h->overflow_checked = false;
+  }
+  if ((rc = phase_mark(h, BEAMS_PHASE_MARCH, false))) return rc;
+  HIP_TRY(h, launch_bricks_march(f, h->dtype, h->map_dtype, ntf, shadows, c, h->stream));
+  if ((rc = phase_mark(h, BEAMS_PHASE_MARCH, true))) return rc;
+  const size_t npix = (size_t)c.width * (size_t)c.height;
+  ImageOut o{};
+  if (mem == BEAMS_MEM_DEVICE) {
+    o.rgba = rgba;
+    o.depth = depth;
+  } else {
+    HIP_TRY(h, h->image.ensure(sizeof(float) * npix * 5));
+    o.rgba = (float*)h->image.p;
+    o.depth = depth ? (float*)h->image.p + 4 * npix : nullptr;
+  }
+  o.use_bg = background ? 1 : 0;
+  for (int k = 0; k < 4; ++k) o.bg[k] = background ? background[k] : 0.0f;
+  if ((rc = phase_mark(h, BEAMS_PHASE_COMPOSITE, false))) return rc;
+  HIP_TRY(h, launch_bricks_composite(f, c, o, h->stream));
+  if ((rc = phase_mark(h, BEAMS_PHASE_COMPOSITE, true))) return rc;
+  if (mem == BEAMS_MEM_HOST) {
+    HIP_TRY(h, hipMemcpyAsync(rgba, o.rgba, sizeof(float) * 4 * npix, hipMemcpyDeviceToHost, h->stream));
+    if (depth) HIP_TRY(h, hipMemcpyAsync(depth, o.depth, sizeof(float) * npix, hipMemcpyDeviceToHost, h->stream));
+    if ((rc = check_overflow(h))) return rc;
+  }
   return BEAMS_OK;
 }
 
@@ -302,7 +613,7 @@ int beams_synchronize(beams_renderer* h)
 {
   CHECK_HANDLE(h);
   HIP_TRY(h, hipStreamSynchronize(h->stream));
-  return BEAMS_OK;
+  return check_overflow(h);
 }
 
 int beams_set_field(beams_renderer* h, const void* values, int32_t dtype, int32_t assoc,
@@ -356,7 +667,95 @@ int beams_set_field(beams_renderer* h, const void* values, int32_t dtype, int32_
   h->assoc = assoc;
   h->has_field = true;
   h->map_valid = false;
+  h->bricks.clear();  // a new field drops the previous brick decomposition
+  h->btab_host.clear();
   return BEAMS_OK;
+}
+
+int beams_set_bricks(beams_renderer* h, int32_t nbricks, const int64_t* point_ranges,
+                     int32_t ghost, const int64_t* map_cells)
+{
+  CHECK_HANDLE(h);
+  if (nbricks == 0) {
+    h->bricks.clear();
+    h->btab_host.clear();
+    return BEAMS_OK;
+  }
+  if (nbricks < 0 || nbricks > 4096 || !point_ranges)
+    return h->fail(BEAMS_ERR_INVALID, "bricks: need 1..4096 point ranges");
+  if (ghost < 0 || ghost > 8) return h->fail(BEAMS_ERR_INVALID, "bricks: ghost layers must be in [0, 8]");
+  if (!h->has_field) return h->fail(BEAMS_ERR_STATE, "bricks: call beams_set_field first");
+  if (h->assoc != BEAMS_ASSOC_POINTS || h->dtype == BEAMS_F64)
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "bricks need a point field stored as F32, F16 or BF16");
+  std::vector<beams_renderer::Brick> bricks((size_t)nbricks);
+  const size_t esz = dtype_size(h->dtype);
+  int64_t off = 0;
+  for (int b = 0; b < nbricks; ++b) {
+    auto& br = bricks[(size_t)b];
+    for (int d = 0; d < 3; ++d) {
+      br.lo[d] = point_ranges[6 * b + 2 * d];
+      br.hi[d] = point_ranges[6 * b + 2 * d + 1];
+      if (br.lo[d] < 0 || br.hi[d] >= h->dims[d] || br.hi[d] <= br.lo[d])
+        return h->fail(BEAMS_ERR_INVALID, "bricks: each range must satisfy 0 <= lo < hi < dims");
+      br.slo[d] = std::max<int64_t>(0, br.lo[d] - ghost);
+      br.sdims[d] = std::min<int64_t>(h->dims[d] - 1, br.hi[d] + ghost) - br.slo[d] + 1;
+      br.cells[d] = map_cells ? map_cells[3 * b + d] : br.hi[d] - br.lo[d];
+      if (br.cells[d] < 1 || br.cells[d] > 4096)
+        return h->fail(BEAMS_ERR_INVALID, "bricks: map cells must be in [1, 4096]");
+    }
+    br.nverts = (br.cells[0] + 1) * (br.cells[1] + 1) * (br.cells[2] + 1);
+    br.seed_off = off;
+    off += br.nverts;
+    HIP_TRY(h, br.field.ensure(esz * (size_t)(br.sdims[0] * br.sdims[1] * br.sdims[2])));
+    HIP_TRY(h, launch_extract_brick(h->field.p, (int32_t)esz, h->dims, br.slo, br.sdims, br.field.p,
+                                    h->stream));
+  }
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->bricks.swap(bricks);
+  for (auto& br : bricks) br.field.release();
+  h->ghost = ghost;
+  h->btab_host.clear();
+  h->map_valid = false;
+  return BEAMS_OK;
+}
+
+int beams_get_brick_map(beams_renderer* h, int32_t brick, int32_t which, float* out)
+{
+  CHECK_HANDLE(h);
+  if (!out || brick < 0 || brick >= (int32_t)h->bricks.size() || (which != 0 && which != 1))
+    return h->fail(BEAMS_ERR_INVALID, "bad brick index, map selector or output");
+  if (!h->map_valid) return h->fail(BEAMS_ERR_STATE, "no brick maps: render with shadows first");
+  const auto& br = h->bricks[(size_t)brick];
+  const size_t msz = dtype_size(h->map_dtype);
+  const unsigned char* src =
+    (const unsigned char*)(which ? h->bmap_final.p : h->bmap_local.p) + msz * (size_t)br.seed_off;
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  if (h->map_dtype == BEAMS_F32) {
+    HIP_TRY(h, hipMemcpy(out, src, sizeof(float) * (size_t)br.nverts, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<uint16_t> tmp((size_t)br.nverts);
+    HIP_TRY(h, hipMemcpy(tmp.data(), src, 2 * (size_t)br.nverts, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < tmp.size(); ++i) out[i] = f16_to_f32(tmp[i]);
+  }
+  return BEAMS_OK;
+}
+
+int beams_set_persistent_groups(beams_renderer* h, int32_t groups)
+{
+  CHECK_HANDLE(h);
+  if (groups < 1 || groups > 65536) return h->fail(BEAMS_ERR_INVALID, "persistent groups must be in [1, 65536]");
+  h->persistent_groups = groups;
+  return BEAMS_OK;
+}
+
+int beams_gen_kd_bricks(uint64_t seed, const int64_t dims[3], int32_t nbricks, int64_t min_cells,
+                        int64_t max_cells, int64_t* point_ranges)
+{
+  if (!dims || !point_ranges || nbricks < 1 || min_cells < 1 || max_cells < 2 * min_cells)
+    return BEAMS_ERR_INVALID;
+  std::string err;
+  return gen_kd_bricks(seed, dims, nbricks, min_cells, max_cells, point_ranges, err) ? BEAMS_ERR_INVALID
+                                                                                      : BEAMS_OK;
 }
 
 int beams_set_scalar_range(beams_renderer* h, double min_value, double max_value)
@@ -521,34 +920,9 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
   int rc = check_ready(h);
   if (rc) return rc;
   CameraParams c{};
-  {
-    double f[3], len = 0.0;
-    for (int d = 0; d < 3; ++d) {
-      f[d] = cam->look_at[d] - cam->eye[d];
-      len += f[d] * f[d];
-    }
-    len = std::sqrt(len);
-    if (!(len > 0.0)) return h->fail(BEAMS_ERR_INVALID, "camera eye equals look_at");
-    for (int d = 0; d < 3; ++d) f[d] /= len;
-    double r[3] = { f[1] * cam->up[2] - f[2] * cam->up[1], f[2] * cam->up[0] - f[0] * cam->up[2],
-                    f[0] * cam->up[1] - f[1] * cam->up[0] };
-    double rl = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-    if (!(rl > 0.0)) return h->fail(BEAMS_ERR_INVALID, "camera up is parallel to the view direction");
-    for (int d = 0; d < 3; ++d) r[d] /= rl;
-    double u[3] = { r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0] };
-    for (int d = 0; d < 3; ++d) {
-      c.eye[d] = cam->eye[d];
-      c.fwd[d] = f[d];
-      c.right[d] = r[d];
-      c.up[d] = u[d];
-    }
-    c.tan_half_fov = std::tan(cam->fov_y_deg * M_PI / 360.0);
-    c.aspect = (double)width / (double)height;
-    c.width = width;
-    c.height = height;
-    c.near_d = (float)cam->near_dist;
-    c.far_d = cam->far_dist > 0.0 ? (float)cam->far_dist : INFINITY;
-  }
+  if (!camera_params(cam, width, height, c))
+    return h->fail(BEAMS_ERR_INVALID, "degenerate camera (eye = look_at or up parallel to the view)");
+  if (!h->bricks.empty()) return render_bricks(h, c, background, rgba, depth, mem);
   if ((rc = frame_begin(h))) return rc;
   GridParams g = grid_params(h);
   ShadeParams s = shade_params(h, g);
@@ -729,30 +1103,7 @@ int beams_gen_camera_rays(const beams_camera* cam, int32_t width, int32_t height
 {
   if (!cam || width < 1 || height < 1 || !origin || !dir) return BEAMS_ERR_INVALID;
   CameraParams c{};
-  double f[3], len = 0.0;
-  for (int d = 0; d < 3; ++d) {
-    f[d] = cam->look_at[d] - cam->eye[d];
-    len += f[d] * f[d];
-  }
-  len = std::sqrt(len);
-  if (!(len > 0.0)) return BEAMS_ERR_INVALID;
-  for (int d = 0; d < 3; ++d) f[d] /= len;
-  double r[3] = { f[1] * cam->up[2] - f[2] * cam->up[1], f[2] * cam->up[0] - f[0] * cam->up[2],
-                  f[0] * cam->up[1] - f[1] * cam->up[0] };
-  double rl = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-  if (!(rl > 0.0)) return BEAMS_ERR_INVALID;
-  for (int d = 0; d < 3; ++d) r[d] /= rl;
-  double u[3] = { r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0] };
-  for (int d = 0; d < 3; ++d) {
-    c.eye[d] = cam->eye[d];
-    c.fwd[d] = f[d];
-    c.right[d] = r[d];
-    c.up[d] = u[d];
-  }
-  c.tan_half_fov = std::tan(cam->fov_y_deg * M_PI / 360.0);
-  c.aspect = (double)width / (double)height;
-  c.width = width;
-  c.height = height;
+  if (!camera_params(cam, width, height, c)) return BEAMS_ERR_INVALID;
   for (int32_t py = 0; py < height; ++py)
     for (int32_t px = 0; px < width; ++px) {
       const size_t i = (size_t)py * (size_t)width + (size_t)px;

@@ -1,0 +1,317 @@
+// Many bricks on one GPU, standing in for the reference's one block per MPI rank
+// (RenderOnDevice phases 1-4, LightedVolumeRenderer.cxx:910-1372, and
+// MapperLightedVolume::GlobalComposite, MapperLightedVolume.cxx:215-303):
+//   phase 1  k_bricks_map (local maps)              persistent, all bricks
+//   phase 2  k_bricks_exchange                       per map vertex: every other brick the
+//            light segment crosses (SegmentAABB), that brick's local opacity at the segment's
+//            exit, over-composited in exit-distance order -> seed opacity
+//   phase 3  k_bricks_map (seeded re-march)          persistent, all bricks
+//   phase 4  k_bricks_march (footprint tiles)        persistent, all bricks
+//   phase 5  k_bricks_composite (depth-ordered over) per pixel
+// The reference routes 32-byte hit records through rank 0 with MPI; here the hits of a
+// vertex live in LDS for the duration of its loop and are never written to memory.
+#include "bricks.hpp"
+#include "camera.hpp"
+#include "march_dev.hpp"
+
+namespace beams {
+
+namespace {
+
+// Fetch the next work item of a persistent workgroup (uniform across the group).
+__device__ __forceinline__ int next_item(unsigned int* counter, int* slot)
+{
+  __syncthreads();  // every thread is done with the previous item and its LDS
+  if (threadIdx.x == 0) *slot = (int)atomicAdd(counter, 1u);
+  __syncthreads();
+  return *slot;
+}
+
+__device__ __forceinline__ void stage_brick_coords(const GridParams& g, float* xs)
+{
+  const int64_t n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+  for (int64_t i = threadIdx.x; i < n0 + n1 + n2; i += blockDim.x)
+    xs[i] = (i < n0) ? g.x[0][i] : ((i < n0 + n1) ? g.x[1][i - n0] : g.x[2][i - n0 - n1]);
+}
+
+// ---- phases 1 and 3: per-brick TransmittanceMapGenerator -------------------------------------
+template <int FDT, int MDT>
+__global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
+{
+  extern __shared__ float lds[];
+  __shared__ int slot;
+  const int ntf = f.bricks[0].s.ntf;
+  float* a_tab = lds;
+  float* xs = lds + ntf;
+  unsigned int* counter = f.queue + (seeded ? 2 : 0);
+  int staged = -1;
+  for (;;) {
+    const int item = next_item(counter, &slot);
+    if (item >= f.n_map_items) break;
+    const int b = f.map_item_brick[item];
+    const BrickDesc& bd = f.bricks[b];
+    if (staged < 0)
+      for (int i = threadIdx.x; i < ntf; i += blockDim.x) a_tab[i] = bd.s.tf[4 * i + 3] / bd.s.max_density;
+    if (b != staged) stage_brick_coords(bd.g, xs);
+    staged = b;
+    __syncthreads();
+    const float* x[3] = { xs, xs + bd.g.n[0], xs + bd.g.n[0] + bd.g.n[1] };
+    const int64_t v = f.map_item_first[item] + threadIdx.x;
+    if (v < bd.nverts) {
+      int64_t taken = 0;
+      const float seed = seeded ? f.seeds[bd.seed_off + v] : 0.0f;
+      const float op = march_vertex<FDT>(bd.g, bd.m, bd.s, a_tab, x, v, seed, &taken);
+      store_t<MDT>(seeded ? bd.map_final : bd.m.map, v, op);
+      if (bd.s.counters && taken) atomicAdd(&bd.s.counters[1], (unsigned long long)taken);
+    }
+  }
+}
+
+// ---- phase 2: non-local hits (TransmittanceMap.h:342-453) + fetch (:505-508) + composite
+// (LightedVolumeRenderer.cxx:1214-1235), one thread per map vertex ------------------------------
+template <int MDT>
+__global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
+{
+  extern __shared__ unsigned long long hits[];  // [kMaxPathHits][64], (t bits << 32) | brick
+  __shared__ int slot;
+  const int lane = threadIdx.x;
+  for (;;) {
+    const int item = next_item(f.queue + 1, &slot);
+    if (item >= f.n_map_items) break;
+    const int b = f.map_item_brick[item];
+    const BrickDesc& bd = f.bricks[b];
+    for (int sub = 0; sub < 4; ++sub) {
+      const int64_t v = f.map_item_first[item] + sub * 64 + lane;
+      if (v >= bd.nverts) continue;
+      float p[3], o[3];
+      map_vertex(bd.m, v, p);
+      light_origin(bd.m, p, o);
+      const float dd[3] = { p[0] - o[0], p[1] - o[1], p[2] - o[2] };
+      const float rs = 1.0f / __builtin_sqrtf(dot3(dd, dd));  // vtkm::Normalize
+      const float dir[3] = { dd[0] * rs, dd[1] * rs, dd[2] * rs };
+      // conservative FP32 cull in the slab form, exact test (FP64 slab terms) for survivors
+      const float inv[3] = { 1.0f / dd[0], 1.0f / dd[1], 1.0f / dd[2] };
+      int nh = 0;
+      for (int j = 0; j < f.nbricks; ++j) {
+        if (j == b) continue;
+        const float* bx = f.bricks[j].box32;
+        float t0 = 0.0f, t1 = 1.0f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          float ta = (bx[2 * a] - o[a]) * inv[a], tb = (bx[2 * a + 1] - o[a]) * inv[a];
+          if (dd[a] == 0.0f) {
+            const bool in = o[a] >= bx[2 * a] && o[a] <= bx[2 * a + 1];
+            ta = in ? -__builtin_inff() : __builtin_inff();
+            tb = in ? __builtin_inff() : -__builtin_inff();
+          }
+          t0 = fmaxf(t0, fminf(ta, tb));
+          t1 = fminf(t1, fmaxf(ta, tb));
+        }
+        if (!(t0 <= t1 + 1e-3f)) continue;
+        float tmin, tmax;
+        if (!segment_aabb(o, p, f.bricks[j].g.bounds, (float)b, tmin, tmax)) continue;
+        if (nh >= kMaxPathHits) {
+          atomicOr(f.overflow, 1u);
+          break;
+        }
+        // tmax >= tmin >= 0; adding +0 turns a -0 into +0 so the bit order is the float order
+        const unsigned long long key =
+          ((unsigned long long)f32_bits(tmax + 0.0f) << 32) | (unsigned long long)(unsigned)j;
+        int k = nh++;
+        while (k > 0 && hits[(k - 1) * 64 + lane] > key) {  // insertion by (RayT, brick)
+          hits[k * 64 + lane] = hits[(k - 1) * 64 + lane];
+          --k;
+        }
+        hits[k * 64 + lane] = key;
+      }
+      float op = 0.0f;
+      for (int k = 0; k < nh; ++k) {
+        const unsigned long long key = hits[k * 64 + lane];
+        const int j = (int)(key & 0xffffffffull);
+        const float t = bits_f32((uint32_t)(key >> 32));
+        const float q[3] = { o[0] + dir[0] * t, o[1] + dir[1] * t, o[2] + dir[2] * t };
+        const float local = map_opacity<MDT>(f.bricks[j].m, q);
+        op = op + (1.0f - op) * local;
+      }
+      f.seeds[bd.seed_off + v] = op;
+    }
+  }
+}
+
+// ---- phase 4: CalcRayStart + Sampler per brick over its screen footprint ---------------------
+template <int FDT, int MDT, bool USE_MAP>
+__global__ void __launch_bounds__(256) k_bricks_march(BrickFrame f, CameraParams cam)
+{
+  extern __shared__ float4 lds4[];
+  __shared__ int slot;
+  const int ntf = f.bricks[0].s.ntf;
+  float4* tf = lds4;
+  float* xs = (float*)(lds4 + ntf);
+  int staged = -1;
+  for (;;) {
+    const int item = next_item(f.queue + 3, &slot);
+    if (item >= f.n_px_items) break;
+    const int b = f.px_item_brick[item];
+    const BrickDesc& bd = f.bricks[b];
+    if (staged < 0) stage_tf(bd.s, tf);
+    if (b != staged) stage_brick_coords(bd.g, xs);
+    staged = b;
+    __syncthreads();
+    const float* x[3] = { xs, xs + bd.g.n[0], xs + bd.g.n[0] + bd.g.n[1] };
+    const int tiles_x = (bd.fp[2] + 15) / 16;
+    const int tile = f.px_item_tile[item];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lx = (tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+    const int ly = (tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+    if (lx >= bd.fp[2] || ly >= bd.fp[3]) continue;
+    float org[3], dir[3];
+    camera_ray(cam, bd.fp[0] + lx, bd.fp[1] + ly, org, dir);
+    float minD = cam.near_d, maxD = cam.far_d, dist = 0.0f;
+    calc_ray_start(bd.g, org, dir, minD, dist, maxD);
+    float color[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+    MapParams m = bd.m;
+    m.map = bd.map_final;
+    shade<FDT, MDT, USE_MAP>(bd.g, m, bd.s, tf, x, org, dir, minD, maxD, color);
+    const int64_t pix = bd.img_off + (int64_t)ly * bd.fp[2] + lx;
+    f.partial[pix] = make_float4(color[0], color[1], color[2], color[3]);
+    f.partial_depth[pix] = (minD == -1.0f) ? __builtin_inff() : minD;
+  }
+}
+
+// ---- phase 5: GlobalComposite, front-to-back over in camera-distance order -------------------
+__global__ void __launch_bounds__(256) k_bricks_composite(BrickFrame f, int32_t width,
+                                                          int32_t height, ImageOut out)
+{
+  const int64_t pix = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (pix >= (int64_t)width * height) return;
+  const int px = (int)(pix % width), py = (int)(pix / width);
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+  float depth = __builtin_inff();
+  for (int k = 0; k < f.nbricks; ++k) {
+    const BrickDesc& bd = f.bricks[f.depth_order[k]];
+    const int lx = px - bd.fp[0], ly = py - bd.fp[1];
+    if (lx < 0 || ly < 0 || lx >= bd.fp[2] || ly >= bd.fp[3]) continue;  // contributes 0
+    const int64_t q = bd.img_off + (int64_t)ly * bd.fp[2] + lx;
+    const float4 c = f.partial[q];
+    const float ia = 1.0f - a3;
+    a0 = a0 + c.x * ia;
+    a1 = a1 + c.y * ia;
+    a2 = a2 + c.z * ia;
+    a3 = a3 + c.w * ia;
+    depth = fminf(depth, f.partial_depth[q]);
+  }
+  if (out.use_bg) {
+    const float ia = 1.0f - a3;
+    a0 = a0 + out.bg[0] * ia;
+    a1 = a1 + out.bg[1] * ia;
+    a2 = a2 + out.bg[2] * ia;
+    a3 = a3 + out.bg[3] * ia;
+  }
+  ((float4*)out.rgba)[pix] = make_float4(a0, a1, a2, a3);
+  if (out.depth) out.depth[pix] = depth;
+}
+
+__global__ void __launch_bounds__(256) k_extract(const unsigned char* src, int32_t eb, int64_t s0,
+                                                 int64_t s1, int64_t lo0, int64_t lo1, int64_t lo2,
+                                                 int64_t d0, int64_t d1, int64_t d2,
+                                                 unsigned char* dst)
+{
+  const int64_t total = d0 * d1 * d2;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * 256) {
+    const int64_t x = i % d0, y = (i / d0) % d1, z = i / (d0 * d1);
+    const int64_t si = ((z + lo2) * s1 + (y + lo1)) * s0 + (x + lo0);
+    if (eb == 2)
+      ((uint16_t*)dst)[i] = ((const uint16_t*)src)[si];
+    else
+      ((uint32_t*)dst)[i] = ((const uint32_t*)src)[si];
+  }
+}
+
+__global__ void k_zero_u32(unsigned int* p, int n)
+{
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+}  // namespace
+
+#define BRICK_DISPATCH_FIELD(V, BODY)                        \
+  switch (V) {                                               \
+    case kF32: { constexpr int FDT = kF32; BODY; } break;    \
+    case kF16: { constexpr int FDT = kF16; BODY; } break;    \
+    case kBF16: { constexpr int FDT = kBF16; BODY; } break;  \
+    default: return hipErrorInvalidValue;                    \
+  }
+#define BRICK_DISPATCH_MAP(V, BODY)                          \
+  switch (V) {                                               \
+    case kF32: { constexpr int MDT = kF32; BODY; } break;    \
+    case kF16: { constexpr int MDT = kF16; BODY; } break;    \
+    default: return hipErrorInvalidValue;                    \
+  }
+
+hipError_t launch_bricks_map(const BrickFrame& f, int32_t field_dtype, int32_t map_dtype,
+                             int32_t ntf, bool seeded, hipStream_t st)
+{
+  if (f.n_map_items == 0) return hipSuccess;
+  const size_t lds = sizeof(float) * (size_t)(ntf + f.max_coords);
+  const dim3 grid((unsigned)f.persistent_groups);
+  BRICK_DISPATCH_FIELD(field_dtype, BRICK_DISPATCH_MAP(map_dtype, {
+    hipLaunchKernelGGL((k_bricks_map<FDT, MDT>), grid, dim3(256), lds, st, f, seeded ? 1 : 0);
+  }));
+  return hipGetLastError();
+}
+
+hipError_t launch_bricks_exchange(const BrickFrame& f, int32_t map_dtype, int64_t total_verts,
+                                  hipStream_t st)
+{
+  (void)total_verts;
+  if (f.n_map_items == 0) return hipSuccess;
+  const size_t lds = sizeof(unsigned long long) * kMaxPathHits * 64;
+  const dim3 grid((unsigned)(f.persistent_groups * 4));
+  BRICK_DISPATCH_MAP(map_dtype, {
+    hipLaunchKernelGGL((k_bricks_exchange<MDT>), grid, dim3(64), lds, st, f);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_bricks_march(const BrickFrame& f, int32_t field_dtype, int32_t map_dtype,
+                               int32_t ntf, bool use_map, const CameraParams& c, hipStream_t st)
+{
+  if (f.n_px_items == 0) return hipSuccess;
+  const size_t lds = sizeof(float4) * (size_t)ntf + sizeof(float) * (size_t)f.max_coords;
+  const dim3 grid((unsigned)f.persistent_groups);
+  BRICK_DISPATCH_FIELD(field_dtype, BRICK_DISPATCH_MAP(map_dtype, {
+    if (use_map)
+      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, true>), grid, dim3(256), lds, st, f, c);
+    else
+      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, false>), grid, dim3(256), lds, st, f, c);
+  }));
+  return hipGetLastError();
+}
+
+hipError_t launch_bricks_composite(const BrickFrame& f, const CameraParams& c, const ImageOut& o,
+                                   hipStream_t st)
+{
+  const int64_t npix = (int64_t)c.width * c.height;
+  hipLaunchKernelGGL(k_bricks_composite, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, f,
+                     c.width, c.height, o);
+  return hipGetLastError();
+}
+
+hipError_t launch_extract_brick(const void* src, int32_t elem_bytes, const int64_t src_dims[3],
+                                const int64_t lo[3], const int64_t dims[3], void* dst,
+                                hipStream_t st)
+{
+  hipLaunchKernelGGL(k_extract, dim3(1024), dim3(256), 0, st, (const unsigned char*)src,
+                     elem_bytes, src_dims[0], src_dims[1], lo[0], lo[1], lo[2], dims[0], dims[1],
+                     dims[2], (unsigned char*)dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace beams

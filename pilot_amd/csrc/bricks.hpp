@@ -1,0 +1,55 @@
+// Brick (one reference block / MPI rank per brick) descriptors and launchers.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace beams {
+
+struct BrickDesc {
+  GridParams g;        // the brick's owned points; g.field points into its ghosted storage
+  MapParams m;         // m.map: the brick's phase-1 (local) map
+  ShadeParams s;       // per-brick sample distance, mesh epsilon and step bound
+  void* map_final;     // phase-3 map (seeded re-march), used by the march
+  int64_t nverts;      // map vertices
+  int64_t seed_off;    // first vertex of this brick in the seed buffer
+  int64_t img_off;     // first pixel of this brick's partial image
+  int32_t fp[4];       // screen footprint: x0, y0, width, height (pixels)
+  float box32[6];      // bounds inflated for the conservative FP32 cull
+};
+
+// Upper bound on light-path hits per map vertex kept in LDS by the exchange kernel.
+constexpr int kMaxPathHits = 64;
+
+struct BrickFrame {
+  const BrickDesc* bricks;
+  int32_t nbricks;
+  const int32_t* map_item_brick;  // per map work item: brick
+  const int64_t* map_item_first;  // per map work item: first vertex
+  int32_t n_map_items;
+  const int32_t* px_item_brick;   // per march work item: brick
+  const int32_t* px_item_tile;    // per march work item: 16x16 tile index in the footprint
+  int32_t n_px_items;
+  const int32_t* depth_order;     // bricks by ascending camera-to-centre distance
+  unsigned int* queue;            // work-queue counters: [0] map-1, [1] exchange, [2] map-3, [3] march
+  unsigned int* overflow;         // set when a light path crosses more than kMaxPathHits bricks
+  float* seeds;                   // non-local opacity per map vertex (phase 2 -> 3)
+  float4* partial;                // per-brick footprint RGBA
+  float* partial_depth;           // per-brick footprint entry distance
+  int32_t max_coords;             // LDS floats for the largest brick's coordinates
+  int32_t persistent_groups;      // workgroups of the persistent launches
+};
+
+hipError_t launch_bricks_map(const BrickFrame& f, int32_t field_dtype, int32_t map_dtype,
+                             int32_t ntf, bool seeded, hipStream_t st);
+hipError_t launch_bricks_exchange(const BrickFrame& f, int32_t map_dtype, int64_t total_verts,
+                                  hipStream_t st);
+hipError_t launch_bricks_march(const BrickFrame& f, int32_t field_dtype, int32_t map_dtype,
+                               int32_t ntf, bool use_map, const CameraParams& c, hipStream_t st);
+hipError_t launch_bricks_composite(const BrickFrame& f, const CameraParams& c, const ImageOut& o,
+                                   hipStream_t st);
+hipError_t launch_extract_brick(const void* src, int32_t elem_bytes, const int64_t src_dims[3],
+                                const int64_t lo[3], const int64_t dims[3], void* dst,
+                                hipStream_t st);
+hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t st);
+
+}  // namespace beams

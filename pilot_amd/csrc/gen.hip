@@ -221,6 +221,83 @@ int gen_field(const FieldRecipe& r, const int64_t dims[3], int32_t dtype, void* 
   return 0;
 }
 
+int gen_kd_bricks(uint64_t seed, const int64_t dims[3], int32_t nbricks, int64_t min_cells,
+                  int64_t max_cells, int64_t* ranges, std::string& err)
+{
+  struct Box {
+    int64_t lo[3], hi[3];  // point index ranges, inclusive; cells = hi - lo
+  };
+  SplitMix64 rng(seed);
+  auto pick = [&](int64_t a, int64_t b) {  // uniform integer in [a, b]
+    return a + (int64_t)(rng.uniform() * (double)(b - a + 1));
+  };
+  auto cells = [](const Box& b, int d) { return b.hi[d] - b.lo[d]; };
+  // 1) cut each axis into the fewest slabs no longer than max_cells, at jittered even
+  //    positions, and take their product
+  std::vector<int64_t> cuts[3];
+  for (int d = 0; d < 3; ++d) {
+    const int64_t e = dims[d] - 1;
+    const int64_t p = (e + max_cells - 1) / max_cells;
+    const int64_t base = e / p;
+    if (base < min_cells) {
+      err = "grid too small for the minimum brick extent";
+      return 1;
+    }
+    const int64_t jit = std::max<int64_t>(0, std::min(base - min_cells, max_cells - base - 1) / 2);
+    cuts[d].push_back(0);
+    for (int64_t i = 1; i < p; ++i) cuts[d].push_back(i * e / p + pick(-jit, jit));
+    cuts[d].push_back(e);
+  }
+  std::vector<Box> boxes;
+  for (size_t k = 0; k + 1 < cuts[2].size(); ++k)
+    for (size_t j = 0; j + 1 < cuts[1].size(); ++j)
+      for (size_t i = 0; i + 1 < cuts[0].size(); ++i)
+        boxes.push_back(Box{ { cuts[0][i], cuts[1][j], cuts[2][k] },
+                             { cuts[0][i + 1], cuts[1][j + 1], cuts[2][k + 1] } });
+  auto split = [&](size_t i, int d, int64_t at) {
+    Box a = boxes[i], b = boxes[i];
+    a.hi[d] = a.lo[d] + at;
+    b.lo[d] = a.hi[d];  // neighbours share the boundary plane of points
+    boxes[i] = a;
+    boxes.push_back(b);
+  };
+  if ((int32_t)boxes.size() > nbricks) {
+    err = "too many bricks needed to respect the maximum extent";
+    return 1;
+  }
+  // 2) split the largest-volume splittable box along its longest axis until nbricks
+  while ((int32_t)boxes.size() < nbricks) {
+    size_t best = boxes.size();
+    double bv = -1.0;
+    for (size_t i = 0; i < boxes.size(); ++i) {
+      int d = 0;
+      for (int k = 1; k < 3; ++k)
+        if (cells(boxes[i], k) > cells(boxes[i], d)) d = k;
+      if (cells(boxes[i], d) < 2 * min_cells) continue;
+      const double v = (double)cells(boxes[i], 0) * cells(boxes[i], 1) * cells(boxes[i], 2);
+      if (v > bv) {
+        bv = v;
+        best = i;
+      }
+    }
+    if (best == boxes.size()) {
+      err = "no box left to split at the minimum extent";
+      return 1;
+    }
+    int d = 0;
+    for (int k = 1; k < 3; ++k)
+      if (cells(boxes[best], k) > cells(boxes[best], d)) d = k;
+    const int64_t e = cells(boxes[best], d);
+    split(best, d, pick(min_cells, e - min_cells));
+  }
+  for (size_t i = 0; i < boxes.size(); ++i)
+    for (int d = 0; d < 3; ++d) {
+      ranges[6 * i + 2 * d] = boxes[i].lo[d];
+      ranges[6 * i + 2 * d + 1] = boxes[i].hi[d];
+    }
+  return 0;
+}
+
 void gen_colormap(uint64_t seed, int32_t npoints, int64_t n, float* rgba32, double* rgba64)
 {
   SplitMix64 rng(seed);

@@ -29,4 +29,9 @@ int gen_field(const FieldRecipe& r, const int64_t dims[3], int32_t dtype, void* 
 
 void gen_colormap(uint64_t seed, int32_t npoints, int64_t n, float* rgba32, double* rgba64);
 
+// Seeded kd decomposition of a dims grid into nbricks boxes whose cell extents lie in
+// [min_cells, max_cells]; ranges receives 6 inclusive point-index bounds per brick.
+int gen_kd_bricks(uint64_t seed, const int64_t dims[3], int32_t nbricks, int64_t min_cells,
+                  int64_t max_cells, int64_t* ranges, std::string& err);
+
 }  // namespace beams

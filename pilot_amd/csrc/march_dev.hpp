@@ -1,0 +1,299 @@
+// Device routines of the exact (reference-order FP32) path shared by the single-block
+// kernels (march.hip) and the brick kernels (bricks.hip).
+#pragma once
+
+#include "march_core.hpp"
+
+namespace beams {
+
+// Load the block's rectilinear coordinates into LDS (xs[0..n0+n1+n2)).
+__device__ __forceinline__ void stage_coords(const GridParams& g, float* xs)
+{
+  const int64_t n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+  for (int64_t i = threadIdx.x; i < n0 + n1 + n2; i += blockDim.x) {
+    float v;
+    if (i < n0)
+      v = g.x[0][i];
+    else if (i < n0 + n1)
+      v = g.x[1][i - n0];
+    else
+      v = g.x[2][i - n0 - n1];
+    xs[i] = v;
+  }
+}
+
+// ---- TransmittanceMap.h:279-331: one map vertex's light march ----------------------------
+// a_tab: alpha / max alpha per TF entry; x: the block's coordinates (LDS).  Returns the
+// opacity; *taken receives the number of steps inside the volume.
+template <int FDT>
+__device__ __forceinline__ float march_vertex(const GridParams& g, const MapParams& m,
+                                              const ShadeParams& s, const float* a_tab,
+                                              const float* const* x, int64_t id, float seed_value,
+                                              int64_t* taken_out)
+{
+  float v[3], o[3], dir[3];
+  map_vertex(m, id, v);
+  light_origin(m, v, o);
+  float dd[3] = { v[0] - o[0], v[1] - o[1], v[2] - o[2] };
+  const float rs = 1.0f / __builtin_sqrtf(dot3(dd, dd));
+  dir[0] = dd[0] * rs;
+  dir[1] = dd[1] * rs;
+  dir[2] = dd[2] * rs;
+  float tmin, tmax;
+  (void)segment_aabb(o, v, g.bounds, 1e-4f, tmin, tmax);
+  float start[3], p[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    start[d] = o[d] + tmin * dir[d];
+    p[d] = start[d];
+  }
+  const float len2 = (tmax - tmin) * (tmax - tmin);
+  const int64_t tfmax = s.ntf - 1;
+  const int64_t c0 = g.n[0] - 1, c1 = g.n[1] - 1, c2 = g.n[2] - 1;
+  const int64_t sx = g.field_stride[0], sy = g.field_stride[1], sz = g.field_stride[2];
+  float op = seed_value;
+  int64_t taken = 0;
+  for (int64_t it = 0; it < kMapMaxSteps; ++it) {
+    float df[3] = { p[0] - start[0], p[1] - start[1], p[2] - start[2] };
+    if (!(dot3(df, df) <= len2)) break;
+    p[0] = p[0] + m.step * dir[0];
+    p[1] = p[1] + m.step * dir[1];
+    p[2] = p[2] + m.step * dir[2];
+    if (!inside(g, p)) break;
+    ++taken;
+    const int64_t ci = oracle_cell(x[0], c0, p[0], g.gx0[0], g.ginv[0]);
+    const int64_t cj = oracle_cell(x[1], c1, p[1], g.gx0[1], g.ginv[1]);
+    const int64_t ck = oracle_cell(x[2], c2, p[2], g.gx0[2], g.ginv[2]);
+    float pc[3];
+    pc[0] = (p[0] - x[0][ci]) / (x[0][ci + 1] - x[0][ci]);
+    pc[1] = (p[1] - x[1][cj]) / (x[1][cj + 1] - x[1][cj]);
+    pc[2] = (p[2] - x[2][ck]) / (x[2][ck + 1] - x[2][ck]);
+    const int64_t b = ci * sx + cj * sy + ck * sz;
+    float val[8];
+    val[0] = load_t<FDT>(g.field, b);
+    val[1] = load_t<FDT>(g.field, b + sx);
+    val[2] = load_t<FDT>(g.field, b + sx + sy);
+    val[3] = load_t<FDT>(g.field, b + sy);
+    val[4] = load_t<FDT>(g.field, b + sz);
+    val[5] = load_t<FDT>(g.field, b + sz + sx);
+    val[6] = load_t<FDT>(g.field, b + sz + sx + sy);
+    val[7] = load_t<FDT>(g.field, b + sz + sy);
+    float sc = interp_hex(val, pc);
+    sc = (sc - s.min_scalar) * s.inv_delta;
+    int64_t idx = (int64_t)(sc * (float)tfmax);
+    idx = idx < tfmax ? idx : tfmax;
+    idx = idx > 0 ? idx : 0;
+    op = op + (1.0f - op) * a_tab[idx];
+    if (op == 1.0f) break;  // every later step leaves an opacity of exactly 1 unchanged
+  }
+  *taken_out = taken;
+  return op;
+}
+
+// ---- LightedVolumeRenderer.cxx:345-498 ----------------------------------------------------
+template <int FDT, int MDT, bool USE_MAP>
+__device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapParams& m,
+                                             const ShadeParams& s, const float4* tf,
+                                             const float* const* x, const float* org,
+                                             const float* dir, float minD, float maxD,
+                                             float* color)
+{
+  if (minD == -1.0f) return 0;
+  int64_t composited = 0;
+  float dist = minD + s.mesh_eps;
+  float p[3];
+  int64_t steps = 0;  // bounded by s.max_steps, which no finite in-box ray reaches
+#pragma unroll
+  for (int d = 0; d < 3; ++d) p[d] = org[d] + dist * dir[d];
+  while (!inside(g, p) && dist < maxD && ++steps < s.max_steps) {
+    dist += s.sample_dist;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) p[d] = org[d] + dist * dir[d];
+  }
+  float bl[3] = { 0.0f, 0.0f, 0.0f };
+  bool newCell = true;
+  float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+  float s0 = 0.0f, s10 = 0.0f, s23 = 0.0f, s3 = 0.0f, s4 = 0.0f, s54 = 0.0f, s67 = 0.0f,
+        s7 = 0.0f;
+  int64_t cell[3] = { 0, 0, 0 };
+  float invsp[3] = { 0.0f, 0.0f, 0.0f };
+  const int64_t cms = s.ntf - 1;
+  const int64_t sx = g.field_stride[0], sy = g.field_stride[1], sz = g.field_stride[2];
+  const float sd = s.sample_dist;
+  while (inside(g, p) && dist < maxD && ++steps < s.max_steps) {
+    const float mint = fminf(tx, fminf(ty, tz));
+    const float maxt = fmaxf(tx, fmaxf(ty, tz));
+    if (maxt > 1.0f || mint < 0.0f) newCell = true;
+    if (newCell) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (p[d] == g.maxp[d]) {
+          cell[d] = g.n[d] - 2;  // invSpacing keeps its previous value
+        } else {
+          const int64_t c = walk_cell(x[d], g.n[d] - 1, p[d], g.gx0[d], g.ginv[d]);
+          invsp[d] = 1.0f / (x[d][c + 1] - x[d][c]);
+          cell[d] = c;
+        }
+      }
+      const int64_t b = cell[0] * sx + cell[1] * sy + cell[2] * sz;
+      bl[0] = x[0][cell[0]];
+      bl[1] = x[1][cell[1]];
+      bl[2] = x[2][cell[2]];
+      s0 = load_t<FDT>(g.field, b);
+      const float s1 = load_t<FDT>(g.field, b + sx);
+      const float s2 = load_t<FDT>(g.field, b + sx + sy);
+      s3 = load_t<FDT>(g.field, b + sy);
+      s4 = load_t<FDT>(g.field, b + sz);
+      const float s5 = load_t<FDT>(g.field, b + sz + sx);
+      const float s6 = load_t<FDT>(g.field, b + sz + sx + sy);
+      s7 = load_t<FDT>(g.field, b + sz + sy);
+      s67 = s6 - s7;
+      s54 = s5 - s4;
+      s10 = s1 - s0;
+      s23 = s2 - s3;
+      tx = (p[0] - bl[0]) * invsp[0];
+      ty = (p[1] - bl[1]) * invsp[1];
+      tz = (p[2] - bl[2]) * invsp[2];
+      newCell = false;
+    }
+    const float l76 = s7 + tx * s67;
+    const float l45 = s4 + tx * s54;
+    const float top = l45 + ty * (l76 - l45);
+    const float l01 = s0 + tx * s10;
+    const float l32 = s3 + tx * s23;
+    const float bot = l01 + ty * (l32 - l01);
+    float fs = bot + tz * (top - bot);
+    fs = (fs - s.min_scalar) * s.inv_delta;
+    const int64_t ci = (int64_t)(fs * (float)cms);
+    // The reference re-evaluates the same sample forever for an index outside the
+    // table; here the sample is skipped and the ray advances.
+    if (ci >= 0 && ci <= cms) {
+      float4 c = tf[ci];
+      c.w *= (1.0f - color[3]);
+      if (USE_MAP) {
+        const float tr = 1.0f - map_opacity<MDT>(m, p);
+        c.x = c.x * (tr * m.light_color[0]);
+        c.y = c.y * (tr * m.light_color[1]);
+        c.z = c.z * (tr * m.light_color[2]);
+      }
+      c.x = c.x < 0.0f ? 0.0f : (c.x > 1.0f ? 1.0f : c.x);
+      c.y = c.y < 0.0f ? 0.0f : (c.y > 1.0f ? 1.0f : c.y);
+      c.z = c.z < 0.0f ? 0.0f : (c.z > 1.0f ? 1.0f : c.z);
+      color[0] = color[0] + c.x * c.w;
+      color[1] = color[1] + c.y * c.w;
+      color[2] = color[2] + c.z * c.w;
+      color[3] = c.w + color[3];
+      ++composited;
+    }
+    dist += sd;
+    p[0] = p[0] + sd * dir[0];
+    p[1] = p[1] + sd * dir[1];
+    p[2] = p[2] + sd * dir[2];
+    tx = (p[0] - bl[0]) * invsp[0];
+    ty = (p[1] - bl[1]) * invsp[1];
+    tz = (p[2] - bl[2]) * invsp[2];
+    if (color[3] >= s.alpha_stop) break;
+  }
+  return composited;
+}
+
+// ---- LightedVolumeRenderer.cxx:540-641 (cell-associated scalars, no shadow term) --------
+template <int FDT>
+__device__ __forceinline__ int64_t sample_cell(const GridParams& g, const ShadeParams& s,
+                                            const float4* tf, const float* const* x,
+                                            const float* org, const float* dir, float minD,
+                                            float maxD, float* color)
+{
+  if (minD == -1.0f) return 0;
+  int64_t composited = 0;
+  float dist = minD + s.mesh_eps;
+  float p[3];
+  int64_t steps = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) p[d] = org[d] + dist * dir[d];
+  while (!inside(g, p) && dist < maxD && ++steps < s.max_steps) {
+    dist += s.sample_dist;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) p[d] = org[d] + dist * dir[d];
+  }
+  bool newCell = true;
+  float tx = 2.0f, ty = 2.0f, tz = 2.0f;
+  float4 sc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float bl[3] = { 0.0f, 0.0f, 0.0f };
+  float invsp[3] = { 0.0f, 0.0f, 0.0f };
+  int64_t cell[3] = { 0, 0, 0 };
+  const int64_t cms = s.ntf - 1;
+  const float sd = s.sample_dist;
+  while (inside(g, p) && dist < maxD && ++steps < s.max_steps) {
+    const float mint = fminf(tx, fminf(ty, tz));
+    const float maxt = fmaxf(tx, fmaxf(ty, tz));
+    if (maxt > 1.0f || mint < 0.0f) newCell = true;
+    if (newCell) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (p[d] == g.maxp[d]) {
+          cell[d] = g.n[d] - 2;
+        } else {
+          const int64_t c = walk_cell(x[d], g.n[d] - 1, p[d], g.gx0[d], g.ginv[d]);
+          invsp[d] = 1.0f / (x[d][c + 1] - x[d][c]);
+          cell[d] = c;
+        }
+      }
+      const int64_t cid = cell[0] * g.field_stride[0] + cell[1] * g.field_stride[1] +
+                          cell[2] * g.field_stride[2];
+      const float v = load_t<FDT>(g.field, cid);
+      const float ns = (v - s.min_scalar) * s.inv_delta;
+      int64_t ci = (int64_t)(ns * (float)cms);
+      if (ci < 0) ci = 0;
+      if (ci > cms) ci = cms;
+      sc = tf[ci];
+      bl[0] = x[0][cell[0]];
+      bl[1] = x[1][cell[1]];
+      bl[2] = x[2][cell[2]];
+      tx = (p[0] - bl[0]) * invsp[0];
+      ty = (p[1] - bl[1]) * invsp[1];
+      tz = (p[2] - bl[2]) * invsp[2];
+      newCell = false;
+    }
+    const float a = sc.w * (1.0f - color[3]);
+    color[0] = color[0] + sc.x * a;
+    color[1] = color[1] + sc.y * a;
+    color[2] = color[2] + sc.z * a;
+    color[3] = a + color[3];
+    ++composited;
+    dist += sd;
+    p[0] = p[0] + sd * dir[0];
+    p[1] = p[1] + sd * dir[1];
+    p[2] = p[2] + sd * dir[2];
+    if (color[3] >= s.alpha_stop) break;
+    tx = (p[0] - bl[0]) * invsp[0];
+    ty = (p[1] - bl[1]) * invsp[1];
+    tz = (p[2] - bl[2]) * invsp[2];
+  }
+  color[0] = fminf(color[0], 1.0f);
+  color[1] = fminf(color[1], 1.0f);
+  color[2] = fminf(color[2], 1.0f);
+  color[3] = fminf(color[3], 1.0f);
+  return composited;
+}
+
+__device__ __forceinline__ void stage_tf(const ShadeParams& s, float4* tf)
+{
+  for (int i = threadIdx.x; i < s.ntf; i += blockDim.x)
+    tf[i] = make_float4(s.tf[4 * i], s.tf[4 * i + 1], s.tf[4 * i + 2], s.tf[4 * i + 3]);
+}
+
+template <int FDT, int MDT, bool USE_MAP>
+__device__ __forceinline__ void shade(const GridParams& g, const MapParams& m,
+                                      const ShadeParams& s, const float4* tf,
+                                      const float* const* x, const float* org, const float* dir,
+                                      float minD, float maxD, float* color)
+{
+  const int64_t n = (g.assoc == 1)
+                      ? sample_cell<FDT>(g, s, tf, x, org, dir, minD, maxD, color)
+                      : sample_point<FDT, MDT, USE_MAP>(g, m, s, tf, x, org, dir, minD, maxD, color);
+  if (s.counters && n) atomicAdd(&s.counters[0], (unsigned long long)n);
+}
+
+
+}  // namespace beams

@@ -322,6 +322,73 @@ def test_calc_ray_start_slab(oracle_mod):
     del vol
 
 
+# ---- non-local hits (TransmittanceMap.h:342-453) --------------------------------------------
+
+def test_nonlocal_hits_brute_force(oracle_mod):
+    """For every map vertex of a brick, the other bricks its light segment crosses and the
+    exit distances equal what dense sampling of the segment finds."""
+    import ctypes as C
+
+    from pilot_amd import beams as B
+    from pilot_amd import scenes
+
+    O = oracle_mod
+    sc = scenes.make_scene("nl", 25, seed=3, nblobs=4, light="point")
+    ranges = B.kd_bricks(sc.dims, 6, 2, 4, 24)
+    vols = O.brick_volumes(sc, ranges, twin="f64")
+    arr = (O.OrcScene * len(vols))(*[v.s for v in vols])
+    light = np.array([float(x) for x in sc.light_pos])
+    boxes = [np.array([v.coords[0][0], v.coords[0][-1], v.coords[1][0], v.coords[1][-1],
+                       v.coords[2][0], v.coords[2][-1]]) for v in vols]
+    L = O.lib()
+    checked = 0
+    for self_b in (0, 3):
+        v = vols[self_b]
+        nv = v.nverts
+        counts = np.zeros(nv, np.int64)
+        offs = np.zeros(nv, np.int64)
+        total = L.orc_nonlocal_hits_f64(len(vols), arr, self_b, counts.ctypes.data, offs.ctypes.data,
+                                        None, 0)
+        hits = np.zeros(max(total, 1), dtype=[("ray", "i4"), ("blk", "i4"), ("frm", "i4"), ("pad", "i4"),
+                                              ("pt", "f8", 3), ("t", "f8"), ("op", "f8")])
+        L.orc_nonlocal_hits_f64(len(vols), arr, self_b, counts.ctypes.data, offs.ctypes.data,
+                                hits.ctypes.data, C.c_int64(total))
+        mc = v.map_cells
+        g = [np.linspace(v.coords[d][0], v.coords[d][-1], mc[d] + 1) for d in range(3)]
+        rng = np.random.default_rng(self_b)
+        for vid in rng.choice(nv, 60, replace=False):
+            i, j, k = vid % (mc[0] + 1), (vid // (mc[0] + 1)) % (mc[1] + 1), vid // ((mc[0] + 1) * (mc[1] + 1))
+            p = np.array([g[0][i], g[1][j], g[2][k]])
+            seg = p - light
+            Ls = np.linalg.norm(seg)
+            ts = np.linspace(0, Ls, 40001)
+            pts = light[None, :] + ts[:, None] * (seg / Ls)[None, :]
+            expect = {}
+            for b, bx in enumerate(boxes):
+                if b == self_b:
+                    continue
+                inside = np.all((pts >= bx[0::2] - 1e-12) & (pts <= bx[1::2] + 1e-12), axis=1)
+                if inside.any():
+                    expect[b] = (ts[inside][0], ts[inside][-1])
+            got = {int(h["blk"]): float(h["t"]) for h in hits[offs[vid]:offs[vid] + counts[vid]]}
+            tol = 2 * Ls / 40000
+            for b in set(expect) | set(got):
+                if b in expect and b in got:
+                    assert abs(expect[b][1] - got[b]) <= tol + 1e-12
+                elif b in expect:
+                    # a contact of zero length (the segment's end on a shared face) may round
+                    # either way in the slab test
+                    assert expect[b][1] - expect[b][0] <= tol
+                # a hit the sampling missed can only be such a contact as well
+                else:
+                    assert b in got
+            for h in hits[offs[vid]:offs[vid] + counts[vid]]:
+                assert h["ray"] == vid and h["frm"] == self_b
+                assert np.allclose(h["pt"], light + h["t"] * seg / Ls, atol=1e-12)
+            checked += 1
+    assert checked == 120
+
+
 # ---- GlobalComposite (MapperLightedVolume.cxx:215-303) --------------------------------------
 
 def test_composite_depth_order_and_over(oracle_mod):
