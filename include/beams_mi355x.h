@@ -64,7 +64,8 @@ typedef struct beams_renderer beams_renderer;
 #define BEAMS_PHASE_REMAP 2     /* phase 3: seeded map re-march (bricks)      */
 #define BEAMS_PHASE_MARCH 3     /* phase 4: CalcRayStart + Sampler            */
 #define BEAMS_PHASE_COMPOSITE 4 /* phase 5: brick composite + background      */
-#define BEAMS_NUM_PHASES 5
+#define BEAMS_PHASE_FRAME 5     /* the whole frame (also with graph replay)   */
+#define BEAMS_NUM_PHASES 6
 
 /* Pinhole camera for beams_render_image.  Replaces the vtkm::rendering::Camera ->
  * raytracing::Camera::CreateRays step (MapperLightedVolume.cxx:141-152). */
@@ -117,6 +118,15 @@ int beams_get_brick_map(beams_renderer* h, int32_t brick, int32_t which, float* 
 
 /* Workgroups of the persistent brick launches (a schedule knob; default 2048). */
 int beams_set_persistent_groups(beams_renderer* h, int32_t groups);
+
+/* Frames of beams_render_image replay a hipGraph captured from their launches; the graph
+ * is captured again whenever a kernel argument changes (camera, outputs, scene state). */
+int beams_set_graph(beams_renderer* h, int32_t enabled);
+
+/* Times `frames` frames of beams_render_image for each candidate schedule (the persistent
+ * workgroup count of the brick launches), keeps the fastest and reports it. */
+int beams_autotune(beams_renderer* h, const beams_camera* cam, int32_t width, int32_t height,
+                   int32_t frames, int32_t* best_groups, float* best_ms);
 
 /* scalarRange argument of SetData (vtkm::Range). */
 int beams_set_scalar_range(beams_renderer* h, double min_value, double max_value);

@@ -19,7 +19,7 @@ MEM_HOST, MEM_DEVICE = 0, 1
 ASSOC_POINTS, ASSOC_CELLS = 0, 1
 LIGHT_POINT, LIGHT_DIRECTIONAL = 0, 1
 SHADOW_REF_MARCH = 0
-PHASES = ("map", "exchange", "remap", "march", "composite")
+PHASES = ("map", "exchange", "remap", "march", "composite", "frame")
 
 STATUS = {
     0: "BEAMS_OK",
@@ -77,6 +77,8 @@ SIGNATURES = {
     "beams_set_bricks": (C.c_int, [_P, _I32, _PI64, _I32, _PI64]),
     "beams_get_brick_map": (C.c_int, [_P, _I32, _I32, _PF]),
     "beams_set_persistent_groups": (C.c_int, [_P, _I32]),
+    "beams_set_graph": (C.c_int, [_P, _I32]),
+    "beams_autotune": (C.c_int, [_P, C.POINTER(Camera), _I32, _I32, _I32, C.POINTER(C.c_int32), _PF]),
     "beams_set_scalar_range": (C.c_int, [_P, _D, _D]),
     "beams_set_colormap": (C.c_int, [_P, _PF, _I64]),
     "beams_set_light": (C.c_int, [_P, _I32, _PF, _PF, _F]),
@@ -319,6 +321,16 @@ class LightedVolumeRenderer:
 
     def SetPersistentGroups(self, groups):
         self._c(lib().beams_set_persistent_groups(self._h, int(groups)))
+
+    def SetGraph(self, enabled):
+        self._c(lib().beams_set_graph(self._h, int(bool(enabled))))
+
+    def Autotune(self, camera, width, height, frames=3):
+        g = C.c_int32()
+        ms = C.c_float()
+        self._c(lib().beams_autotune(self._h, C.byref(camera), width, height, frames, C.byref(g),
+                                     C.byref(ms)))
+        return g.value, ms.value
 
     def GetTimingStats(self):
         t = np.zeros(len(PHASES), np.float32)

@@ -111,6 +111,11 @@ struct beams_renderer {
   std::vector<unsigned char> btab_host;  // tables of the last frame (reused while unchanged)
   int32_t n_map_items = 0, n_px_items = 0, max_coords = 0;
   int32_t persistent_groups = 2048;
+  int64_t btab_version = 0;
+  // hipGraph of the frame's launches, replayed while its key (every kernel argument) holds
+  bool use_graph = false;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<unsigned char> graph_key;
   bool overflow_checked = true;
 
   int fail(int code, const std::string& msg)
@@ -465,6 +470,7 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
     HIP_TRY(h, h->btables.ensure(tab.size()));
     HIP_TRY(h, hipMemcpy(h->btables.p, tab.data(), tab.size(), hipMemcpyHostToDevice));
     h->btab_host.swap(tab);
+    ++h->btab_version;
   }
   unsigned char* base = (unsigned char*)h->btables.p;
   f = BrickFrame{};
@@ -488,6 +494,50 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
   h->n_px_items = f.n_px_items;
   h->max_coords = max_coords;
   return BEAMS_OK;
+}
+
+struct KeyBuilder {
+  std::vector<unsigned char> bytes;
+  template <typename T>
+  KeyBuilder& put(const T& v)
+  {
+    const unsigned char* p = (const unsigned char*)&v;
+    bytes.insert(bytes.end(), p, p + sizeof(T));
+    return *this;
+  }
+};
+
+// Runs the frame's launches `issue(timed)` eagerly, or as a replay of a hipGraph captured
+// from them; the graph is re-captured whenever the key (all kernel arguments) changes.
+template <typename Issue>
+int run_frame(beams_renderer* h, const std::vector<unsigned char>& key, Issue&& issue)
+{
+  int rc;
+  if (!h->use_graph) {
+    if ((rc = phase_mark(h, BEAMS_PHASE_FRAME, false))) return rc;
+    if ((rc = issue(true))) return rc;
+    return phase_mark(h, BEAMS_PHASE_FRAME, true);
+  }
+  if (!h->graph_exec || key != h->graph_key) {
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    h->graph_exec = nullptr;
+    HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    rc = issue(false);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+    if (rc) {
+      if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    HIP_TRY(h, e);
+    const hipError_t ei = hipGraphInstantiate(&h->graph_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_TRY(h, ei);
+    h->graph_key = key;
+  }
+  if ((rc = phase_mark(h, BEAMS_PHASE_FRAME, false))) return rc;
+  HIP_TRY(h, hipGraphLaunch(h->graph_exec, h->stream));
+  return phase_mark(h, BEAMS_PHASE_FRAME, true);
 }
 
 int check_overflow(beams_renderer* h)
@@ -522,23 +572,6 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
   if ((rc = prepare_bricks(h, c, f, total_verts, partial_pixels))) return rc;
   const int32_t ntf = (int32_t)(h->htf.size() / 4);
   const bool shadows = h->use_shadow;
-  HIP_TRY(h, launch_zero_u32(f.queue, 5, h->stream));
-  if (shadows) {
-    if ((rc = phase_mark(h, BEAMS_PHASE_MAP, false))) return rc;
-    HIP_TRY(h, launch_bricks_map(f, h->dtype, h->map_dtype, ntf, false, h->stream));
-    if ((rc = phase_mark(h, BEAMS_PHASE_MAP, true))) return rc;
-    if ((rc = phase_mark(h, BEAMS_PHASE_EXCHANGE, false))) return rc;
-    HIP_TRY(h, launch_bricks_exchange(f, h->map_dtype, total_verts, h->stream));
-    if ((rc = phase_mark(h, BEAMS_PHASE_EXCHANGE, true))) return rc;
-    if ((rc = phase_mark(h, BEAMS_PHASE_REMAP, false))) return rc;
-    HIP_TRY(h, launch_bricks_map(f, h->dtype, h->map_dtype, ntf, true, h->stream));
-    if ((rc = phase_mark(h, BEAMS_PHASE_REMAP, true))) return rc;
-    h->map_valid = true;
-    h->overflow_checked = false;
-  }
-  if ((rc = phase_mark(h, BEAMS_PHASE_MARCH, false))) return rc;
-  HIP_TRY(h, launch_bricks_march(f, h->dtype, h->map_dtype, ntf, shadows, c, h->stream));
-  if ((rc = phase_mark(h, BEAMS_PHASE_MARCH, true))) return rc;
   const size_t npix = (size_t)c.width * (size_t)c.height;
   ImageOut o{};
   if (mem == BEAMS_MEM_DEVICE) {
@@ -551,9 +584,35 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
   }
   o.use_bg = background ? 1 : 0;
   for (int k = 0; k < 4; ++k) o.bg[k] = background ? background[k] : 0.0f;
-  if ((rc = phase_mark(h, BEAMS_PHASE_COMPOSITE, false))) return rc;
-  HIP_TRY(h, launch_bricks_composite(f, c, o, h->stream));
-  if ((rc = phase_mark(h, BEAMS_PHASE_COMPOSITE, true))) return rc;
+  auto issue = [&](bool timed) -> int {
+    int r;
+    auto mark = [&](int p, bool end) { return timed ? phase_mark(h, p, end) : BEAMS_OK; };
+    HIP_TRY(h, launch_zero_u32(f.queue, 5, h->stream));
+    if (shadows) {
+      if ((r = mark(BEAMS_PHASE_MAP, false))) return r;
+      HIP_TRY(h, launch_bricks_map(f, h->dtype, h->map_dtype, ntf, false, h->stream));
+      if ((r = mark(BEAMS_PHASE_MAP, true))) return r;
+      if ((r = mark(BEAMS_PHASE_EXCHANGE, false))) return r;
+      HIP_TRY(h, launch_bricks_exchange(f, h->map_dtype, total_verts, h->stream));
+      if ((r = mark(BEAMS_PHASE_EXCHANGE, true))) return r;
+      if ((r = mark(BEAMS_PHASE_REMAP, false))) return r;
+      HIP_TRY(h, launch_bricks_map(f, h->dtype, h->map_dtype, ntf, true, h->stream));
+      if ((r = mark(BEAMS_PHASE_REMAP, true))) return r;
+    }
+    if ((r = mark(BEAMS_PHASE_MARCH, false))) return r;
+    HIP_TRY(h, launch_bricks_march(f, h->dtype, h->map_dtype, ntf, shadows, c, h->stream));
+    if ((r = mark(BEAMS_PHASE_MARCH, true))) return r;
+    if ((r = mark(BEAMS_PHASE_COMPOSITE, false))) return r;
+    HIP_TRY(h, launch_bricks_composite(f, c, o, h->stream));
+    return mark(BEAMS_PHASE_COMPOSITE, true);
+  };
+  KeyBuilder key;
+  key.put(1).put(f).put(c).put(o).put(h->btab_version).put(shadows).put(h->dtype).put(h->map_dtype).put(ntf);
+  if ((rc = run_frame(h, key.bytes, issue))) return rc;
+  if (shadows) {
+    h->map_valid = true;
+    h->overflow_checked = false;
+  }
   if (mem == BEAMS_MEM_HOST) {
     HIP_TRY(h, hipMemcpyAsync(rgba, o.rgba, sizeof(float) * 4 * npix, hipMemcpyDeviceToHost, h->stream));
     if (depth) HIP_TRY(h, hipMemcpyAsync(depth, o.depth, sizeof(float) * npix, hipMemcpyDeviceToHost, h->stream));
@@ -602,6 +661,14 @@ int beams_destroy(beams_renderer* h)
     for (auto& e : f.ev)
       if (e) (void)hipEventDestroy(e);
   h->counters.release();
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  for (auto& br : h->bricks) br.field.release();
+  h->bmap_local.release();
+  h->bmap_final.release();
+  h->bseeds.release();
+  h->btables.release();
+  h->bpartial.release();
+  h->bqueue.release();
   (void)hipStreamDestroy(h->stream);
   delete h;
   return BEAMS_OK;
@@ -746,6 +813,68 @@ int beams_set_persistent_groups(beams_renderer* h, int32_t groups)
   if (groups < 1 || groups > 65536) return h->fail(BEAMS_ERR_INVALID, "persistent groups must be in [1, 65536]");
   h->persistent_groups = groups;
   return BEAMS_OK;
+}
+
+int beams_set_graph(beams_renderer* h, int32_t enabled)
+{
+  CHECK_HANDLE(h);
+  h->use_graph = enabled != 0;
+  if (!h->use_graph && h->graph_exec) {
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    (void)hipGraphExecDestroy(h->graph_exec);
+    h->graph_exec = nullptr;
+    h->graph_key.clear();
+  }
+  return BEAMS_OK;
+}
+
+int beams_autotune(beams_renderer* h, const beams_camera* cam, int32_t width, int32_t height,
+                   int32_t frames, int32_t* best_groups, float* best_ms)
+{
+  CHECK_HANDLE(h);
+  if (!cam || width < 1 || height < 1 || frames < 1 || frames > 1000)
+    return h->fail(BEAMS_ERR_INVALID, "autotune: bad camera, size or frame count");
+  const size_t npix = (size_t)width * (size_t)height;
+  HIP_TRY(h, h->image.ensure(sizeof(float) * npix * 5));
+  float* img = (float*)h->image.p;
+  std::vector<int32_t> cand;
+  if (h->bricks.empty())
+    cand.push_back(h->persistent_groups);
+  else
+    cand = { 256, 512, 1024, 2048, 4096, 8192 };
+  hipEvent_t e0, e1;
+  HIP_TRY(h, hipEventCreate(&e0));
+  HIP_TRY(h, hipEventCreate(&e1));
+  float best = INFINITY;
+  int32_t bg = h->persistent_groups;
+  int rc = BEAMS_OK;
+  for (int32_t g : cand) {
+    h->persistent_groups = g;
+    if ((rc = beams_render_image(h, cam, width, height, nullptr, img, img + 4 * npix, BEAMS_MEM_DEVICE)))
+      break;
+    (void)hipEventRecord(e0, h->stream);
+    for (int32_t k = 0; k < frames && !rc; ++k)
+      rc = beams_render_image(h, cam, width, height, nullptr, img, img + 4 * npix, BEAMS_MEM_DEVICE);
+    if (rc) break;
+    (void)hipEventRecord(e1, h->stream);
+    float ms = 0.0f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = h->fail(BEAMS_ERR_HIP, "autotune: event timing failed");
+      break;
+    }
+    ms /= (float)frames;
+    if (ms < best) {
+      best = ms;
+      bg = g;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h->persistent_groups = bg;
+  if (rc) return rc;
+  if (best_groups) *best_groups = bg;
+  if (best_ms) *best_ms = best;
+  return beams_reset_timings(h);
 }
 
 int beams_gen_kd_bricks(uint64_t seed, const int64_t dims[3], int32_t nbricks, int64_t min_cells,
@@ -926,10 +1055,10 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
   if ((rc = frame_begin(h))) return rc;
   GridParams g = grid_params(h);
   ShadeParams s = shade_params(h, g);
-  if (h->use_shadow && h->assoc == 0) {
-    if ((rc = phase_mark(h, BEAMS_PHASE_MAP, false))) return rc;
-    if ((rc = build_map(h, g, s))) return rc;
-    if ((rc = phase_mark(h, BEAMS_PHASE_MAP, true))) return rc;
+  const bool with_map = h->use_shadow && h->assoc == 0;
+  if (with_map) {
+    const int64_t nv = (h->map_cells[0] + 1) * (h->map_cells[1] + 1) * (h->map_cells[2] + 1);
+    HIP_TRY(h, h->map.ensure((size_t)nv * dtype_size(h->map_dtype)));
   }
   const size_t npix = (size_t)width * (size_t)height;
   ImageOut o{};
@@ -943,10 +1072,23 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
   }
   o.use_bg = background ? 1 : 0;
   for (int k = 0; k < 4; ++k) o.bg[k] = background ? background[k] : 0.0f;
-  MapParams m = map_params(h, g);
-  if ((rc = phase_mark(h, BEAMS_PHASE_MARCH, false))) return rc;
-  HIP_TRY(h, launch_march_image(g, m, s, c, o, h->stream));
-  if ((rc = phase_mark(h, BEAMS_PHASE_MARCH, true))) return rc;
+  const MapParams m = map_params(h, g);
+  auto issue = [&](bool timed) -> int {
+    int r;
+    auto mark = [&](int p, bool end) { return timed ? phase_mark(h, p, end) : BEAMS_OK; };
+    if (with_map) {
+      if ((r = mark(BEAMS_PHASE_MAP, false))) return r;
+      HIP_TRY(h, launch_tmap_ref(g, m, s, nullptr, h->stream));
+      if ((r = mark(BEAMS_PHASE_MAP, true))) return r;
+    }
+    if ((r = mark(BEAMS_PHASE_MARCH, false))) return r;
+    HIP_TRY(h, launch_march_image(g, m, s, c, o, h->stream));
+    return mark(BEAMS_PHASE_MARCH, true);
+  };
+  KeyBuilder key;
+  key.put(0).put(g).put(m).put(s).put(c).put(o).put(with_map);
+  if ((rc = run_frame(h, key.bytes, issue))) return rc;
+  if (with_map) h->map_valid = true;
   if (mem == BEAMS_MEM_HOST) {
     HIP_TRY(h, hipMemcpyAsync(rgba, o.rgba, sizeof(float) * 4 * npix, hipMemcpyDeviceToHost, h->stream));
     if (depth) HIP_TRY(h, hipMemcpyAsync(depth, o.depth, sizeof(float) * npix, hipMemcpyDeviceToHost, h->stream));
