@@ -57,6 +57,9 @@ typedef struct beams_renderer beams_renderer;
 
 /* shadow-map construction */
 #define BEAMS_SHADOW_REF_MARCH 0 /* per-vertex light march, TransmittanceMap.h:239-340 */
+#define BEAMS_SHADOW_SWEEP 1     /* exact sweep: light-aligned slices carrying optical depth
+                                    (one pass; needs the light outside the bounds' slab on
+                                    one axis; a re-discretisation, not bit-equal) */
 
 /* phases reported by beams_get_timings (Beams.cxx:30-34 Phase1..5Time) */
 #define BEAMS_PHASE_MAP 0       /* phase 1: transmittance map build           */

@@ -5,6 +5,7 @@
 #include "../../include/beams_mi355x.h"
 #include "bricks.hpp"
 #include "camera.hpp"
+#include "sweep.hpp"
 #include "gen.hpp"
 #include "kernels.hpp"
 
@@ -79,6 +80,7 @@ struct beams_renderer {
   int32_t shadow_mode = BEAMS_SHADOW_REF_MARCH;
   int32_t map_dtype = BEAMS_F32;
   DevBuf map;
+  DevBuf sweep_slices;  // (tau, sigma) ping-pong slices of the exact sweep
   bool map_valid = false;
   // march
   float sample_distance = -1.0f;
@@ -177,6 +179,33 @@ GridParams grid_params(const beams_renderer* h)
 }
 
 // CreateDataSetForOpacityMap + GenerateEstimator set-up (TransmittanceMap.h:550-621).
+// Sweep geometry: the axis along which the light crosses the map (the light must be
+// outside the bounds' slab on that axis), its direction and the transverse axes.
+bool sweep_axis(const beams_renderer* h, int& axis, int& a1, int& a2, int& step)
+{
+  if (h->light_kind == BEAMS_LIGHT_DIRECTIONAL) {
+    axis = 0;
+    for (int d = 1; d < 3; ++d)
+      if (std::fabs(h->light[d]) > std::fabs(h->light[axis])) axis = d;
+    step = h->light[axis] > 0.0f ? 1 : -1;
+  } else {
+    double best = 0.0;
+    axis = -1;
+    for (int d = 0; d < 3; ++d) {
+      const double out = std::max((double)h->bounds[2 * d] - h->light[d], (double)h->light[d] - h->bounds[2 * d + 1]);
+      if (out > best) {
+        best = out;
+        axis = d;
+      }
+    }
+    if (axis < 0) return false;
+    step = (double)h->light[axis] < h->bounds[2 * axis] ? 1 : -1;
+  }
+  a1 = axis == 0 ? 1 : 0;
+  a2 = axis == 2 ? 1 : 2;
+  return true;
+}
+
 MapParams map_params(const beams_renderer* h, const GridParams& g)
 {
   MapParams m{};
@@ -192,6 +221,15 @@ MapParams map_params(const beams_renderer* h, const GridParams& g)
     m.minv[d] = 1.0f / m.msp[d];
     m.light[d] = h->light[d];
     m.light_color[d] = h->light_color[d];
+  }
+  m.ms[0] = 1;
+  m.ms[1] = m.mp[0];
+  m.ms[2] = m.mp[0] * m.mp[1];
+  int axis, a1, a2, step;
+  if (h->shadow_mode == BEAMS_SHADOW_SWEEP && sweep_axis(h, axis, a1, a2, step)) {
+    m.ms[a1] = 1;  // slice-contiguous layout, the sweep axis slowest
+    m.ms[a2] = m.mp[a1];
+    m.ms[axis] = m.mp[a1] * m.mp[a2];
   }
   m.step = std::sqrt(size[0] * size[0] + size[1] * size[1] + size[2] * size[2]) / 128.0f;
   m.dir_far = 4.0f * g.ext_mag;
@@ -295,12 +333,48 @@ int phase_mark(beams_renderer* h, int phase, bool end)
 }
 
 // Phase 1 (and 3, which equals phase 1 for a single block): build the map.
-int build_map(beams_renderer* h, const GridParams& g, const ShadeParams& s)
+// Device buffers of the single-block map build (allocated outside any graph capture).
+int prepare_map(beams_renderer* h)
 {
   const int64_t nv = (h->map_cells[0] + 1) * (h->map_cells[1] + 1) * (h->map_cells[2] + 1);
   HIP_TRY(h, h->map.ensure((size_t)nv * dtype_size(h->map_dtype)));
+  if (h->shadow_mode == BEAMS_SHADOW_SWEEP) {
+    int axis, a1, a2, step;
+    if (!sweep_axis(h, axis, a1, a2, step))
+      return h->fail(BEAMS_ERR_UNSUPPORTED, "the sweep needs a light outside the volume's bounds");
+    const size_t ns = (size_t)(h->map_cells[a1] + 1) * (size_t)(h->map_cells[a2] + 1);
+    HIP_TRY(h, h->sweep_slices.ensure(2 * ns * sizeof(float2)));
+  }
+  return BEAMS_OK;
+}
+
+// Enqueues the map build: the reference's per-vertex march, or the exact sweep.
+int enqueue_map(beams_renderer* h, const GridParams& g, const MapParams& m, const ShadeParams& s)
+{
+  if (h->shadow_mode == BEAMS_SHADOW_SWEEP) {
+    SweepParams sp{};
+    sp.g = g;
+    sp.m = m;
+    sp.s = s;
+    sweep_axis(h, sp.axis, sp.a1, sp.a2, sp.step);
+    sp.h = m.step;
+    sp.inv_h = 1.0f / m.step;
+    const size_t ns = (size_t)m.mp[sp.a1] * (size_t)m.mp[sp.a2];
+    sp.prev = (float2*)h->sweep_slices.p;
+    sp.cur = sp.prev + ns;
+    HIP_TRY(h, launch_sweep(sp, h->stream));
+  } else {
+    HIP_TRY(h, launch_tmap_ref(g, m, s, nullptr, h->stream));
+  }
+  return BEAMS_OK;
+}
+
+int build_map(beams_renderer* h, const GridParams& g, const ShadeParams& s)
+{
+  int rc = prepare_map(h);
+  if (rc) return rc;
   MapParams m = map_params(h, g);
-  HIP_TRY(h, launch_tmap_ref(g, m, s, nullptr, h->stream));
+  if ((rc = enqueue_map(h, g, m, s))) return rc;
   h->map_valid = true;
   return BEAMS_OK;
 }
@@ -417,6 +491,9 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
       m.light[k] = h->light[k];
       m.light_color[k] = h->light_color[k];
     }
+    m.ms[0] = 1;
+    m.ms[1] = m.mp[0];
+    m.ms[2] = m.mp[0] * m.mp[1];
     m.step = std::sqrt(size[0] * size[0] + size[1] * size[1] + size[2] * size[2]) / 128.0f;
     m.dir_far = 4.0f * whole.ext_mag;  // one light origin for all bricks
     m.map = (unsigned char*)h->bmap_local.p + msz * (size_t)br.seed_off;
@@ -558,6 +635,8 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
 {
   int rc = check_overflow(h);
   if (rc) return rc;
+  if (h->use_shadow && h->shadow_mode != BEAMS_SHADOW_REF_MARCH)
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "bricks build their maps with the reference march");
   const size_t msz = dtype_size(h->map_dtype);
   int64_t nv_total = 0;
   for (const auto& br : h->bricks) nv_total += br.nverts;
@@ -938,7 +1017,8 @@ int beams_set_shadow(beams_renderer* h, int32_t enabled, const int64_t map_cells
                      int32_t map_dtype)
 {
   CHECK_HANDLE(h);
-  if (mode != BEAMS_SHADOW_REF_MARCH) return h->fail(BEAMS_ERR_UNSUPPORTED, "unknown shadow mode");
+  if (mode != BEAMS_SHADOW_REF_MARCH && mode != BEAMS_SHADOW_SWEEP)
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "unknown shadow mode");
   if (map_dtype != BEAMS_F32 && map_dtype != BEAMS_F16)
     return h->fail(BEAMS_ERR_UNSUPPORTED, "map storage must be F32 or F16");
   if (map_cells)
@@ -1056,10 +1136,7 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
   GridParams g = grid_params(h);
   ShadeParams s = shade_params(h, g);
   const bool with_map = h->use_shadow && h->assoc == 0;
-  if (with_map) {
-    const int64_t nv = (h->map_cells[0] + 1) * (h->map_cells[1] + 1) * (h->map_cells[2] + 1);
-    HIP_TRY(h, h->map.ensure((size_t)nv * dtype_size(h->map_dtype)));
-  }
+  if (with_map && (rc = prepare_map(h))) return rc;
   const size_t npix = (size_t)width * (size_t)height;
   ImageOut o{};
   if (mem == BEAMS_MEM_DEVICE) {
@@ -1078,7 +1155,7 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
     auto mark = [&](int p, bool end) { return timed ? phase_mark(h, p, end) : BEAMS_OK; };
     if (with_map) {
       if ((r = mark(BEAMS_PHASE_MAP, false))) return r;
-      HIP_TRY(h, launch_tmap_ref(g, m, s, nullptr, h->stream));
+      if ((r = enqueue_map(h, g, m, s))) return r;
       if ((r = mark(BEAMS_PHASE_MAP, true))) return r;
     }
     if ((r = mark(BEAMS_PHASE_MARCH, false))) return r;
@@ -1104,16 +1181,27 @@ int beams_get_map(beams_renderer* h, float* out, int32_t mem)
   if (!h->map_valid) return h->fail(BEAMS_ERR_STATE, "no map: render with shadows first");
   const int64_t nv = (h->map_cells[0] + 1) * (h->map_cells[1] + 1) * (h->map_cells[2] + 1);
   HIP_TRY(h, hipStreamSynchronize(h->stream));
-  if (h->map_dtype == BEAMS_F32) {
+  const MapParams m = map_params(h, grid_params(h));
+  const bool xfast = m.ms[0] == 1 && m.ms[1] == m.mp[0];
+  if (h->map_dtype == BEAMS_F32 && xfast) {
     HIP_TRY(h, hipMemcpy(out, h->map.p, sizeof(float) * (size_t)nv, copy_kind(mem, false)));
+    return BEAMS_OK;
+  }
+  // FP16 storage and/or the sweep's slice layout: decode and reorder to x fastest
+  std::vector<float> raw((size_t)nv), f((size_t)nv);
+  if (h->map_dtype == BEAMS_F32) {
+    HIP_TRY(h, hipMemcpy(raw.data(), h->map.p, sizeof(float) * (size_t)nv, hipMemcpyDeviceToHost));
   } else {
     std::vector<uint16_t> tmp((size_t)nv);
     HIP_TRY(h, hipMemcpy(tmp.data(), h->map.p, 2 * (size_t)nv, hipMemcpyDeviceToHost));
-    std::vector<float> f((size_t)nv);
-    for (size_t i = 0; i < f.size(); ++i) f[i] = f16_to_f32(tmp[i]);
-    HIP_TRY(h, hipMemcpy(out, f.data(), sizeof(float) * (size_t)nv,
-                         mem == BEAMS_MEM_DEVICE ? hipMemcpyHostToDevice : hipMemcpyHostToHost));
+    for (size_t i = 0; i < raw.size(); ++i) raw[i] = f16_to_f32(tmp[i]);
   }
+  size_t o = 0;
+  for (int64_t k = 0; k < m.mp[2]; ++k)
+    for (int64_t j = 0; j < m.mp[1]; ++j)
+      for (int64_t i = 0; i < m.mp[0]; ++i) f[o++] = raw[(size_t)(i * m.ms[0] + j * m.ms[1] + k * m.ms[2])];
+  HIP_TRY(h, hipMemcpy(out, f.data(), sizeof(float) * (size_t)nv,
+                       mem == BEAMS_MEM_DEVICE ? hipMemcpyHostToDevice : hipMemcpyHostToHost));
   return BEAMS_OK;
 }
 

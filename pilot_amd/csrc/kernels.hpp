@@ -28,6 +28,7 @@ struct MapParams {
   int32_t dtype;       // kF32 / kF16
   int32_t light_kind;  // 0 point, 1 directional
   int64_t mp[3];       // map point dims
+  int64_t ms[3];       // element strides of the map layout (x fastest: 1, mp0, mp0*mp1)
   float morg[3], msp[3], minv[3];
   float step;          // |size| / 128
   float light[3];      // position (point) or travel direction (directional)

@@ -138,16 +138,16 @@ __device__ __forceinline__ float map_opacity(const MapParams& m, const float* p)
     float hi = m.morg[d] + m.msp[d] * (float)(cell[d] + 1);
     pc[d] = (p[d] - lo) / (hi - lo);
   }
-  const int64_t sx = m.mp[0], sxy = m.mp[0] * m.mp[1];
-  const int64_t i0 = (cell[2] * m.mp[1] + cell[1]) * sx + cell[0];
+  const int64_t s0 = m.ms[0], sx = m.ms[1], sxy = m.ms[2];
+  const int64_t i0 = cell[2] * sxy + cell[1] * sx + cell[0] * s0;
   float v[8];
   v[0] = load_t<MDT>(m.map, i0);
-  v[1] = load_t<MDT>(m.map, i0 + 1);
-  v[2] = load_t<MDT>(m.map, i0 + 1 + sx);
+  v[1] = load_t<MDT>(m.map, i0 + s0);
+  v[2] = load_t<MDT>(m.map, i0 + s0 + sx);
   v[3] = load_t<MDT>(m.map, i0 + sx);
   v[4] = load_t<MDT>(m.map, i0 + sxy);
-  v[5] = load_t<MDT>(m.map, i0 + sxy + 1);
-  v[6] = load_t<MDT>(m.map, i0 + sxy + 1 + sx);
+  v[5] = load_t<MDT>(m.map, i0 + sxy + s0);
+  v[6] = load_t<MDT>(m.map, i0 + sxy + s0 + sx);
   v[7] = load_t<MDT>(m.map, i0 + sxy + sx);
   return interp_hex(v, pc);
 }
