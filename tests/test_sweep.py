@@ -66,7 +66,8 @@ def test_sweep_constant_field_matches_path_length(kind, light):
     # bilinear interpolation is exact where the path length is linear across a cell; where a
     # cell straddles the line at which rays switch from a side face to the near face, the
     # path is underestimated by up to ~half a cell (here 1/16: ~0.12 in opacity at worst)
-    assert np.median(err) < 1e-4
+    # (a point light's path length is also curved across a cell)
+    assert np.median(err) < (1e-4 if kind == 1 else 1e-3)
     assert np.quantile(err, 0.9) < 0.03
     assert err.max() < 0.15
 
