@@ -60,6 +60,9 @@ typedef struct beams_renderer beams_renderer;
 #define BEAMS_SHADOW_SWEEP 1     /* exact sweep: light-aligned slices carrying optical depth
                                     (one pass; needs the light outside the bounds' slab on
                                     one axis; a re-discretisation, not bit-equal) */
+#define BEAMS_SHADOW_LEARNED 2   /* coarse 32^3-mip optical depth + network correction per map
+                                    vertex, checked against the light march at sparse
+                                    vertices; falls back to an exact mode over the budget */
 
 /* phases reported by beams_get_timings (Beams.cxx:30-34 Phase1..5Time) */
 #define BEAMS_PHASE_MAP 0       /* phase 1: transmittance map build           */
@@ -237,6 +240,52 @@ int beams_gen_orbit_camera(const double bounds[6], double azimuth_deg, double el
 int beams_gen_camera_rays(const beams_camera* cam, int32_t width, int32_t height,
                           float* const origin[3], float* const dir[3], double* const origin64[3],
                           double* const dir64[3]);
+
+/* ==== Learned shadow closure (extension: no counterpart in the reference) =================
+ * A network of 32 inputs (31 features + a zero pad) -> 64 -> 64 -> 1 with ReLU, BF16 weights
+ * and activations, FP32 accumulation, run on the matrix cores.  It predicts, per map vertex,
+ * y = min(tau_coarse, 16) - min(tau_exact, 16) with tau = -ln(1 - opacity), from the coarse
+ * optical depth, the 3x3x3 alpha/max-alpha patch around the nearest field vertex and the
+ * light direction.  Parameters are FP32 master copies packed as W1[32][64], b1[64],
+ * W2[64][64], b2[64], W3[64], b3 (6,337 floats); W1 row 31 multiplies the zero pad. */
+typedef struct beams_mlp beams_mlp;
+#define BEAMS_MLP_FEATURES 32
+
+int beams_mlp_num_params(void);
+/* He-uniform weights from SplitMix64(seed), zero biases, zero optimiser state. */
+int beams_mlp_create(beams_mlp** out, uint64_t seed);
+int beams_mlp_destroy(beams_mlp* h);
+const char* beams_mlp_last_error(const beams_mlp* h);
+int beams_mlp_get_params(beams_mlp* h, float* out);
+/* Replaces the parameters (finite values only) and resets the optimiser state. */
+int beams_mlp_set_params(beams_mlp* h, const float* in);
+/* y[n] = network(x[n][32]). */
+int beams_mlp_forward(beams_mlp* h, const float* x, int64_t n, float* y, int32_t mem);
+/* Gradients (6,337 floats, host memory) of the mean squared error over n rows with respect to
+ * the BF16-rounded parameters the forward pass uses; *loss (may be NULL) = that error. */
+int beams_mlp_gradients(beams_mlp* h, const float* x, const float* t, int64_t n, float* grads,
+                        float* loss, int32_t mem);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over consecutive minibatches of `batch` rows for `epochs`
+ * passes; *loss (may be NULL) = mean squared error over the last pass. */
+int beams_mlp_train(beams_mlp* h, const float* x, const float* t, int64_t n, int32_t epochs,
+                    int32_t batch, float lr, float* loss, int32_t mem);
+
+/* Attach the network used by BEAMS_SHADOW_LEARNED (net NULL: the coarse optical depth alone).
+ * fallback_mode (BEAMS_SHADOW_REF_MARCH or BEAMS_SHADOW_SWEEP) builds the map instead when the
+ * largest opacity error at the check vertices (every check_stride-th map vertex per axis and
+ * the last) exceeds error_budget.  The renderer keeps the pointer: destroy the renderer or
+ * attach another network before destroying net. */
+int beams_set_learned(beams_renderer* h, const beams_mlp* net, int32_t fallback_mode,
+                      float error_budget, int32_t check_stride);
+/* Training data from the current field, colormap and light: builds the exact map with
+ * `mode` (BEAMS_SHADOW_REF_MARCH or BEAMS_SHADOW_SWEEP) and writes the features
+ * (n x 32) and targets (n) of n map vertices drawn uniformly by SplitMix64(seed). */
+int beams_learned_samples(beams_renderer* h, int32_t mode, int64_t n, uint64_t seed,
+                          float* features, float* targets, int32_t mem);
+/* The last learned map build: err[0] = largest |learned - exact| opacity at the check
+ * vertices, err[1] = the same for the coarse optical depth alone; *fell_back = 1 when the
+ * fallback mode built the map. */
+int beams_get_learned_stats(beams_renderer* h, float err[2], int32_t* fell_back);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,9 @@ MEM_HOST, MEM_DEVICE = 0, 1
 ASSOC_POINTS, ASSOC_CELLS = 0, 1
 LIGHT_POINT, LIGHT_DIRECTIONAL = 0, 1
 SHADOW_REF_MARCH = 0
+SHADOW_SWEEP = 1
+SHADOW_LEARNED = 2
+MLP_FEATURES = 32
 PHASES = ("map", "exchange", "remap", "march", "composite", "frame")
 
 STATUS = {
@@ -99,6 +102,18 @@ SIGNATURES = {
     "beams_gen_orbit_camera": (C.c_int, [_PD, _D, _D, _D, C.POINTER(Camera)]),
     "beams_gen_camera_rays": (C.c_int, [C.POINTER(Camera), _I32, _I32, C.POINTER(_PF), C.POINTER(_PF),
                                         C.POINTER(_PD), C.POINTER(_PD)]),
+    "beams_mlp_num_params": (C.c_int, []),
+    "beams_mlp_create": (C.c_int, [C.POINTER(_P), C.c_uint64]),
+    "beams_mlp_destroy": (C.c_int, [_P]),
+    "beams_mlp_last_error": (C.c_char_p, [_P]),
+    "beams_mlp_get_params": (C.c_int, [_P, _PF]),
+    "beams_mlp_set_params": (C.c_int, [_P, _PF]),
+    "beams_mlp_forward": (C.c_int, [_P, _P, _I64, _P, _I32]),
+    "beams_mlp_gradients": (C.c_int, [_P, _P, _P, _I64, _PF, _PF, _I32]),
+    "beams_mlp_train": (C.c_int, [_P, _P, _P, _I64, _I32, _I32, _F, _PF, _I32]),
+    "beams_set_learned": (C.c_int, [_P, _P, _I32, _F, _I32]),
+    "beams_learned_samples": (C.c_int, [_P, _I32, _I64, C.c_uint64, _P, _P, _I32]),
+    "beams_get_learned_stats": (C.c_int, [_P, _PF, C.POINTER(C.c_int32)]),
 }
 
 _LIB = None
@@ -352,6 +367,107 @@ class LightedVolumeRenderer:
 
     def Synchronize(self):
         self._c(lib().beams_synchronize(self._h))
+
+    # ---- learned shadow closure
+    def SetLearned(self, net=None, fallback=SHADOW_REF_MARCH, error_budget=2e-3, check_stride=8):
+        """Network for SHADOW_LEARNED (None: the coarse optical depth alone)."""
+        self._net = net  # the renderer keeps the pointer
+        self._c(lib().beams_set_learned(self._h, net._h if net is not None else None, fallback,
+                                        float(error_budget), int(check_stride)))
+
+    def LearnedSamples(self, n, seed, mode=SHADOW_SWEEP, features=None, targets=None):
+        """Features (n, 32) and targets (n,) from the exact map of `mode`; device outputs when
+        features/targets are device arrays."""
+        if features is not None:
+            self._c(lib().beams_learned_samples(self._h, mode, int(n), int(seed), C.c_void_p(_ptr(features)),
+                                                C.c_void_p(_ptr(targets)), _mem_of(features)))
+            return features, targets
+        x = np.zeros((n, MLP_FEATURES), np.float32)
+        t = np.zeros(n, np.float32)
+        self._c(lib().beams_learned_samples(self._h, mode, int(n), int(seed), C.c_void_p(x.ctypes.data),
+                                            C.c_void_p(t.ctypes.data), MEM_HOST))
+        return x, t
+
+    def GetLearnedStats(self):
+        e = np.zeros(2, np.float32)
+        fb = C.c_int32()
+        self._c(lib().beams_get_learned_stats(self._h, _f32p(e), C.byref(fb)))
+        return float(e[0]), float(e[1]), bool(fb.value)
+
+
+class ShadowMLP:
+    """The learned closure's network (beams_mlp): parameters, forward, gradients, training."""
+
+    def __init__(self, seed=1):
+        h = C.c_void_p()
+        st = lib().beams_mlp_create(C.byref(h), int(seed))
+        if st != 0:
+            raise BeamsError(st, "beams_mlp_create failed (no gfx950 device?)")
+        self._h = h
+        self.nparams = lib().beams_mlp_num_params()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().beams_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _c(self, status):
+        if status != 0:
+            raise BeamsError(status, lib().beams_mlp_last_error(self._h).decode())
+
+    def get_params(self):
+        p = np.zeros(self.nparams, np.float32)
+        self._c(lib().beams_mlp_get_params(self._h, _f32p(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        self._c(lib().beams_mlp_set_params(self._h, _f32p(p)))
+
+    def forward(self, x):
+        if hasattr(x, "data_ptr"):
+            raise TypeError("forward takes host arrays; use the device entry point directly")
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        y = np.zeros(x.shape[0], np.float32)
+        self._c(lib().beams_mlp_forward(self._h, C.c_void_p(x.ctypes.data), x.shape[0],
+                                        C.c_void_p(y.ctypes.data), MEM_HOST))
+        return y
+
+    def gradients(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        g = np.zeros(self.nparams, np.float32)
+        loss = C.c_float()
+        self._c(lib().beams_mlp_gradients(self._h, C.c_void_p(x.ctypes.data), C.c_void_p(t.ctypes.data),
+                                          x.shape[0], _f32p(g), C.byref(loss), MEM_HOST))
+        return g, loss.value
+
+    def train(self, x, t, epochs=1, batch=4096, lr=1e-3):
+        """Adam over consecutive minibatches; x/t host arrays or device arrays."""
+        loss = C.c_float()
+        if hasattr(x, "data_ptr"):
+            n = int(t.shape[0])
+            self._c(lib().beams_mlp_train(self._h, C.c_void_p(_ptr(x)), C.c_void_p(_ptr(t)), n, int(epochs),
+                                          int(batch), float(lr), C.byref(loss), MEM_DEVICE))
+            return loss.value
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        self._c(lib().beams_mlp_train(self._h, C.c_void_p(x.ctypes.data), C.c_void_p(t.ctypes.data),
+                                      x.shape[0], int(epochs), int(batch), float(lr), C.byref(loss), MEM_HOST))
+        return loss.value
 
 
 # ---- scene inputs (DataSetFactory / SceneFactory side) -----------------------------------------

@@ -8,6 +8,7 @@
 #include "sweep.hpp"
 #include "gen.hpp"
 #include "kernels.hpp"
+#include "learned.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -82,6 +83,16 @@ struct beams_renderer {
   DevBuf map;
   DevBuf sweep_slices;  // (tau, sigma) ping-pong slices of the exact sweep
   bool map_valid = false;
+  int32_t built_mode = BEAMS_SHADOW_REF_MARCH;  // the mode (and so the layout) of the map held
+  // learned closure
+  const beams_mlp* net = nullptr;
+  int32_t fallback_mode = BEAMS_SHADOW_REF_MARCH;
+  float error_budget = 2e-3f;
+  int32_t check_stride = 8;
+  DevBuf coarse;  // coarse extinction and optical depth, (kCoarseCells + 1)^3 each
+  DevBuf lerr;    // check errors as float bits
+  float last_err[2] = { -1.0f, -1.0f };
+  int32_t fell_back = 0;
   // march
   float sample_distance = -1.0f;
   float alpha_stop = 1.0f;
@@ -206,7 +217,9 @@ bool sweep_axis(const beams_renderer* h, int& axis, int& a1, int& a2, int& step)
   return true;
 }
 
-MapParams map_params(const beams_renderer* h, const GridParams& g)
+// Map parameters of a map built by `mode`: the sweep stores slices contiguously, the other
+// modes x fastest.
+MapParams map_params(const beams_renderer* h, const GridParams& g, int32_t mode)
 {
   MapParams m{};
   m.map = h->map.p;
@@ -226,7 +239,7 @@ MapParams map_params(const beams_renderer* h, const GridParams& g)
   m.ms[1] = m.mp[0];
   m.ms[2] = m.mp[0] * m.mp[1];
   int axis, a1, a2, step;
-  if (h->shadow_mode == BEAMS_SHADOW_SWEEP && sweep_axis(h, axis, a1, a2, step)) {
+  if (mode == BEAMS_SHADOW_SWEEP && sweep_axis(h, axis, a1, a2, step)) {
     m.ms[a1] = 1;  // slice-contiguous layout, the sweep axis slowest
     m.ms[a2] = m.mp[a1];
     m.ms[axis] = m.mp[a1] * m.mp[a2];
@@ -334,11 +347,16 @@ int phase_mark(beams_renderer* h, int phase, bool end)
 
 // Phase 1 (and 3, which equals phase 1 for a single block): build the map.
 // Device buffers of the single-block map build (allocated outside any graph capture).
-int prepare_map(beams_renderer* h)
+int prepare_map(beams_renderer* h, int32_t mode)
 {
   const int64_t nv = (h->map_cells[0] + 1) * (h->map_cells[1] + 1) * (h->map_cells[2] + 1);
   HIP_TRY(h, h->map.ensure((size_t)nv * dtype_size(h->map_dtype)));
-  if (h->shadow_mode == BEAMS_SHADOW_SWEEP) {
+  if (mode == BEAMS_SHADOW_LEARNED) {
+    const size_t nc = (size_t)(kCoarseCells + 1) * (kCoarseCells + 1) * (kCoarseCells + 1);
+    HIP_TRY(h, h->coarse.ensure(2 * nc * sizeof(float)));
+    HIP_TRY(h, h->lerr.ensure(2 * sizeof(unsigned int)));
+  }
+  if (mode == BEAMS_SHADOW_SWEEP) {
     int axis, a1, a2, step;
     if (!sweep_axis(h, axis, a1, a2, step))
       return h->fail(BEAMS_ERR_UNSUPPORTED, "the sweep needs a light outside the volume's bounds");
@@ -349,9 +367,10 @@ int prepare_map(beams_renderer* h)
 }
 
 // Enqueues the map build: the reference's per-vertex march, or the exact sweep.
-int enqueue_map(beams_renderer* h, const GridParams& g, const MapParams& m, const ShadeParams& s)
+int enqueue_map(beams_renderer* h, const GridParams& g, const MapParams& m, const ShadeParams& s,
+                int32_t mode)
 {
-  if (h->shadow_mode == BEAMS_SHADOW_SWEEP) {
+  if (mode == BEAMS_SHADOW_SWEEP) {
     SweepParams sp{};
     sp.g = g;
     sp.m = m;
@@ -369,12 +388,65 @@ int enqueue_map(beams_renderer* h, const GridParams& g, const MapParams& m, cons
   return BEAMS_OK;
 }
 
+LearnedParams learned_params(beams_renderer* h, const GridParams& g, const ShadeParams& s,
+                             const MapParams& m)
+{
+  LearnedParams p{};
+  p.g = g;
+  p.m = m;
+  p.s = s;
+  p.cn = kCoarseCells + 1;
+  p.coarse_sigma = (float*)h->coarse.p;
+  p.coarse_tau = p.coarse_sigma + (size_t)p.cn * p.cn * p.cn;
+  for (int d = 0; d < 3; ++d) {
+    p.corg[d] = (float)h->bounds[2 * d];
+    p.csp[d] = (float)(h->bounds[2 * d + 1] - h->bounds[2 * d]) / (float)kCoarseCells;
+    p.cinv[d] = 1.0f / p.csp[d];
+  }
+  p.inv_h = 1.0f / m.step;
+  p.params = h->net ? mlp_device_params(h->net) : nullptr;
+  p.err = (unsigned int*)h->lerr.p;
+  p.check_stride = h->check_stride;
+  return p;
+}
+
+// The learned map: coarse optical depth, the network's correction per vertex, the sparse
+// exact check, and the fallback build when the check exceeds the budget (a host decision:
+// the check's two words are read back before the march is enqueued).
+int build_learned(beams_renderer* h, const GridParams& g, const ShadeParams& s)
+{
+  int rc = prepare_map(h, BEAMS_SHADOW_LEARNED);
+  if (rc) return rc;
+  const MapParams m = map_params(h, g, BEAMS_SHADOW_LEARNED);
+  const LearnedParams p = learned_params(h, g, s, m);
+  HIP_TRY(h, launch_coarse(p, h->stream));
+  HIP_TRY(h, launch_learned_map(p, h->stream));
+  HIP_TRY(h, launch_zero_u64((unsigned long long*)h->lerr.p, 1, h->stream));
+  HIP_TRY(h, launch_learned_check(p, h->stream));
+  unsigned int e[2] = { 0, 0 };
+  HIP_TRY(h, hipMemcpyAsync(e, h->lerr.p, sizeof(e), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->last_err[0] = bits_f32(e[0]);
+  h->last_err[1] = bits_f32(e[1]);
+  h->fell_back = !(h->last_err[0] <= h->error_budget);
+  h->built_mode = BEAMS_SHADOW_LEARNED;
+  if (h->fell_back) {
+    if ((rc = prepare_map(h, h->fallback_mode))) return rc;
+    if ((rc = enqueue_map(h, g, map_params(h, g, h->fallback_mode), s, h->fallback_mode))) return rc;
+    h->built_mode = h->fallback_mode;
+  }
+  h->map_valid = true;
+  return BEAMS_OK;
+}
+
 int build_map(beams_renderer* h, const GridParams& g, const ShadeParams& s)
 {
-  int rc = prepare_map(h);
+  if (h->shadow_mode == BEAMS_SHADOW_LEARNED) return build_learned(h, g, s);
+  int rc = prepare_map(h, h->shadow_mode);
   if (rc) return rc;
-  MapParams m = map_params(h, g);
-  if ((rc = enqueue_map(h, g, m, s))) return rc;
+  MapParams m = map_params(h, g, h->shadow_mode);
+  if ((rc = enqueue_map(h, g, m, s, h->shadow_mode))) return rc;
+  h->built_mode = h->shadow_mode;
   h->map_valid = true;
   return BEAMS_OK;
 }
@@ -1017,7 +1089,7 @@ int beams_set_shadow(beams_renderer* h, int32_t enabled, const int64_t map_cells
                      int32_t map_dtype)
 {
   CHECK_HANDLE(h);
-  if (mode != BEAMS_SHADOW_REF_MARCH && mode != BEAMS_SHADOW_SWEEP)
+  if (mode != BEAMS_SHADOW_REF_MARCH && mode != BEAMS_SHADOW_SWEEP && mode != BEAMS_SHADOW_LEARNED)
     return h->fail(BEAMS_ERR_UNSUPPORTED, "unknown shadow mode");
   if (map_dtype != BEAMS_F32 && map_dtype != BEAMS_F16)
     return h->fail(BEAMS_ERR_UNSUPPORTED, "map storage must be F32 or F16");
@@ -1104,7 +1176,7 @@ int beams_render_rays(beams_renderer* h, int64_t n, const float* const origin[3]
     r.rgba = drgba;
   }
   if ((rc = phase_mark(h, BEAMS_PHASE_MARCH, false))) return rc;
-  MapParams m = map_params(h, g);
+  MapParams m = map_params(h, g, h->built_mode);
   HIP_TRY(h, launch_march_rays(g, m, s, r, h->stream));
   if ((rc = phase_mark(h, BEAMS_PHASE_MARCH, true))) return rc;
   if (mem == BEAMS_MEM_HOST) {
@@ -1135,8 +1207,16 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
   if ((rc = frame_begin(h))) return rc;
   GridParams g = grid_params(h);
   ShadeParams s = shade_params(h, g);
-  const bool with_map = h->use_shadow && h->assoc == 0;
-  if (with_map && (rc = prepare_map(h))) return rc;
+  // The learned map is built eagerly (its fallback decision reads the check back); the exact
+  // modes' map build is part of the frame's launch sequence (and graph).
+  const bool learned = h->use_shadow && h->assoc == 0 && h->shadow_mode == BEAMS_SHADOW_LEARNED;
+  const bool with_map = h->use_shadow && h->assoc == 0 && !learned;
+  if (learned) {
+    if ((rc = phase_mark(h, BEAMS_PHASE_MAP, false))) return rc;
+    if ((rc = build_learned(h, g, s))) return rc;
+    if ((rc = phase_mark(h, BEAMS_PHASE_MAP, true))) return rc;
+  }
+  if (with_map && (rc = prepare_map(h, h->shadow_mode))) return rc;
   const size_t npix = (size_t)width * (size_t)height;
   ImageOut o{};
   if (mem == BEAMS_MEM_DEVICE) {
@@ -1149,13 +1229,13 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
   }
   o.use_bg = background ? 1 : 0;
   for (int k = 0; k < 4; ++k) o.bg[k] = background ? background[k] : 0.0f;
-  const MapParams m = map_params(h, g);
+  const MapParams m = map_params(h, g, learned ? h->built_mode : h->shadow_mode);
   auto issue = [&](bool timed) -> int {
     int r;
     auto mark = [&](int p, bool end) { return timed ? phase_mark(h, p, end) : BEAMS_OK; };
     if (with_map) {
       if ((r = mark(BEAMS_PHASE_MAP, false))) return r;
-      if ((r = enqueue_map(h, g, m, s))) return r;
+      if ((r = enqueue_map(h, g, m, s, h->shadow_mode))) return r;
       if ((r = mark(BEAMS_PHASE_MAP, true))) return r;
     }
     if ((r = mark(BEAMS_PHASE_MARCH, false))) return r;
@@ -1165,7 +1245,10 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
   KeyBuilder key;
   key.put(0).put(g).put(m).put(s).put(c).put(o).put(with_map);
   if ((rc = run_frame(h, key.bytes, issue))) return rc;
-  if (with_map) h->map_valid = true;
+  if (with_map) {
+    h->map_valid = true;
+    h->built_mode = h->shadow_mode;
+  }
   if (mem == BEAMS_MEM_HOST) {
     HIP_TRY(h, hipMemcpyAsync(rgba, o.rgba, sizeof(float) * 4 * npix, hipMemcpyDeviceToHost, h->stream));
     if (depth) HIP_TRY(h, hipMemcpyAsync(depth, o.depth, sizeof(float) * npix, hipMemcpyDeviceToHost, h->stream));
@@ -1181,7 +1264,7 @@ int beams_get_map(beams_renderer* h, float* out, int32_t mem)
   if (!h->map_valid) return h->fail(BEAMS_ERR_STATE, "no map: render with shadows first");
   const int64_t nv = (h->map_cells[0] + 1) * (h->map_cells[1] + 1) * (h->map_cells[2] + 1);
   HIP_TRY(h, hipStreamSynchronize(h->stream));
-  const MapParams m = map_params(h, grid_params(h));
+  const MapParams m = map_params(h, grid_params(h), h->built_mode);
   const bool xfast = m.ms[0] == 1 && m.ms[1] == m.mp[0];
   if (h->map_dtype == BEAMS_F32 && xfast) {
     HIP_TRY(h, hipMemcpy(out, h->map.p, sizeof(float) * (size_t)nv, copy_kind(mem, false)));
@@ -1202,6 +1285,71 @@ int beams_get_map(beams_renderer* h, float* out, int32_t mem)
       for (int64_t i = 0; i < m.mp[0]; ++i) f[o++] = raw[(size_t)(i * m.ms[0] + j * m.ms[1] + k * m.ms[2])];
   HIP_TRY(h, hipMemcpy(out, f.data(), sizeof(float) * (size_t)nv,
                        mem == BEAMS_MEM_DEVICE ? hipMemcpyHostToDevice : hipMemcpyHostToHost));
+  return BEAMS_OK;
+}
+
+int beams_set_learned(beams_renderer* h, const beams_mlp* net, int32_t fallback_mode,
+                      float error_budget, int32_t check_stride)
+{
+  CHECK_HANDLE(h);
+  if (fallback_mode != BEAMS_SHADOW_REF_MARCH && fallback_mode != BEAMS_SHADOW_SWEEP)
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "the fallback must be an exact mode");
+  if (!(error_budget >= 0.0f)) return h->fail(BEAMS_ERR_INVALID, "error budget must be >= 0");
+  if (check_stride < 1) return h->fail(BEAMS_ERR_INVALID, "check stride must be >= 1");
+  h->net = net;
+  h->fallback_mode = fallback_mode;
+  h->error_budget = error_budget;
+  h->check_stride = check_stride;
+  return BEAMS_OK;
+}
+
+int beams_learned_samples(beams_renderer* h, int32_t mode, int64_t n, uint64_t seed,
+                          float* features, float* targets, int32_t mem)
+{
+  CHECK_HANDLE(h);
+  if (mode != BEAMS_SHADOW_REF_MARCH && mode != BEAMS_SHADOW_SWEEP)
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "training maps come from an exact mode");
+  if (n < 0 || !features || !targets) return h->fail(BEAMS_ERR_INVALID, "bad sample output");
+  if (mem != BEAMS_MEM_HOST && mem != BEAMS_MEM_DEVICE) return h->fail(BEAMS_ERR_INVALID, "bad mem kind");
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (h->assoc != BEAMS_ASSOC_POINTS) return h->fail(BEAMS_ERR_UNSUPPORTED, "learned maps need a point field");
+  if (!h->has_light) return h->fail(BEAMS_ERR_STATE, "samples need a light: call beams_set_light");
+  if (!h->bricks.empty()) return h->fail(BEAMS_ERR_UNSUPPORTED, "samples come from the single-block map");
+  GridParams g = grid_params(h);
+  ShadeParams s = shade_params(h, g);
+  s.counters = nullptr;
+  if ((rc = prepare_map(h, mode))) return rc;
+  if ((rc = prepare_map(h, BEAMS_SHADOW_LEARNED))) return rc;
+  const MapParams em = map_params(h, g, mode);
+  if ((rc = enqueue_map(h, g, em, s, mode))) return rc;
+  h->built_mode = mode;
+  h->map_valid = true;
+  const LearnedParams p = learned_params(h, g, s, map_params(h, g, BEAMS_SHADOW_LEARNED));
+  HIP_TRY(h, launch_coarse(p, h->stream));
+  float *dx = features, *dt = targets;
+  if (mem == BEAMS_MEM_HOST) {
+    HIP_TRY(h, h->rays.ensure(sizeof(float) * (size_t)n * (BEAMS_MLP_FEATURES + 1) + 64));
+    dx = (float*)h->rays.p;
+    dt = dx + (size_t)n * BEAMS_MLP_FEATURES;
+  }
+  HIP_TRY(h, launch_learned_samples(p, em, n, seed, dx, dt, h->stream));
+  if (mem == BEAMS_MEM_HOST) {
+    HIP_TRY(h, hipMemcpyAsync(features, dx, sizeof(float) * (size_t)n * BEAMS_MLP_FEATURES, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(targets, dt, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+  }
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return BEAMS_OK;
+}
+
+int beams_get_learned_stats(beams_renderer* h, float err[2], int32_t* fell_back)
+{
+  CHECK_HANDLE(h);
+  if (err) {
+    err[0] = h->last_err[0];
+    err[1] = h->last_err[1];
+  }
+  if (fell_back) *fell_back = h->fell_back;
   return BEAMS_OK;
 }
 

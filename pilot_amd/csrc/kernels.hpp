@@ -76,6 +76,23 @@ struct ImageOut {
   int32_t use_bg;
 };
 
+// Instantiate BODY for the run-time field (FDT) or map (MDT) storage type; unknown types
+// return hipErrorInvalidValue from the enclosing launcher.
+#define BEAMS_DISPATCH_FIELD(FDT_VAR, BODY)          \
+  switch (FDT_VAR) {                                 \
+    case kF32: { constexpr int FDT = kF32; BODY; } break;   \
+    case kF16: { constexpr int FDT = kF16; BODY; } break;   \
+    case kBF16: { constexpr int FDT = kBF16; BODY; } break; \
+    default: return hipErrorInvalidValue;            \
+  }
+
+#define BEAMS_DISPATCH_MAP(MDT_VAR, BODY)            \
+  switch (MDT_VAR) {                                 \
+    case kF32: { constexpr int MDT = kF32; BODY; } break;   \
+    case kF16: { constexpr int MDT = kF16; BODY; } break;   \
+    default: return hipErrorInvalidValue;            \
+  }
+
 // ---- launchers (return hipError_t) ------------------------------------------------------
 hipError_t launch_tmap_ref(const GridParams& g, const MapParams& m, const ShadeParams& s,
                            const float* seed, hipStream_t st);

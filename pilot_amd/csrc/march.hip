@@ -105,21 +105,6 @@ size_t lds_bytes_coords(const GridParams& g) { return sizeof(float) * (g.n[0] + 
 }  // namespace
 
 // ---- launchers -----------------------------------------------------------------------------
-#define BEAMS_DISPATCH_FIELD(FDT_VAR, BODY)          \
-  switch (FDT_VAR) {                                 \
-    case kF32: { constexpr int FDT = kF32; BODY; } break;   \
-    case kF16: { constexpr int FDT = kF16; BODY; } break;   \
-    case kBF16: { constexpr int FDT = kBF16; BODY; } break; \
-    default: return hipErrorInvalidValue;            \
-  }
-
-#define BEAMS_DISPATCH_MAP(MDT_VAR, BODY)            \
-  switch (MDT_VAR) {                                 \
-    case kF32: { constexpr int MDT = kF32; BODY; } break;   \
-    case kF16: { constexpr int MDT = kF16; BODY; } break;   \
-    default: return hipErrorInvalidValue;            \
-  }
-
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t st)
 {
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p, n);

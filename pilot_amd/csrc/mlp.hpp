@@ -1,0 +1,167 @@
+// The learned shadow closure's network: 32 inputs (31 features + a zero pad) -> 64 -> 64 -> 1,
+// ReLU, BF16 weights and activations, FP32 accumulation, evaluated 64 rows per wave on
+// the gfx950 matrix cores (v_mfma_f32_16x16x32_bf16).
+//
+// Fragment layouts of v_mfma_f32_16x16x32_bf16 (lane l, g = l / 16, r = l % 16):
+//   A (16 x 32): lane holds A[r][8g .. 8g+7]       B (32 x 16): lane holds B[8g .. 8g+7][r]
+//   D (16 x 16): lane holds D[4g .. 4g+3][r]
+// Activations move between layers through LDS rows of kLd bf16 (72: 144-byte rows keep the
+// 16 rows a fragment load touches in distinct banks).
+#pragma once
+
+#include "common.hpp"
+
+namespace beams {
+
+constexpr int kFeat = 32;
+constexpr int kHid = 64;
+constexpr int kLd = kHid + 8;
+// packed FP32 parameters: W1[k][n] (32 x 64, row 31 multiplies the zero pad), b1[64],
+// W2[k][n] (64 x 64), b2[64], W3[64], b3
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kFeat * kHid;
+constexpr int kOffW2 = kOffB1 + kHid;
+constexpr int kOffB2 = kOffW2 + kHid * kHid;
+constexpr int kOffW3 = kOffB2 + kHid;
+constexpr int kOffB3 = kOffW3 + kHid;
+constexpr int kParams = kOffB3 + 1;
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+typedef __attribute__((__vector_size__(8 * sizeof(uint16_t)))) uint16_t u16x8;
+
+// Weights as the MFMA B operands want them (n-major rows of k), plus the FP32 vectors.
+struct MlpLds {
+  uint16_t w1t[kHid][kFeat + 8];
+  uint16_t w2t[kHid][kLd];  // w2t[n][k] = W2[k][n]
+  float b1[kHid], b2[kHid], w3[kHid];
+  float b3;
+};
+
+__device__ __forceinline__ void mlp_stage(const float* params, MlpLds& w)
+{
+  for (int i = threadIdx.x; i < kFeat * kHid; i += blockDim.x) {
+    const int k = i / kHid, n = i % kHid;
+    w.w1t[n][k] = f32_to_bf16(params[kOffW1 + i]);
+  }
+  for (int i = threadIdx.x; i < kHid * kHid; i += blockDim.x) {
+    const int k = i / kHid, n = i % kHid;
+    w.w2t[n][k] = f32_to_bf16(params[kOffW2 + i]);
+  }
+  for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
+    w.b1[i] = params[kOffB1 + i];
+    w.b2[i] = params[kOffB2 + i];
+    w.w3[i] = bf16_to_f32(f32_to_bf16(params[kOffW3 + i]));
+  }
+  if (threadIdx.x == 0) w.b3 = params[kOffB3];
+}
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ bf16x8 lds_frag(const uint16_t* p)
+{
+  return __builtin_bit_cast(bf16x8, *(const u16x8*)p);
+}
+
+__device__ __forceinline__ bf16x8 pack_frag(const float* v)
+{
+  u16x8 u;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) u[i] = f32_to_bf16(v[i]);
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Forward pass of the whole wave: lane l supplies row l's 32 features; returns row l's
+// output.  buf: a per-wave LDS buffer of 64 * kLd bf16; ybuf: 64 floats.  Every lane of
+// the wave must call it.  h1pre/h2pre (optional) receive the pre-activations in D layout
+// ([tile rt][tile ct][i]: row 16 rt + 4 g + i, column 16 ct + r).
+__device__ __forceinline__ float mlp_forward_wave(const MlpLds& w, uint16_t* buf, float* ybuf,
+                                                  const float* feat, f32x4 (*h1pre)[4] = nullptr,
+                                                  f32x4 (*h2pre)[4] = nullptr)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  // rows -> LDS, then the layer-1 A fragments into registers
+  {
+    u16x8 a[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) a[q][i] = f32_to_bf16(feat[8 * q + i]);
+      *(u16x8*)(buf + lane * kLd + 8 * q) = a[q];
+    }
+  }
+  wave_lds_sync();
+  bf16x8 xa[4];
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) xa[rt] = lds_frag(buf + (16 * rt + r) * kLd + 8 * g);
+  wave_lds_sync();
+  // layer 1: 4 x 4 output tiles, K = 32
+#pragma unroll
+  for (int ct = 0; ct < 4; ++ct) {
+    const bf16x8 b = lds_frag(&w.w1t[16 * ct + r][8 * g]);
+    const float bias = w.b1[16 * ct + r];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      f32x4 acc = { 0.0f, 0.0f, 0.0f, 0.0f };
+      acc = mfma(xa[rt], b, acc);
+      if (h1pre) h1pre[rt][ct] = acc + bias;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        buf[(16 * rt + 4 * g + i) * kLd + 16 * ct + r] = f32_to_bf16(fmaxf(acc[i] + bias, 0.0f));
+    }
+  }
+  wave_lds_sync();
+  // layer 2: K = 64 in two steps; layer 3 folded into the epilogue
+  float part[4][4];
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) part[rt][i] = 0.0f;
+#pragma unroll
+  for (int ct = 0; ct < 4; ++ct) {
+    const bf16x8 b0 = lds_frag(&w.w2t[16 * ct + r][8 * g]);
+    const bf16x8 b1 = lds_frag(&w.w2t[16 * ct + r][32 + 8 * g]);
+    const float bias = w.b2[16 * ct + r], w3 = w.w3[16 * ct + r];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      f32x4 acc = { 0.0f, 0.0f, 0.0f, 0.0f };
+      acc = mfma(lds_frag(buf + (16 * rt + r) * kLd + 8 * g), b0, acc);
+      acc = mfma(lds_frag(buf + (16 * rt + r) * kLd + 32 + 8 * g), b1, acc);
+      if (h2pre) h2pre[rt][ct] = acc + bias;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) part[rt][i] += fmaxf(acc[i] + bias, 0.0f) * w3;
+    }
+  }
+  // sum the 16 columns each lane group holds (lanes sharing g)
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float v = part[rt][i];
+      v += __shfl_xor(v, 1, 64);
+      v += __shfl_xor(v, 2, 64);
+      v += __shfl_xor(v, 4, 64);
+      v += __shfl_xor(v, 8, 64);
+      part[rt][i] = v;
+    }
+  if (r == 0)
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) ybuf[16 * rt + 4 * g + i] = part[rt][i] + w.b3;
+  wave_lds_sync();
+  const float y = ybuf[lane];
+  wave_lds_sync();
+  return y;
+}
+
+}  // namespace beams

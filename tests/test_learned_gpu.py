@@ -1,0 +1,211 @@
+"""The learned shadow closure on the GPU: the network's forward and gradients against the
+CPU restatement (BF16 rounding points emulated), training, the features and targets, the
+learned map and its sparse exact check, the fallback, and what the network gains over the
+coarse optical depth alone."""
+
+import numpy as np
+import pytest
+
+from pilot_amd import beams as B
+from pilot_amd import scenes
+
+import learned_ref as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_params(seed):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(scale=0.25, size=L.NPARAMS).astype(np.float32)
+    p[L.OFF_W1 + 31 * L.HID:L.OFF_B1] = 0.0
+    return p
+
+
+def _inputs(seed, n):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, size=(n, L.FEAT)).astype(np.float32)
+    x[:, 31] = 0.0
+    t = rng.normal(size=n).astype(np.float32)
+    return x, t
+
+
+@pytest.mark.parametrize("n", [64, 1000, 4097])
+def test_mlp_forward_matches_restatement(n):
+    p = _random_params(11)
+    x, _ = _inputs(12, n)
+    with B.ShadowMLP(seed=1) as net:
+        net.set_params(p)
+        y = net.forward(x)
+    ref, _ = L.forward(p, x, emulate=True)
+    err = np.abs(y - ref)
+    scale = np.abs(ref).max()
+    print(f"forward n={n}: max |d| {err.max():.3e} (scale {scale:.3f}), median {np.median(err):.3e}")
+    # FP32 accumulation order differs; a hidden activation can round to the neighbouring BF16
+    assert np.median(err) <= 1e-3 * scale
+    assert err.max() <= 2e-2 * scale
+
+
+def test_mlp_gradients_match_restatement():
+    p = _random_params(21)
+    x, t = _inputs(22, 1000)
+    with B.ShadowMLP(seed=1) as net:
+        net.set_params(p)
+        g, loss = net.gradients(x, t)
+    gr, lr = L.gradients(p, x, t, emulate=True)
+    assert abs(loss - lr) <= 1e-3 * lr
+    blocks = {"W1": (L.OFF_W1, L.OFF_B1), "b1": (L.OFF_B1, L.OFF_W2), "W2": (L.OFF_W2, L.OFF_B2),
+              "b2": (L.OFF_B2, L.OFF_W3), "W3": (L.OFF_W3, L.OFF_B3), "b3": (L.OFF_B3, L.NPARAMS)}
+    for name, (a, b) in blocks.items():
+        rel = np.linalg.norm(g[a:b] - gr[a:b]) / max(np.linalg.norm(gr[a:b]), 1e-30)
+        print(f"grad {name}: relative error {rel:.2e}")
+        assert rel < 2e-2, name
+    # the zero pad's weights receive no gradient
+    assert np.all(g[L.OFF_W1 + 31 * L.HID:L.OFF_B1] == 0)
+
+
+def test_mlp_training_fits_a_smooth_target():
+    rng = np.random.default_rng(31)
+    x = rng.uniform(-1, 1, size=(16384, L.FEAT)).astype(np.float32)
+    x[:, 31] = 0
+    t = (np.sin(2 * x[:, 0]) + x[:, 1] * x[:, 2] + 0.5 * np.abs(x[:, 3])).astype(np.float32)
+    with B.ShadowMLP(seed=3) as net:
+        y0 = net.forward(x)
+        l0 = float(np.mean((y0 - t) ** 2))
+        loss = net.train(x, t, epochs=30, batch=512, lr=2e-3)
+        y1 = net.forward(x)
+    l1 = float(np.mean((y1 - t) ** 2))
+    print(f"training: loss {l0:.4f} -> {l1:.4f} (reported {loss:.4f})")
+    assert l1 < 0.05 * l0
+    assert abs(loss - l1) < 0.5 * l0
+
+
+def _scene(seed, n=33, cells=16, light="point"):
+    return scenes.make_scene("learned_small", n, seed=seed, nblobs=12, noise_octaves=2, noise_amp=0.2,
+                             light=light, map_cells=(cells, cells, cells), width=96, height=80)
+
+
+def _ref_scene(sc):
+    return L.Scene(sc.field32, sc.dims, sc.coords, sc.tf32, sc.scalar_range, sc.light_kind,
+                   sc.light_pos, sc.map_cells)
+
+
+@pytest.mark.parametrize("light", ["point", "directional"])
+def test_samples_match_restatement(oracle_mod, light):
+    O = oracle_mod
+    sc = _scene(41, light=light)
+    with B.LightedVolumeRenderer() as r:
+        scenes.configure(r, sc)
+        x, t = r.LearnedSamples(400, seed=9, mode=B.SHADOW_REF_MARCH)
+        gmap = r.GetMap(sc.map_cells)
+    vol = O.volume_from_scene(sc, "f32")
+    omap = O.tmap_build(vol)
+    assert np.array_equal(gmap, omap)
+    rs = _ref_scene(sc)
+    nv = int(np.prod([c + 1 for c in sc.map_cells]))
+    ids = L.sample_ids(9, 400, nv)
+    ctau = rs.coarse_tau(rs.coarse_sigma())
+    f, tc = rs.features(ids, ctau)
+    assert np.allclose(x[:, 0], f[:, 0], rtol=1e-4, atol=1e-5)
+    assert np.array_equal(x[:, 1:28], f[:, 1:28].astype(np.float32))
+    assert np.allclose(x[:, 28:31], f[:, 28:31], atol=1e-6)
+    assert np.all(x[:, 31] == 0)
+    assert np.allclose(t, L.target(tc, omap[ids]), rtol=1e-4, atol=1e-4)
+
+
+def test_learned_map_and_check(oracle_mod):
+    O = oracle_mod
+    sc = _scene(51)
+    with B.ShadowMLP(seed=5) as net:
+        p = _random_params(52) * 0.2
+        net.set_params(p)
+        with B.LightedVolumeRenderer() as r:
+            scenes.configure(r, sc)
+            r.SetShadow(True, sc.map_cells, B.F32, mode=B.SHADOW_LEARNED)
+            r.SetLearned(net, fallback=B.SHADOW_REF_MARCH, error_budget=1.0, check_stride=4)
+            o, d = B.camera_rays(sc.camera, sc.width, sc.height)
+            rays = r.Render(B.Rays(o, d))
+            lmap = r.GetMap(sc.map_cells)
+            e_learned, e_coarse, fell = r.GetLearnedStats()
+        rs = _ref_scene(sc)
+        nv = lmap.size
+        ctau = rs.coarse_tau(rs.coarse_sigma())
+        f, tc = rs.features(np.arange(nv), ctau)
+        y = net.forward(f.astype(np.float32))
+    assert not fell
+    ref = L.learned_opacity(tc, y)
+    err = np.abs(lmap - ref)
+    print(f"learned map vs restatement: max {err.max():.2e}")
+    assert err.max() < 2e-3
+    # the check: every 4th vertex per axis (and the last) against the exact light march
+    vol = O.volume_from_scene(sc, "f32")
+    omap = O.tmap_build(vol).reshape([c + 1 for c in sc.map_cells][::-1])
+    sel = np.arange(0, sc.map_cells[0] + 1, 4)
+    chk = np.ix_(sel, sel, sel)
+    lm = lmap.reshape(omap.shape)
+    assert e_learned == pytest.approx(np.abs(lm[chk] - omap[chk]).max(), abs=1e-6)
+    cm = L.learned_opacity(tc, 0.0).reshape(omap.shape)
+    assert e_coarse == pytest.approx(np.abs(cm[chk] - omap[chk]).max(), abs=1e-4)
+    # the march over the learned map is the exact sampler over that map
+    rb = O.RayBuf(o, d, "f32")
+    O.sample_rays(vol, lmap, rb)
+    assert np.array_equal(rays.rgba, rb.rgba)
+
+
+def test_learned_fallback_is_the_exact_path(oracle_mod):
+    O = oracle_mod
+    sc = _scene(61, light="directional")
+    with B.ShadowMLP(seed=6) as net:
+        with B.LightedVolumeRenderer() as r:
+            scenes.configure(r, sc)
+            r.SetShadow(True, sc.map_cells, B.F32, mode=B.SHADOW_LEARNED)
+            r.SetLearned(net, fallback=B.SHADOW_REF_MARCH, error_budget=0.0, check_stride=8)
+            o, d = B.camera_rays(sc.camera, sc.width, sc.height)
+            rays = r.Render(B.Rays(o, d))
+            gmap = r.GetMap(sc.map_cells)
+            e_learned, _, fell = r.GetLearnedStats()
+    assert fell and e_learned > 0
+    rb = O.RayBuf(o, d, "f32")
+    O.render_rays(O.volume_from_scene(sc, "f32"), rb)
+    assert np.array_equal(gmap, O.tmap_build(O.volume_from_scene(sc, "f32")))
+    assert np.array_equal(rays.rgba, rb.rgba)
+
+
+def test_learned_gain_over_coarse_only(oracle_mod):
+    """Trained on four seeded fields' sweep maps, evaluated on two held-out fields against the
+    exact light march: the network's correction lowers the mean opacity error of the coarse
+    optical depth."""
+    O = oracle_mod
+    n, cells = 48, 32
+    xs, ts = [], []
+    for seed in (101, 102, 103, 104):
+        sc = _scene(seed, n=n, cells=cells)
+        with B.LightedVolumeRenderer() as r:
+            scenes.configure(r, sc)
+            x, t = r.LearnedSamples(20000, seed=seed, mode=B.SHADOW_SWEEP)
+        xs.append(x)
+        ts.append(t)
+    x = np.concatenate(xs)
+    t = np.concatenate(ts)
+    perm = np.random.default_rng(0).permutation(len(t))
+    x, t = x[perm], t[perm]
+    with B.ShadowMLP(seed=7) as net:
+        loss = net.train(x, t, epochs=40, batch=1024, lr=1e-3)
+        out = []
+        for seed in (201, 202):
+            sc = _scene(seed, n=n, cells=cells)
+            res = {}
+            for name, model in (("learned", net), ("coarse", None)):
+                with B.LightedVolumeRenderer() as r:
+                    scenes.configure(r, sc)
+                    r.SetShadow(True, sc.map_cells, B.F32, mode=B.SHADOW_LEARNED)
+                    r.SetLearned(model, error_budget=1.0)
+                    r.Render(B.Rays(*B.camera_rays(sc.camera, 8, 8)))
+                    res[name] = r.GetMap(sc.map_cells)
+            exact = O.tmap_build(O.volume_from_scene(sc, "f32"))
+            el = np.abs(res["learned"] - exact)
+            ec = np.abs(res["coarse"] - exact)
+            print(f"seed {seed}: loss {loss:.4f}; mean |err| learned {el.mean():.4f} coarse {ec.mean():.4f}; "
+                  f"max learned {el.max():.4f} coarse {ec.max():.4f}")
+            out.append((el.mean(), ec.mean()))
+    for learned, coarse in out:
+        assert learned < 0.8 * coarse
