@@ -38,6 +38,15 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-variants", action="store_true", help="skip the FP16/BF16 storage variants")
     ap.add_argument("--no-bricks", action="store_true", help="skip the batched-bricks workload")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-closure workloads")
+    ap.add_argument("--no-series", action="store_true", help="skip the 1024^3 time-series workload")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the RGBA error report against the FP64 oracle")
+    ap.add_argument("--learned-size", type=int, default=512)
+    ap.add_argument("--learned-samples", type=int, default=4 << 20)
+    ap.add_argument("--learned-epochs", type=int, default=12)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device 5 times (the byte-counter calibration run)")
     return ap.parse_args()
 
 
@@ -108,6 +117,246 @@ def bricks_workload(steps, warmup, n=512, nbricks=96, img=1024):
     }
 
 
+class DevView:
+    """A device address (an allocation's base plus a byte offset) for device-memory calls."""
+
+    is_cuda = True
+
+    def __init__(self, addr):
+        self.addr = int(addr)
+
+    def data_ptr(self):
+        return self.addr
+
+
+def image_error(a, b):
+    d = abs(a.astype("float64") - b.astype("float64"))
+    return {"max": float(d.max()), "mean": float(d.mean())}
+
+
+def learned_scene(seed, n, img, **kw):
+    from pilot_amd import beams as B
+    from pilot_amd import scenes
+
+    return scenes.make_scene("learned", n, seed=seed, nblobs=64, noise_octaves=3, noise_amp=0.2,
+                             light="point", width=img, height=img, field_dtype=B.F16,
+                             map_dtype=B.F16, device=True, **kw)
+
+
+def train_closure(n, seeds, per_field, epochs, batch=4096, lr=1e-3):
+    """The network trained on per_field map vertices of each seeded field's exact-sweep map
+    (features and targets drawn on the GPU, shuffled once on the host, Adam on the GPU)."""
+    import numpy as np
+
+    from pilot_amd import beams as B
+    from pilot_amd import scenes
+    from pilot_amd.devmem import DeviceArray
+
+    total = per_field * len(seeds)
+    x = DeviceArray((total, B.MLP_FEATURES), np.float32)
+    t = DeviceArray((total,), np.float32)
+    t0 = time.perf_counter()
+    for i, s in enumerate(seeds):
+        sc = learned_scene(s, n, 64)
+        with B.LightedVolumeRenderer() as r:
+            scenes.configure(r, sc)
+            r.LearnedSamples(per_field, seed=s, mode=B.SHADOW_SWEEP,
+                             features=DevView(x.data_ptr() + i * per_field * B.MLP_FEATURES * 4),
+                             targets=DevView(t.data_ptr() + i * per_field * 4))
+        sc.field_store.free()
+    t_samples = time.perf_counter() - t0
+    perm = np.random.default_rng(0).permutation(total)
+    xs = DeviceArray.from_host(x.numpy()[perm])
+    ts = DeviceArray.from_host(t.numpy()[perm])
+    x.free()
+    t.free()
+    net = B.ShadowMLP(seed=1)
+    t0 = time.perf_counter()
+    losses = []
+    for stage_lr in (lr, lr, lr * 0.3):
+        losses.append(net.train(xs, ts, epochs=max(1, epochs // 3), batch=batch, lr=stage_lr))
+    t_train = time.perf_counter() - t0
+    xs.free()
+    ts.free()
+    steps = 3 * max(1, epochs // 3) * ((total + batch - 1) // batch)
+    return net, {"samples": total, "fields": len(seeds), "sample_s": t_samples, "train_s": t_train,
+                 "optimizer_steps": steps, "batch": batch, "loss_per_stage": losses}
+
+
+def learned_workload(steps, warmup, net, n=512, img=2048, seeds=(2001, 2002, 2003, 2004),
+                     budget=2e-3):
+    """Held-out fields: the exact paths (reference per-vertex march and the exact sweep) and
+    the learned closure (with and without its fallback) on the same frame."""
+    import numpy as np
+
+    from pilot_amd import beams as B
+    from pilot_amd import scenes
+    from pilot_amd.devmem import DeviceArray
+
+    rgba = DeviceArray((img, img, 4), np.float32)
+    depth = DeviceArray((img, img), np.float32)
+    out = []
+    for s in seeds:
+        sc = learned_scene(s, n, img)
+        row = {"seed": s}
+        images = {}
+        for label, mode, budget_ in (("exact_ref_march", B.SHADOW_REF_MARCH, None),
+                                     ("exact_sweep", B.SHADOW_SWEEP, None),
+                                     ("learned_no_fallback", B.SHADOW_LEARNED, 1e30),
+                                     ("learned", B.SHADOW_LEARNED, budget)):
+            with B.LightedVolumeRenderer() as r:
+                scenes.configure(r, sc)
+                r.SetShadow(True, sc.map_cells, B.F16, mode=mode)
+                if mode == B.SHADOW_LEARNED:
+                    r.SetLearned(net, fallback=B.SHADOW_SWEEP, error_budget=budget_, check_stride=8)
+                sec, stats, cam_s, _ = frame_loop(r, sc, rgba, depth, steps, warmup)
+                images[label] = rgba.numpy()
+                row[label] = {"frame_ms": sec * 1e3, "map_ms": stats["map"], "march_ms": stats["march"],
+                              "samples_per_frame": cam_s, "ray_samples_per_s": cam_s / sec}
+                if mode == B.SHADOW_LEARNED:
+                    e_l, e_c, fb = r.GetLearnedStats()
+                    row[label].update({"check_max_opacity_err": e_l, "coarse_only_check_err": e_c,
+                                       "fell_back": fb})
+            print(f"[bench] learned seed {s} {label}: {sec * 1e3:.2f} ms", file=sys.stderr, flush=True)
+        for label in ("exact_sweep", "learned_no_fallback", "learned"):
+            row[label]["rgba_err_vs_exact_ref_march"] = image_error(images[label], images["exact_ref_march"])
+        out.append(row)
+        sc.field_store.free()
+    rgba.free()
+    depth.free()
+
+    def mean(key, sub):
+        return float(np.mean([r[key][sub] for r in out]))
+
+    return {
+        "workload": (f"learned shadow closure: {n}^3 FP16 blob+noise fields, {n - 1}^3-cell FP16 maps, "
+                     f"{img}x{img}, point light, step 0.5 voxel; held-out seeds {list(seeds)}; "
+                     f"fallback to the exact sweep above a {budget} opacity budget at every 8th vertex"),
+        "frame_ms": {k: mean(k, "frame_ms") for k in ("exact_ref_march", "exact_sweep",
+                                                      "learned_no_fallback", "learned")},
+        "speedup_learned_no_fallback_vs_ref_march": mean("exact_ref_march", "frame_ms") /
+        mean("learned_no_fallback", "frame_ms"),
+        "speedup_learned_no_fallback_vs_sweep": mean("exact_sweep", "frame_ms") /
+        mean("learned_no_fallback", "frame_ms"),
+        "per_seed": out,
+    }
+
+
+def time_series_workload(steps, warmup, net, n=1024, img=2048, timesteps=8, nbricks=128):
+    """Drifting 1024^3 FP16 field over 8 timesteps in 128 kd bricks (one ghost layer, maps at
+    half the voxel resolution), the shadow maps rebuilt every timestep, one hipGraph per frame
+    for the exact path; exact and learned side by side."""
+    import numpy as np
+
+    from pilot_amd import beams as B
+    from pilot_amd import scenes
+    from pilot_amd.devmem import DeviceArray
+
+    ranges = B.kd_bricks((n, n, n), nbricks, 5, 96, 320)
+    cells = np.maximum((ranges[:, 1::2] - ranges[:, 0::2]) // 2, 1)
+    rgba = DeviceArray((img, img, 4), np.float32)
+    depth = DeviceArray((img, img), np.float32)
+    rows = []
+    for ts in range(timesteps):
+        sc = scenes.make_scene("series", n, seed=9, nblobs=64, noise_octaves=3, noise_amp=0.2,
+                               light="point", width=img, height=img, field_dtype=B.F16,
+                               map_dtype=B.F16, device=True, time=float(ts), drift_speed=0.01,
+                               noise_phase_rate=0.25)
+        row = {"timestep": ts}
+        imgs = {}
+        for label, mode in (("exact", B.SHADOW_REF_MARCH), ("learned", B.SHADOW_LEARNED)):
+            with B.LightedVolumeRenderer() as r:
+                scenes.configure(r, sc)
+                r.SetShadow(True, sc.map_cells, B.F16, mode=mode)
+                if mode == B.SHADOW_LEARNED:
+                    r.SetLearned(net, fallback=B.SHADOW_REF_MARCH, error_budget=2e-3, check_stride=8)
+                r.SetBricks(ranges, ghost=1, map_cells=cells)
+                r.SetGraph(True)
+                sec, stats, cam_s, _ = frame_loop(r, sc, rgba, depth, steps, warmup)
+                imgs[label] = rgba.numpy()
+                row[label] = {"frame_ms": sec * 1e3, "phase_ms": stats, "samples_per_frame": cam_s,
+                              "ray_samples_per_s": cam_s / sec}
+                if mode == B.SHADOW_LEARNED:
+                    e_l, _, fb = r.GetLearnedStats()
+                    row[label].update({"check_max_opacity_err": e_l, "bricks_fell_back": fb})
+            print(f"[bench] series t={ts} {label}: {sec * 1e3:.2f} ms", file=sys.stderr, flush=True)
+        row["learned"]["rgba_err_vs_exact"] = image_error(imgs["learned"], imgs["exact"])
+        rows.append(row)
+        sc.field_store.free()
+    rgba.free()
+    depth.free()
+    ext = ranges[:, 1::2] - ranges[:, 0::2]
+    return {
+        "workload": (f"time series: {n}^3 FP16 blob+noise field, {timesteps} timesteps (blob drift 0.01 "
+                     f"box/step, noise phase +0.25/step), {nbricks} kd bricks (cell extents "
+                     f"{int(ext.min())}..{int(ext.max())}, 1 ghost layer, maps at half resolution), "
+                     f"{img}x{img}, maps rebuilt every timestep; exact = multi-rank exchange in one "
+                     "hipGraph per frame, learned = per-brick closure with per-brick fallback"),
+        "frame_ms_exact": float(np.mean([r["exact"]["frame_ms"] for r in rows])),
+        "frame_ms_learned": float(np.mean([r["learned"]["frame_ms"] for r in rows])),
+        "per_timestep": rows,
+    }
+
+
+def precision_report(n, img, threads, net=None):
+    """Every storage and shadow variant of the precision workload against the FP64 oracle:
+    the reference algorithm in FP64 on the exact (FP64) field with FP64 camera rays."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from pilot_amd import beams as B
+    from pilot_amd import scenes
+
+    def scene(fdt, mdt):
+        return scenes.make_scene("precision", n, seed=2, nblobs=64, noise_octaves=3, noise_amp=0.2,
+                                 light="point", width=img, height=img, field_dtype=fdt, map_dtype=mdt)
+
+    sc = scene(B.F32, B.F32)
+    o, d, o64, d64 = B.camera_rays(sc.camera, img, img, want64=True)
+    t0 = time.perf_counter()
+    rb = O.RayBuf(o64, d64, "f64")
+    O.render_rays(O.volume_from_scene(sc, "f64"), rb, threads)
+    t_oracle = time.perf_counter() - t0
+    ref = rb.rgba
+    variants = [("f32 field / f32 map, reference march", B.F32, B.F32, B.SHADOW_REF_MARCH),
+                ("f16 field / f16 map, reference march", B.F16, B.F16, B.SHADOW_REF_MARCH),
+                ("bf16 field / f16 map, reference march", B.BF16, B.F16, B.SHADOW_REF_MARCH),
+                ("f32 field / f32 map, exact sweep", B.F32, B.F32, B.SHADOW_SWEEP),
+                ("f16 field / f16 map, exact sweep", B.F16, B.F16, B.SHADOW_SWEEP)]
+    if net is not None:
+        variants.append(("f16 field / f16 map, learned (no fallback)", B.F16, B.F16, B.SHADOW_LEARNED))
+    rows = {}
+    for label, fdt, mdt, mode in variants:
+        scv = sc if (fdt, mdt) == (B.F32, B.F32) else scene(fdt, mdt)
+        with B.LightedVolumeRenderer() as r:
+            scenes.configure(r, scv)
+            r.SetShadow(True, scv.map_cells, mdt, mode=mode)
+            if mode == B.SHADOW_LEARNED:
+                r.SetLearned(net, error_budget=1e30)
+            out, _ = r.RenderImage(scv.camera, img, img)
+        rows[label] = image_error(out.reshape(-1, 4), ref)
+        print(f"[bench] precision {label}: {rows[label]}", file=sys.stderr, flush=True)
+    return {
+        "workload": (f"precision sweep: {n}^3 blob+noise field, {n - 1}^3-cell map, point light, "
+                     f"{img}x{img}; per-pixel RGBA error against the FP64 oracle "
+                     f"({threads} threads, {t_oracle:.1f} s)"),
+        "rgba_err_vs_fp64_oracle": rows,
+    }
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the byte-counter runs summarised in
+    profiles/traffic.json (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the counters' reading
+    of a copy of known size), or None when that workload was not counted."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        entry = json.load(f).get(workload, {}).get(kernel)
+    return None if entry is None else entry["bytes_per_launch"]
+
+
 def cpu_baseline(sc, field32, gpu_map, threads, samples_per_frame):
     """Oracle (FP32 restatement of the reference) on a bounded sample of the same frame:
     a z-slab of the map and a centred ray tile, extrapolated to the full frame."""
@@ -163,7 +412,16 @@ def main():
 
     from pilot_amd import beams as B
     from pilot_amd import scenes
-    from pilot_amd.devmem import DeviceArray
+    from pilot_amd.devmem import D2D, DeviceArray, hip
+
+    if args.copy_calibration:
+        a = DeviceArray((1 << 28,), np.float32).zero()
+        b = DeviceArray((1 << 28,), np.float32)
+        for _ in range(5):
+            hip().hipMemcpy(b.data_ptr(), a.data_ptr(), a.nbytes, D2D)
+        hip().hipDeviceSynchronize()
+        print(json.dumps({"copy_calibration_bytes": a.nbytes, "copies": 5}))
+        return
 
     n, img = args.size, args.image
 
@@ -236,7 +494,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom, f"{n}^3 f32/f32 {img}x{img}"),
             "algorithmic_bytes": dom_bytes,
         },
         "cpu_baseline": None,
@@ -245,10 +503,25 @@ def main():
         line["workloads"] = {"bricks_512_96": bricks_workload(max(3, args.steps // 2), 1)}
         print(f"[bench] bricks: {line['workloads']['bricks_512_96']['frame_ms_graph']:.3f} ms/frame",
               file=sys.stderr, flush=True)
+    net = None
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    if not args.no_learned:
+        seeds = tuple(1001 + i for i in range(8))
+        net, train = train_closure(args.learned_size, seeds, args.learned_samples // len(seeds),
+                                   args.learned_epochs)
+        print(f"[bench] closure trained: {train}", file=sys.stderr, flush=True)
+        wl = learned_workload(max(2, args.steps // 4), 1, net, n=args.learned_size)
+        wl["training"] = train
+        line.setdefault("workloads", {})["learned"] = wl
+        if not args.no_series:
+            line["workloads"]["time_series"] = time_series_workload(2, 1, net)
+    if not args.no_precision:
+        line["precision"] = precision_report(n, img, threads, net)
+    if net is not None:
+        net.close()
     if not args.no_cpu:
         field32 = sc.field_store.numpy().astype(np.float32)
         gmap = r.GetMap(sc.map_cells)
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(sc, field32, gmap, threads, cam_s)
     r.close()
     print(json.dumps(line))

@@ -149,7 +149,8 @@ int beams_set_light(beams_renderer* h, int32_t kind, const float xyz[3], const f
 
 /* ---- SetUseShadowMap / SetShadowMapSize (LightedVolumeRenderer.h:59-62) ------------------
  * map_cells: cells per axis of the transmittance map (the map has cells+1 vertices).
- * mode: BEAMS_SHADOW_REF_MARCH.  map_dtype: BEAMS_F32 or BEAMS_F16 storage. */
+ * mode: BEAMS_SHADOW_REF_MARCH (the reference), BEAMS_SHADOW_SWEEP or BEAMS_SHADOW_LEARNED
+ * (extensions; bricks take REF_MARCH or LEARNED).  map_dtype: BEAMS_F32 or BEAMS_F16 storage. */
 int beams_set_shadow(beams_renderer* h, int32_t enabled, const int64_t map_cells[3],
                      int32_t mode, int32_t map_dtype);
 
@@ -273,7 +274,9 @@ int beams_mlp_train(beams_mlp* h, const float* x, const float* t, int64_t n, int
 /* Attach the network used by BEAMS_SHADOW_LEARNED (net NULL: the coarse optical depth alone).
  * fallback_mode (BEAMS_SHADOW_REF_MARCH or BEAMS_SHADOW_SWEEP) builds the map instead when the
  * largest opacity error at the check vertices (every check_stride-th map vertex per axis and
- * the last) exceeds error_budget.  The renderer keeps the pointer: destroy the renderer or
+ * the last) exceeds error_budget.  With bricks the check and the fallback are per brick: each
+ * brick's map is predicted from the whole volume (no non-local exchange), and a brick over
+ * the budget is re-marched through the whole volume with the single-block light march.  The renderer keeps the pointer: destroy the renderer or
  * attach another network before destroying net. */
 int beams_set_learned(beams_renderer* h, const beams_mlp* net, int32_t fallback_mode,
                       float error_budget, int32_t check_stride);
@@ -283,8 +286,9 @@ int beams_set_learned(beams_renderer* h, const beams_mlp* net, int32_t fallback_
 int beams_learned_samples(beams_renderer* h, int32_t mode, int64_t n, uint64_t seed,
                           float* features, float* targets, int32_t mem);
 /* The last learned map build: err[0] = largest |learned - exact| opacity at the check
- * vertices, err[1] = the same for the coarse optical depth alone; *fell_back = 1 when the
- * fallback mode built the map. */
+ * vertices, err[1] = the same for the coarse optical depth alone (single block only, else -1);
+ * *fell_back = the number of maps the fallback built (0 or 1 for the single block; with
+ * bricks, the bricks over the budget, whose maps the whole volume's light march rebuilds). */
 int beams_get_learned_stats(beams_renderer* h, float err[2], int32_t* fell_back);
 
 #ifdef __cplusplus

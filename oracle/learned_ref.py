@@ -23,7 +23,7 @@ OFF_W3 = OFF_B2 + HID
 OFF_B3 = OFF_W3 + HID
 NPARAMS = OFF_B3 + 1
 COARSE_CELLS = 32
-TAU_MAX = 16.0
+TAU_MAX = 8.0
 
 
 def bf16(a):
@@ -126,6 +126,11 @@ class Scene:
     def alpha(self, val):
         return self.a_tab[self.tf_index(val)].astype(np.float64)
 
+    def patch(self, val):
+        """Density-patch feature: -ln(1 - min(a, 1 - 2^-24)) / 4, in FP32 like the product."""
+        a = np.minimum(self.a_tab[self.tf_index(val)], np.float32(1.0 - 2.0 ** -24))
+        return (-np.log1p(-a) * np.float32(0.25)).astype(np.float32).astype(np.float64)
+
     def sigma(self, val):
         a = np.minimum(self.alpha(val), 1.0 - 2.0 ** -24)
         return -np.log1p(-a) / self.h
@@ -220,7 +225,7 @@ class Scene:
         v = self.map_vertices(ids)
         f = np.zeros((len(ids), FEAT))
         tc = np.minimum(self.coarse_lerp(ctau, v), TAU_MAX)
-        f[:, 0] = tc / 8
+        f[:, 0] = tc / 4
         c = np.array([[self.nearest(d, vv[d]) for d in range(3)] for vv in v], np.int64)
         q = 1
         for dz in (-1, 0, 1):
@@ -229,14 +234,15 @@ class Scene:
                     i = np.clip(c[:, 0] + dx, 0, self.dims[0] - 1)
                     j = np.clip(c[:, 1] + dy, 0, self.dims[1] - 1)
                     k = np.clip(c[:, 2] + dz, 0, self.dims[2] - 1)
-                    f[:, q] = self.alpha(self.f3[k, j, i])
+                    f[:, q] = self.patch(self.f3[k, j, i])
                     q += 1
         f[:, 28:31] = self.light_dir(v)
         return f, tc
 
 
 def target(tc, exact_opacity):
-    te = np.minimum(-np.log(np.maximum(1.0 - np.asarray(exact_opacity, np.float64), 1.12535175e-7)), TAU_MAX)
+    with np.errstate(divide="ignore"):
+        te = np.minimum(-np.log(1.0 - np.asarray(exact_opacity, np.float64)), TAU_MAX)
     return tc - te
 
 

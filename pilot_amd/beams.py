@@ -392,7 +392,7 @@ class LightedVolumeRenderer:
         e = np.zeros(2, np.float32)
         fb = C.c_int32()
         self._c(lib().beams_get_learned_stats(self._h, _f32p(e), C.byref(fb)))
-        return float(e[0]), float(e[1]), bool(fb.value)
+        return float(e[0]), float(e[1]), int(fb.value)
 
 
 class ShadowMLP:

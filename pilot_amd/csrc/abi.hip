@@ -121,6 +121,7 @@ struct beams_renderer {
   std::vector<Brick> bricks;
   int32_t ghost = 0;
   DevBuf bmap_local, bmap_final, bseeds, btables, bpartial, bqueue;
+  DevBuf bredo;  // learned bricks: per-brick fallback flags
   std::vector<unsigned char> btab_host;  // tables of the last frame (reused while unchanged)
   int32_t n_map_items = 0, n_px_items = 0, max_coords = 0;
   int32_t persistent_groups = 2048;
@@ -702,13 +703,66 @@ int check_overflow(beams_renderer* h)
 }
 
 // RenderOnDevice phases 1-4 for every brick + GlobalComposite + BlendBackground.
+// Learned bricks: every brick's final map from the whole volume's coarse optical depth and
+// the network, a per-brick sparse check against the whole volume's light march, and that
+// march for the bricks over the budget (timed as phases 1 and 3).  The whole field stays
+// resident beside the bricks' ghosted copies.
+int build_bricks_learned(beams_renderer* h, const BrickFrame& f)
+{
+  const int nb = f.nbricks;
+  const size_t nc = (size_t)(kCoarseCells + 1) * (kCoarseCells + 1) * (kCoarseCells + 1);
+  HIP_TRY(h, h->coarse.ensure(2 * nc * sizeof(float)));
+  HIP_TRY(h, h->lerr.ensure(sizeof(float) * (size_t)nb));
+  HIP_TRY(h, h->bredo.ensure(sizeof(int32_t) * (size_t)nb));
+  const GridParams g = grid_params(h);
+  const float user_sd = h->sample_distance;
+  ShadeParams s = shade_params(h, g);
+  h->sample_distance = user_sd;
+  s.counters = nullptr;
+  const LearnedParams p = learned_params(h, g, s, map_params(h, g, BEAMS_SHADOW_LEARNED));
+  int rc;
+  if ((rc = phase_mark(h, BEAMS_PHASE_MAP, false))) return rc;
+  HIP_TRY(h, launch_coarse(p, h->stream));
+  HIP_TRY(h, launch_bricks_learned_map(p, f, h->map_dtype, h->stream));
+  HIP_TRY(h, launch_bricks_learned_check(p, f, h->map_dtype, (float*)h->lerr.p, h->stream));
+  if ((rc = phase_mark(h, BEAMS_PHASE_MAP, true))) return rc;
+  std::vector<float> err((size_t)nb);
+  HIP_TRY(h, hipMemcpyAsync(err.data(), h->lerr.p, sizeof(float) * (size_t)nb, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  std::vector<int32_t> redo((size_t)nb, 0);
+  int32_t count = 0;
+  float worst = 0.0f;
+  for (int b = 0; b < nb; ++b) {
+    worst = std::fmax(worst, err[(size_t)b]);
+    if (!(err[(size_t)b] <= h->error_budget)) {
+      redo[(size_t)b] = 1;
+      ++count;
+    }
+  }
+  h->last_err[0] = worst;
+  h->last_err[1] = -1.0f;
+  h->fell_back = count;
+  if (count) {
+    HIP_TRY(h, hipMemcpyAsync(h->bredo.p, redo.data(), sizeof(int32_t) * (size_t)nb, hipMemcpyHostToDevice,
+                              h->stream));
+    if ((rc = phase_mark(h, BEAMS_PHASE_REMAP, false))) return rc;
+    HIP_TRY(h, launch_bricks_exact_map(p, f, h->map_dtype, (const int32_t*)h->bredo.p, h->stream));
+    if ((rc = phase_mark(h, BEAMS_PHASE_REMAP, true))) return rc;
+    // the host vector must outlive the asynchronous copy
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+  }
+  return BEAMS_OK;
+}
+
 int render_bricks(beams_renderer* h, const CameraParams& c, const float* background, float* rgba,
                   float* depth, int32_t mem)
 {
   int rc = check_overflow(h);
   if (rc) return rc;
-  if (h->use_shadow && h->shadow_mode != BEAMS_SHADOW_REF_MARCH)
-    return h->fail(BEAMS_ERR_UNSUPPORTED, "bricks build their maps with the reference march");
+  if (h->use_shadow && h->shadow_mode == BEAMS_SHADOW_SWEEP)
+    return h->fail(BEAMS_ERR_UNSUPPORTED,
+                   "bricks build their maps with the reference march or the learned closure");
+  const bool learned = h->use_shadow && h->shadow_mode == BEAMS_SHADOW_LEARNED;
   const size_t msz = dtype_size(h->map_dtype);
   int64_t nv_total = 0;
   for (const auto& br : h->bricks) nv_total += br.nverts;
@@ -735,11 +789,12 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
   }
   o.use_bg = background ? 1 : 0;
   for (int k = 0; k < 4; ++k) o.bg[k] = background ? background[k] : 0.0f;
+  if (learned && (rc = build_bricks_learned(h, f))) return rc;
   auto issue = [&](bool timed) -> int {
     int r;
     auto mark = [&](int p, bool end) { return timed ? phase_mark(h, p, end) : BEAMS_OK; };
     HIP_TRY(h, launch_zero_u32(f.queue, 5, h->stream));
-    if (shadows) {
+    if (shadows && !learned) {
       if ((r = mark(BEAMS_PHASE_MAP, false))) return r;
       HIP_TRY(h, launch_bricks_map(f, h->dtype, h->map_dtype, ntf, false, h->stream));
       if ((r = mark(BEAMS_PHASE_MAP, true))) return r;
@@ -758,7 +813,8 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
     return mark(BEAMS_PHASE_COMPOSITE, true);
   };
   KeyBuilder key;
-  key.put(1).put(f).put(c).put(o).put(h->btab_version).put(shadows).put(h->dtype).put(h->map_dtype).put(ntf);
+  key.put(1).put(f).put(c).put(o).put(h->btab_version).put(shadows).put(learned).put(h->dtype)
+     .put(h->map_dtype).put(ntf);
   if ((rc = run_frame(h, key.bytes, issue))) return rc;
   if (shadows) {
     h->map_valid = true;

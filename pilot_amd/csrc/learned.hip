@@ -51,6 +51,13 @@ __device__ __forceinline__ void stage_alpha(const ShadeParams& s, float* a_tab)
   for (int i = threadIdx.x; i < s.ntf; i += blockDim.x) a_tab[i] = s.tf[4 * i + 3] / s.max_density;
 }
 
+// Density-patch feature per TF entry: -ln(1 - min(a, 1 - 2^-24)) / 4.
+__device__ __forceinline__ void stage_patch(const ShadeParams& s, float* d_tab)
+{
+  for (int i = threadIdx.x; i < s.ntf; i += blockDim.x)
+    d_tab[i] = -log1pf(-fminf(s.tf[4 * i + 3] / s.max_density, 1.0f - 5.9604645e-8f)) * 0.25f;
+}
+
 // Trilinear interpolation in a cn^3 vertex grid (x fastest), p clamped to the grid.
 __device__ __forceinline__ float coarse_lerp(const LearnedParams& p, const float* grid, const float* v)
 {
@@ -92,14 +99,14 @@ __device__ __forceinline__ void light_dir(const MapParams& m, const float* v, fl
   d[2] = dd[2] * rs;
 }
 
-// Features of map vertex v; returns min(tau_c, 16).
+// Features of map vertex v (d_tab: the staged density-patch table); returns min(tau_c, 8).
 template <int FDT>
-__device__ __forceinline__ float features(const LearnedParams& p, const float* a_tab, const float* v,
+__device__ __forceinline__ float features(const LearnedParams& p, const float* d_tab, const float* v,
                                           float* f)
 {
   const GridParams& g = p.g;
   const float tc = fminf(coarse_lerp(p, p.coarse_tau, v), kTauMax);
-  f[0] = tc * 0.125f;
+  f[0] = tc * 0.25f;
   int64_t c[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) c[d] = nearest_index(g.x[d], g.n[d], v[d], g.gx0[d], g.ginv[d]);
@@ -117,7 +124,7 @@ __device__ __forceinline__ float features(const LearnedParams& p, const float* a
         int64_t i = c[0] + dx;
         i = i < 0 ? 0 : (i > g.n[0] - 1 ? g.n[0] - 1 : i);
         const float val = load_t<FDT>(g.field, i * g.field_stride[0] + j * g.field_stride[1] + k * g.field_stride[2]);
-        f[q++] = a_tab[tf_index(p.s, val)];
+        f[q++] = d_tab[tf_index(p.s, val)];
       }
     }
   }
@@ -177,10 +184,12 @@ __global__ void __launch_bounds__(kBlock) k_coarse_sigma(LearnedParams p)
 }
 
 // ---- coarse optical depth: midpoint rule toward the light through the coarse extinction ---
+// One wave per coarse vertex; its lanes take every 64th midpoint and the wave sums them.
 __global__ void __launch_bounds__(kBlock) k_coarse_tau(LearnedParams p)
 {
-  const int64_t cn = p.cn, id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= cn * cn * cn) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t cn = p.cn, id = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (id >= cn * cn * cn) return;  // uniform across the wave
   float v[3];
   v[0] = p.corg[0] + p.csp[0] * (float)(id % cn);
   v[1] = p.corg[1] + p.csp[1] * (float)((id / cn) % cn);
@@ -205,13 +214,15 @@ __global__ void __launch_bounds__(kBlock) k_coarse_tau(LearnedParams p)
     int n = (int)ceilf(len / hstep);
     n = n < 1 ? 1 : (n > 512 ? 512 : n);
     const float dt = len / (float)n;
-    for (int k = 0; k < n; ++k) {
+    for (int k = lane; k < n; k += 64) {
       const float t = ((float)k + 0.5f) * dt;
       const float q[3] = { v[0] - t * d[0], v[1] - t * d[1], v[2] - t * d[2] };
       tau += coarse_lerp(p, p.coarse_sigma, q) * dt;
     }
   }
-  p.coarse_tau[id] = tau;
+#pragma unroll
+  for (int mk = 32; mk >= 1; mk >>= 1) tau += __shfl_xor(tau, mk, 64);
+  if (lane == 0) p.coarse_tau[id] = tau;
 }
 
 // ---- the learned map: 64 vertices per wave, four waves per workgroup -----------------------
@@ -227,7 +238,7 @@ __global__ void __launch_bounds__(kBlock) k_learned_map(LearnedParams p, int64_t
 {
   __shared__ MapLds s;
   if (p.params) mlp_stage(p.params, s.w);
-  stage_alpha(p.s, s.a_tab);
+  stage_patch(p.s, s.a_tab);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
@@ -289,7 +300,7 @@ __global__ void __launch_bounds__(kBlock) k_learned_samples(LearnedParams p, Map
                                                             uint64_t seed, float* x, float* t)
 {
   __shared__ float a_tab[1024];
-  stage_alpha(p.s, a_tab);
+  stage_patch(p.s, a_tab);
   __syncthreads();
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -301,13 +312,144 @@ __global__ void __launch_bounds__(kBlock) k_learned_samples(LearnedParams p, Map
   const float tc = features<FDT>(p, a_tab, v, f);
   const int64_t i = id % em.mp[0], j = (id / em.mp[0]) % em.mp[1], kk = id / (em.mp[0] * em.mp[1]);
   const float op = load_t<EDT>(em.map, i * em.ms[0] + j * em.ms[1] + kk * em.ms[2]);
-  const float te = fminf(-logf(fmaxf(1.0f - op, 1.12535175e-7f)), kTauMax);
+  const float te = fminf(-logf(fmaxf(1.0f - op, 1e-30f)), kTauMax);
 #pragma unroll
   for (int q = 0; q < kFeat; ++q) x[k * kFeat + q] = f[q];
   t[k] = tc - te;
 }
 
+// ---- bricks: every brick's map from the whole volume's coarse optical depth --------------
+// One map work item (256 vertices of one brick) per workgroup iteration, 64 per wave.
+template <int FDT, int MDT>
+__global__ void __launch_bounds__(kBlock) k_bricks_learned_map(LearnedParams p, BrickFrame f)
+{
+  __shared__ MapLds s;
+  if (p.params) mlp_stage(p.params, s.w);
+  stage_patch(p.s, s.a_tab);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int item = blockIdx.x; item < f.n_map_items; item += gridDim.x) {
+    const BrickDesc& d = f.bricks[f.map_item_brick[item]];
+    const int64_t nv = d.nverts;
+    const int64_t id = f.map_item_first[item] + wave * 64 + lane;
+    const bool valid = id < nv;
+    float v[3], feat[kFeat];
+    map_vertex(d.m, valid ? id : nv - 1, v);
+    const float tc = features<FDT>(p, s.a_tab, v, feat);
+    const float y = p.params ? mlp_forward_wave(s.w, s.buf[wave], s.y[wave], feat) : 0.0f;
+    if (valid) store_t<MDT>(d.map_final, id, 1.0f - expf(-fmaxf(tc - y, 0.0f)));
+  }
+}
+
+// The brick's map geometry with the whole volume's light-march step.
+__device__ __forceinline__ MapParams whole_volume_march(const BrickDesc& d, const MapParams& whole)
+{
+  MapParams m = d.m;
+  m.step = whole.step;
+  m.dir_far = whole.dir_far;
+  return m;
+}
+
+// Per brick (one workgroup each): largest |learned - exact| over its check vertices, the exact
+// value being the whole volume's light march (the single-block reference march).
+template <int FDT, int MDT>
+__global__ void __launch_bounds__(kBlock) k_bricks_learned_check(LearnedParams p, BrickFrame f, float* err)
+{
+  extern __shared__ float lds[];
+  float* a_tab = lds;
+  float* xs = lds + p.s.ntf;
+  __shared__ float red[kBlock / 64];
+  stage_alpha(p.s, a_tab);
+  stage_coords(p.g, xs);
+  __syncthreads();
+  const float* x[3] = { xs, xs + p.g.n[0], xs + p.g.n[0] + p.g.n[1] };
+  const BrickDesc& d = f.bricks[blockIdx.x];
+  const MapParams m = whole_volume_march(d, p.m);
+  const int64_t n0 = check_axis(m.mp[0] - 1, p.check_stride);
+  const int64_t n1 = check_axis(m.mp[1] - 1, p.check_stride);
+  const int64_t n2 = check_axis(m.mp[2] - 1, p.check_stride);
+  float worst = 0.0f;
+  for (int64_t q = threadIdx.x; q < n0 * n1 * n2; q += blockDim.x) {
+    const int64_t qi[3] = { q % n0, (q / n0) % n1, q / (n0 * n1) };
+    int64_t vi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int64_t i = qi[a] * p.check_stride;
+      vi[a] = i < m.mp[a] - 1 ? i : m.mp[a] - 1;
+    }
+    const int64_t id = vi[0] + m.mp[0] * (vi[1] + m.mp[1] * vi[2]);
+    int64_t taken = 0;
+    const float exact = march_vertex<FDT>(p.g, m, p.s, a_tab, x, id, 0.0f, &taken);
+    worst = fmaxf(worst, fabsf(load_t<MDT>(d.map_final, id) - exact));
+  }
+#pragma unroll
+  for (int mk = 32; mk >= 1; mk >>= 1) worst = fmaxf(worst, __shfl_xor(worst, mk, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = worst;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float w = red[0];
+    for (int i = 1; i < kBlock / 64; ++i) w = fmaxf(w, red[i]);
+    err[blockIdx.x] = w;
+  }
+}
+
+// Fallback: the whole volume's light march for every vertex of the bricks flagged in redo.
+template <int FDT, int MDT>
+__global__ void __launch_bounds__(kBlock) k_bricks_exact_map(LearnedParams p, BrickFrame f, const int32_t* redo)
+{
+  extern __shared__ float lds[];
+  float* a_tab = lds;
+  float* xs = lds + p.s.ntf;
+  stage_alpha(p.s, a_tab);
+  stage_coords(p.g, xs);
+  __syncthreads();
+  const float* x[3] = { xs, xs + p.g.n[0], xs + p.g.n[0] + p.g.n[1] };
+  for (int item = blockIdx.x; item < f.n_map_items; item += gridDim.x) {
+    const int b = f.map_item_brick[item];
+    if (!redo[b]) continue;
+    const BrickDesc& d = f.bricks[b];
+    const int64_t id = f.map_item_first[item] + threadIdx.x;
+    if (id >= d.nverts) continue;
+    const MapParams m = whole_volume_march(d, p.m);
+    int64_t taken = 0;
+    store_t<MDT>(d.map_final, id, march_vertex<FDT>(p.g, m, p.s, a_tab, x, id, 0.0f, &taken));
+  }
+}
+
 }  // namespace
+
+hipError_t launch_bricks_learned_map(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
+                                     hipStream_t st)
+{
+  if (p.s.ntf > 1024) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)std::max(1, std::min(f.n_map_items, f.persistent_groups)));
+  BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(map_dtype, {
+    hipLaunchKernelGGL((k_bricks_learned_map<FDT, MDT>), grid, dim3(kBlock), 0, st, p, f);
+  }));
+  return hipGetLastError();
+}
+
+hipError_t launch_bricks_learned_check(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
+                                       float* err, hipStream_t st)
+{
+  const size_t lds = sizeof(float) * (size_t)(p.s.ntf + p.g.n[0] + p.g.n[1] + p.g.n[2]);
+  BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(map_dtype, {
+    hipLaunchKernelGGL((k_bricks_learned_check<FDT, MDT>), dim3((unsigned)f.nbricks), dim3(kBlock), lds,
+                       st, p, f, err);
+  }));
+  return hipGetLastError();
+}
+
+hipError_t launch_bricks_exact_map(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
+                                   const int32_t* redo, hipStream_t st)
+{
+  const size_t lds = sizeof(float) * (size_t)(p.s.ntf + p.g.n[0] + p.g.n[1] + p.g.n[2]);
+  const dim3 grid((unsigned)std::max(1, std::min(f.n_map_items, f.persistent_groups)));
+  BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(map_dtype, {
+    hipLaunchKernelGGL((k_bricks_exact_map<FDT, MDT>), grid, dim3(kBlock), lds, st, p, f, redo);
+  }));
+  return hipGetLastError();
+}
 
 int64_t learned_check_count(const MapParams& m, int32_t stride)
 {
@@ -328,7 +470,8 @@ hipError_t launch_coarse(const LearnedParams& p, hipStream_t st)
   });
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_coarse_tau, dim3((unsigned)((nc + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, p);
+  const int64_t per_block = kBlock / 64;
+  hipLaunchKernelGGL(k_coarse_tau, dim3((unsigned)((nc + per_block - 1) / per_block)), dim3(kBlock), 0, st, p);
   return hipGetLastError();
 }
 

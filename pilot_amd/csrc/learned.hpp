@@ -7,15 +7,18 @@
 // reference's map march (TransmittanceMap.h:329).  Optical depth tau = -ln(1 - opacity).
 //
 // Features of a map vertex v (kFeat = 32):
-//   [0]       min(tau_c(v), 16) / 8, tau_c trilinear in the coarse optical-depth grid
-//   [1..27]   a at the 3 x 3 x 3 field vertices around the field vertex nearest v
-//             (index clamped to the grid; dz slowest, dx fastest)
+//   [0]       min(tau_c(v), 8) / 4, tau_c trilinear in the coarse optical-depth grid
+//   [1..27]   the density patch: -ln(1 - min(a, 1 - 2^-24)) / 4 (the optical depth of one
+//             map step, scaled) at the 3 x 3 x 3 field vertices around the field vertex
+//             nearest v (index clamped to the grid; dz slowest, dx fastest)
 //   [28..30]  the light's direction of travel at v
 //   [31]      0
-// Target / output y = min(tau_c, 16) - min(tau_exact, 16); the learned opacity is
-// 1 - exp(-max(min(tau_c, 16) - y, 0)).
+// Target / output y = min(tau_c, 8) - min(tau_exact, 8); the learned opacity is
+// 1 - exp(-max(min(tau_c, 8) - y, 0)).  Depths beyond 8 (transmittance < 3.4e-4) are
+// clamped: the map's shading weight (1 - opacity) is below any error budget there.
 #pragma once
 
+#include "bricks.hpp"
 #include "kernels.hpp"
 
 struct beams_mlp;
@@ -23,7 +26,7 @@ struct beams_mlp;
 namespace beams {
 
 constexpr int kCoarseCells = 32;
-constexpr float kTauMax = 16.0f;
+constexpr float kTauMax = 8.0f;
 
 struct LearnedParams {
   GridParams g;
@@ -47,6 +50,16 @@ hipError_t launch_learned_check(const LearnedParams& p, hipStream_t st);
 hipError_t launch_learned_samples(const LearnedParams& p, const MapParams& em, int64_t n,
                                   uint64_t seed, float* x, float* t, hipStream_t st);
 int64_t learned_check_count(const MapParams& m, int32_t stride);
+
+// Bricks: p describes the whole volume (field, coarse grids, light, map step); each brick's
+// map_final receives its learned map, err[b] its check, and the fallback re-marches the
+// bricks flagged in redo through the whole volume.
+hipError_t launch_bricks_learned_map(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
+                                     hipStream_t st);
+hipError_t launch_bricks_learned_check(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
+                                       float* err, hipStream_t st);
+hipError_t launch_bricks_exact_map(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
+                                   const int32_t* redo, hipStream_t st);
 
 hipError_t launch_mlp_infer(const float* params, const float* x, int64_t n, float* y, hipStream_t st);
 hipError_t launch_mlp_grad(const float* params, const float* x, const float* t, int64_t n,

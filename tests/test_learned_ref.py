@@ -79,20 +79,20 @@ def test_features_of_constant_field():
     sc = L.Scene(np.full(n ** 3, 0.25), (n, n, n), [c, c, c], tf, (0, 1), 1, (0, 0, 1), (4, 4, 4))
     ctau = sc.coarse_tau(sc.coarse_sigma())
     f, tc = sc.features(np.arange(125), ctau)
-    assert np.allclose(f[:, 1:28], 0.25)
+    assert np.allclose(f[:, 1:28], -np.log1p(-0.25) / 4, rtol=1e-6)
     assert np.allclose(f[:, 28:31], [0, 0, 1])
     assert np.all(f[:, 31] == 0)
     # directional light along +z: path from the z = 0 face to the vertex
     z = sc.map_vertices(np.arange(125))[:, 2]
     sig = -np.log1p(-0.25) / sc.h
     assert np.allclose(tc, np.minimum(sig * z, L.TAU_MAX), rtol=1e-9, atol=1e-9)
-    assert np.allclose(f[:, 0], tc / 8)
+    assert np.allclose(f[:, 0], tc / 4)
 
 
 def test_target_round_trip():
     rng = np.random.default_rng(5)
-    op = rng.uniform(0, 0.999, 1000)
-    tc = rng.uniform(0, 16, 1000)
+    op = rng.uniform(0, 1 - np.exp(-L.TAU_MAX) - 1e-6, 1000)
+    tc = rng.uniform(0, L.TAU_MAX, 1000)
     y = L.target(tc, op)
     assert np.allclose(L.learned_opacity(tc, y), op, atol=1e-12)
 
