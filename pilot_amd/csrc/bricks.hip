@@ -67,19 +67,117 @@ __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
   }
 }
 
+// Bounding sphere (centre c, radius R) of the map vertices [first, last] of a brick: the
+// vertices of an x-fastest id range lie in the index box spanned by its rows and slices.
+__device__ __forceinline__ void item_sphere(const MapParams& m, int64_t first, int64_t last, float* c,
+                                            float& R)
+{
+  const int64_t n0 = m.mp[0], n01 = m.mp[0] * m.mp[1];
+  int64_t lo[3] = { first % n0, (first / n0) % m.mp[1], first / n01 };
+  int64_t hi[3] = { last % n0, (last / n0) % m.mp[1], last / n01 };
+  if (lo[2] != hi[2]) {
+    lo[0] = 0, hi[0] = m.mp[0] - 1, lo[1] = 0, hi[1] = m.mp[1] - 1;
+  } else if (lo[1] != hi[1]) {
+    lo[0] = 0, hi[0] = m.mp[0] - 1;
+  }
+  float r2 = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float pl = m.morg[a] + m.msp[a] * (float)lo[a];
+    const float ph = m.morg[a] + m.msp[a] * (float)hi[a];
+    c[a] = 0.5f * (pl + ph);
+    r2 += 0.25f * (ph - pl) * (ph - pl);
+  }
+  R = sqrtf(r2) * 1.0001f + 1e-6f;
+}
+
+// Conservative test: can a light segment ending in the sphere (c, R) meet the sphere (s, r)?
+// Point light at o: the segments lie in the cone from o through the item sphere, no longer
+// than |c - o| + R.  Directional light: they lie in the capsule swept from the item sphere
+// up-light by dir_far.
+__device__ __forceinline__ bool may_cross(const MapParams& m, const float* c, float R, const float* s,
+                                          float r)
+{
+  if (m.light_kind == 1) {
+    // capsule axis from c to c - dir_far * dir
+    float w[3], t = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      w[a] = s[a] - c[a];
+      t -= w[a] * m.light[a];
+    }
+    t = fminf(fmaxf(t, 0.0f), m.dir_far);
+    float d2 = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float q = w[a] + t * m.light[a];
+      d2 += q * q;
+    }
+    const float lim = (R + r) * 1.001f + 1e-5f;
+    return d2 <= lim * lim;
+  }
+  float ax[3], d[3], la2 = 0.0f, ld2 = 0.0f, dot = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    ax[a] = c[a] - m.light[a];
+    d[a] = s[a] - m.light[a];
+    la2 += ax[a] * ax[a];
+    ld2 += d[a] * d[a];
+    dot += ax[a] * d[a];
+  }
+  const float la = sqrtf(la2), ld = sqrtf(ld2);
+  if (la <= R * 1.001f || ld <= r * 1.001f) return true;  // the light inside either sphere
+  if (ld - r > (la + R) * 1.001f) return false;           // beyond every segment's end
+  const float cos_phi = fminf(fmaxf(dot / (la * ld), -1.0f), 1.0f);
+  const float phi = acosf(cos_phi);
+  const float half = asinf(fminf(R / la, 1.0f)) + asinf(fminf(r / ld, 1.0f));
+  return phi <= half * 1.001f + 2e-3f;
+}
+
 // ---- phase 2: non-local hits (TransmittanceMap.h:342-453) + fetch (:505-508) + composite
 // (LightedVolumeRenderer.cxx:1214-1235), one thread per map vertex ------------------------------
+// Per work item the wave first keeps the bricks whose bounding sphere may meet a light segment
+// of the item (conservative; the per-vertex FP32 slab cull and the exact FP64 test follow).
+constexpr int kMaxCandidates = 1024;
+
 template <int MDT>
 __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
 {
   extern __shared__ unsigned long long hits[];  // [kMaxPathHits][64], (t bits << 32) | brick
   __shared__ int slot;
+  __shared__ int cand[kMaxCandidates];
   const int lane = threadIdx.x;
   for (;;) {
     const int item = next_item(f.queue + 1, &slot);
     if (item >= f.n_map_items) break;
     const int b = f.map_item_brick[item];
     const BrickDesc& bd = f.bricks[b];
+    // candidate bricks of this item
+    int ncand = 0;
+    {
+      const int64_t first = f.map_item_first[item];
+      const int64_t last = first + 255 < bd.nverts - 1 ? first + 255 : bd.nverts - 1;
+      float c[3], R;
+      item_sphere(bd.m, first, last, c, R);
+      for (int j0 = 0; j0 < f.nbricks; j0 += 64) {
+        const int j = j0 + lane;
+        bool keep = false;
+        if (j < f.nbricks && j != b) {
+          const float* bx = f.bricks[j].box32;
+          const float s[3] = { 0.5f * (bx[0] + bx[1]), 0.5f * (bx[2] + bx[3]), 0.5f * (bx[4] + bx[5]) };
+          const float e[3] = { bx[1] - bx[0], bx[3] - bx[2], bx[5] - bx[4] };
+          const float r = 0.5f * sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) * 1.0001f + 1e-6f;
+          keep = may_cross(bd.m, c, R, s, r);
+        }
+        const unsigned long long mask = __ballot(keep);
+        const int pos = ncand + __popcll(mask & ((1ull << lane) - 1ull));
+        if (keep && pos < kMaxCandidates) cand[pos] = j;
+        ncand += __popcll(mask);
+      }
+      __syncthreads();
+    }
+    const bool all = ncand > kMaxCandidates;  // too many to list: test every brick
+    const int nloop = all ? f.nbricks : ncand;
     for (int sub = 0; sub < 4; ++sub) {
       const int64_t v = f.map_item_first[item] + sub * 64 + lane;
       if (v >= bd.nverts) continue;
@@ -92,7 +190,8 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
       // conservative FP32 cull in the slab form, exact test (FP64 slab terms) for survivors
       const float inv[3] = { 1.0f / dd[0], 1.0f / dd[1], 1.0f / dd[2] };
       int nh = 0;
-      for (int j = 0; j < f.nbricks; ++j) {
+      for (int q = 0; q < nloop; ++q) {
+        const int j = all ? q : cand[q];
         if (j == b) continue;
         const float* bx = f.bricks[j].box32;
         float t0 = 0.0f, t1 = 1.0f;

@@ -171,17 +171,17 @@ def test_learned_fallback_is_the_exact_path(oracle_mod):
 
 
 def test_learned_gain_over_coarse_only(oracle_mod):
-    """Trained on four seeded fields' sweep maps, evaluated on two held-out fields against the
-    exact light march: the network's correction lowers the mean opacity error of the coarse
-    optical depth."""
+    """Trained on eight seeded fields' reference-march maps, evaluated on three held-out fields
+    against the oracle's map: the network's correction lowers the mean opacity error of the
+    coarse optical depth (several-fold over the held-out set; never much worse on a field)."""
     O = oracle_mod
-    n, cells = 48, 32
+    n, cells = 96, 64
     xs, ts = [], []
-    for seed in (101, 102, 103, 104):
+    for seed in range(100, 108):
         sc = _scene(seed, n=n, cells=cells)
         with B.LightedVolumeRenderer() as r:
             scenes.configure(r, sc)
-            x, t = r.LearnedSamples(20000, seed=seed, mode=B.SHADOW_SWEEP)
+            x, t = r.LearnedSamples(60000, seed=seed, mode=B.SHADOW_REF_MARCH)
         xs.append(x)
         ts.append(t)
     x = np.concatenate(xs)
@@ -189,9 +189,9 @@ def test_learned_gain_over_coarse_only(oracle_mod):
     perm = np.random.default_rng(0).permutation(len(t))
     x, t = x[perm], t[perm]
     with B.ShadowMLP(seed=7) as net:
-        loss = net.train(x, t, epochs=40, batch=1024, lr=1e-3)
+        losses = [net.train(x, t, epochs=10, batch=2048, lr=lr) for lr in (1e-3, 1e-3, 3e-4)]
         out = []
-        for seed in (201, 202):
+        for seed in (200, 201, 202):
             sc = _scene(seed, n=n, cells=cells)
             res = {}
             for name, model in (("learned", net), ("coarse", None)):
@@ -204,11 +204,12 @@ def test_learned_gain_over_coarse_only(oracle_mod):
             exact = O.tmap_build(O.volume_from_scene(sc, "f32"))
             el = np.abs(res["learned"] - exact)
             ec = np.abs(res["coarse"] - exact)
-            print(f"seed {seed}: loss {loss:.4f}; mean |err| learned {el.mean():.4f} coarse {ec.mean():.4f}; "
-                  f"max learned {el.max():.4f} coarse {ec.max():.4f}")
+            print(f"seed {seed}: losses {losses}; mean |err| learned {el.mean():.4f} coarse {ec.mean():.4f}; "
+                  f"p99 learned {np.quantile(el, 0.99):.4f} coarse {np.quantile(ec, 0.99):.4f}")
             out.append((el.mean(), ec.mean()))
-    for learned, coarse in out:
-        assert learned < 0.8 * coarse
+    out = np.array(out)
+    assert out[:, 0].mean() < 0.5 * out[:, 1].mean()
+    assert np.all(out[:, 0] < 1.25 * out[:, 1])
 
 
 @pytest.mark.parametrize("budget", [1.0, 0.0])
