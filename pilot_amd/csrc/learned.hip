@@ -226,19 +226,20 @@ __global__ void __launch_bounds__(kBlock) k_coarse_tau(LearnedParams p)
 }
 
 // ---- the learned map: 64 vertices per wave, four waves per workgroup -----------------------
+// Static part of the learned-map kernels' LDS (51 KB); the density-patch table (ntf floats)
+// is dynamic, so that three workgroups fit a CU for the usual 256-entry TF.
 struct MapLds {
   MlpLds w;
   uint16_t buf[kBlock / 64][64 * kLd];
-  float y[kBlock / 64][64];
-  float a_tab[1024];
 };
 
 template <int FDT, int MDT>
 __global__ void __launch_bounds__(kBlock) k_learned_map(LearnedParams p, int64_t nv)
 {
   __shared__ MapLds s;
+  extern __shared__ float d_tab[];
   if (p.params) mlp_stage(p.params, s.w);
-  stage_patch(p.s, s.a_tab);
+  stage_patch(p.s, d_tab);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
@@ -247,8 +248,8 @@ __global__ void __launch_bounds__(kBlock) k_learned_map(LearnedParams p, int64_t
     const bool valid = id < nv;
     float v[3], f[kFeat];
     map_vertex(p.m, valid ? id : nv - 1, v);
-    const float tc = features<FDT>(p, s.a_tab, v, f);
-    const float y = p.params ? mlp_forward_wave(s.w, s.buf[wave], s.y[wave], f) : 0.0f;
+    const float tc = features<FDT>(p, d_tab, v, f);
+    const float y = p.params ? mlp_forward_fast(s.w, s.buf[wave], f) : 0.0f;
     const float tau = fmaxf(tc - y, 0.0f);
     if (valid) store_t<MDT>(p.m.map, id, 1.0f - expf(-tau));
   }
@@ -324,8 +325,9 @@ template <int FDT, int MDT>
 __global__ void __launch_bounds__(kBlock) k_bricks_learned_map(LearnedParams p, BrickFrame f)
 {
   __shared__ MapLds s;
+  extern __shared__ float d_tab[];
   if (p.params) mlp_stage(p.params, s.w);
-  stage_patch(p.s, s.a_tab);
+  stage_patch(p.s, d_tab);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int item = blockIdx.x; item < f.n_map_items; item += gridDim.x) {
@@ -335,8 +337,8 @@ __global__ void __launch_bounds__(kBlock) k_bricks_learned_map(LearnedParams p, 
     const bool valid = id < nv;
     float v[3], feat[kFeat];
     map_vertex(d.m, valid ? id : nv - 1, v);
-    const float tc = features<FDT>(p, s.a_tab, v, feat);
-    const float y = p.params ? mlp_forward_wave(s.w, s.buf[wave], s.y[wave], feat) : 0.0f;
+    const float tc = features<FDT>(p, d_tab, v, feat);
+    const float y = p.params ? mlp_forward_fast(s.w, s.buf[wave], feat) : 0.0f;
     if (valid) store_t<MDT>(d.map_final, id, 1.0f - expf(-fmaxf(tc - y, 0.0f)));
   }
 }
@@ -424,7 +426,7 @@ hipError_t launch_bricks_learned_map(const LearnedParams& p, const BrickFrame& f
   if (p.s.ntf > 1024) return hipErrorInvalidValue;
   const dim3 grid((unsigned)std::max(1, std::min(f.n_map_items, f.persistent_groups)));
   BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(map_dtype, {
-    hipLaunchKernelGGL((k_bricks_learned_map<FDT, MDT>), grid, dim3(kBlock), 0, st, p, f);
+    hipLaunchKernelGGL((k_bricks_learned_map<FDT, MDT>), grid, dim3(kBlock), sizeof(float) * p.s.ntf, st, p, f);
   }));
   return hipGetLastError();
 }
@@ -481,7 +483,8 @@ hipError_t launch_learned_map(const LearnedParams& p, hipStream_t st)
   const int64_t nv = p.m.mp[0] * p.m.mp[1] * p.m.mp[2];
   const int64_t groups = std::min<int64_t>((nv + kBlock - 1) / kBlock, 256 * 8);
   BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(p.m.dtype, {
-    hipLaunchKernelGGL((k_learned_map<FDT, MDT>), dim3((unsigned)groups), dim3(kBlock), 0, st, p, nv);
+    hipLaunchKernelGGL((k_learned_map<FDT, MDT>), dim3((unsigned)groups), dim3(kBlock), sizeof(float) * p.s.ntf,
+                       st, p, nv);
   }));
   return hipGetLastError();
 }

@@ -190,7 +190,6 @@ __global__ void k_zero_f32(float* p, int32_t n)
 struct InferLds {
   MlpLds w;
   uint16_t buf[4][64 * kLd];
-  float y[4][64];
 };
 
 __global__ void __launch_bounds__(256) k_mlp_infer(const float* params, const float* x, int64_t n,
@@ -205,7 +204,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float* params, const fl
     float feat[kFeat];
 #pragma unroll
     for (int k = 0; k < kFeat; ++k) feat[k] = row < n ? x[row * kFeat + k] : 0.0f;
-    const float out = mlp_forward_wave(s.w, s.buf[wave], s.y[wave], feat);
+    const float out = mlp_forward_fast(s.w, s.buf[wave], feat);
     if (row < n) y[row] = out;
   }
 }

@@ -164,4 +164,80 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpLds& w, uint16_t* buf
   return y;
 }
 
+// Inference forward pass, computed transposed (H^T = W^T X^T): the weights are the A
+// operands and the rows the B operands, so a lane's accumulator holds four consecutive hidden
+// units of one row.  Layer 1 leaves one 8-byte LDS store per tile, and the output needs only
+// a reduction over the four lane groups.  Same contract as mlp_forward_wave (no ybuf).
+__device__ __forceinline__ float mlp_forward_fast(const MlpLds& w, uint16_t* buf, const float* feat)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  {
+    u16x8 a[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) a[q][i] = f32_to_bf16(feat[8 * q + i]);
+      *(u16x8*)(buf + lane * kLd + 8 * q) = a[q];
+    }
+  }
+  wave_lds_sync();
+  bf16x8 xb[4];  // X^T fragments: row 16 vt + r, features 8g .. 8g+7
+#pragma unroll
+  for (int vt = 0; vt < 4; ++vt) xb[vt] = lds_frag(buf + (16 * vt + r) * kLd + 8 * g);
+  wave_lds_sync();
+  // layer 1: unit tile nt x row tile vt
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    const bf16x8 a = lds_frag(&w.w1t[16 * nt + r][8 * g]);
+    float bias[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bias[i] = w.b1[16 * nt + 4 * g + i];
+#pragma unroll
+    for (int vt = 0; vt < 4; ++vt) {
+      f32x4 acc = { 0.0f, 0.0f, 0.0f, 0.0f };
+      acc = mfma(a, xb[vt], acc);
+      typedef __attribute__((__vector_size__(4 * sizeof(uint16_t)))) uint16_t u16x4;
+      u16x4 h;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h[i] = f32_to_bf16(fmaxf(acc[i] + bias[i], 0.0f));
+      *(u16x4*)(buf + (16 * vt + r) * kLd + 16 * nt + 4 * g) = h;
+    }
+  }
+  wave_lds_sync();
+  bf16x8 hb[4][2];  // H1^T fragments: row 16 vt + r, units 32 ks + 8g .. +7
+#pragma unroll
+  for (int vt = 0; vt < 4; ++vt)
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) hb[vt][ks] = lds_frag(buf + (16 * vt + r) * kLd + 32 * ks + 8 * g);
+  wave_lds_sync();
+  float part[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    const bf16x8 a0 = lds_frag(&w.w2t[16 * nt + r][8 * g]);
+    const bf16x8 a1 = lds_frag(&w.w2t[16 * nt + r][32 + 8 * g]);
+    float bias[4], w3[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      bias[i] = w.b2[16 * nt + 4 * g + i];
+      w3[i] = w.w3[16 * nt + 4 * g + i];
+    }
+#pragma unroll
+    for (int vt = 0; vt < 4; ++vt) {
+      f32x4 acc = { 0.0f, 0.0f, 0.0f, 0.0f };
+      acc = mfma(a0, hb[vt][0], acc);
+      acc = mfma(a1, hb[vt][1], acc);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) part[vt] += fmaxf(acc[i] + bias[i], 0.0f) * w3[i];
+    }
+  }
+  // row 16 vt + r's output: sum over the four lane groups; lane 16 g + r keeps row tile g
+#pragma unroll
+  for (int vt = 0; vt < 4; ++vt) {
+    part[vt] += __shfl_xor(part[vt], 16, 64);
+    part[vt] += __shfl_xor(part[vt], 32, 64);
+  }
+  const float y = g == 0 ? part[0] : (g == 1 ? part[1] : (g == 2 ? part[2] : part[3]));
+  return y + w.b3;
+}
+
 }  // namespace beams
