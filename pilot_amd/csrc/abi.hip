@@ -130,7 +130,6 @@ struct beams_renderer {
   bool use_graph = false;
   hipGraphExec_t graph_exec = nullptr;
   std::vector<unsigned char> graph_key;
-  bool overflow_checked = true;
 
   int fail(int code, const std::string& msg)
   {
@@ -354,7 +353,7 @@ int prepare_map(beams_renderer* h, int32_t mode)
   HIP_TRY(h, h->map.ensure((size_t)nv * dtype_size(h->map_dtype)));
   if (mode == BEAMS_SHADOW_LEARNED) {
     const size_t nc = (size_t)(kCoarseCells + 1) * (kCoarseCells + 1) * (kCoarseCells + 1);
-    HIP_TRY(h, h->coarse.ensure(2 * nc * sizeof(float)));
+    HIP_TRY(h, h->coarse.ensure(2 * nc * sizeof(float) + 6 * (kCoarseCells + 1) * sizeof(int32_t)));
     HIP_TRY(h, h->lerr.ensure(2 * sizeof(unsigned int)));
   }
   if (mode == BEAMS_SHADOW_SWEEP) {
@@ -399,6 +398,7 @@ LearnedParams learned_params(beams_renderer* h, const GridParams& g, const Shade
   p.cn = kCoarseCells + 1;
   p.coarse_sigma = (float*)h->coarse.p;
   p.coarse_tau = p.coarse_sigma + (size_t)p.cn * p.cn * p.cn;
+  p.coarse_ranges = (int32_t*)(p.coarse_tau + (size_t)p.cn * p.cn * p.cn);
   for (int d = 0; d < 3; ++d) {
     p.corg[d] = (float)h->bounds[2 * d];
     p.csp[d] = (float)(h->bounds[2 * d + 1] - h->bounds[2 * d]) / (float)kCoarseCells;
@@ -634,7 +634,6 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
   f.n_px_items = (int32_t)px_brick.size();
   f.depth_order = (const int32_t*)(base + o_ord);
   f.queue = (unsigned int*)h->bqueue.p;
-  f.overflow = (unsigned int*)h->bqueue.p + 4;
   f.seeds = (float*)h->bseeds.p;
   f.partial = (float4*)h->bpartial.p;
   f.partial_depth = (float*)((float4*)h->bpartial.p + partial_pixels);
@@ -690,18 +689,6 @@ int run_frame(beams_renderer* h, const std::vector<unsigned char>& key, Issue&& 
   return phase_mark(h, BEAMS_PHASE_FRAME, true);
 }
 
-int check_overflow(beams_renderer* h)
-{
-  if (h->bricks.empty() || h->overflow_checked) return BEAMS_OK;
-  unsigned int flag = 0;
-  HIP_TRY(h, hipStreamSynchronize(h->stream));
-  HIP_TRY(h, hipMemcpy(&flag, (unsigned int*)h->bqueue.p + 4, sizeof(flag), hipMemcpyDeviceToHost));
-  h->overflow_checked = true;
-  if (flag)
-    return h->fail(BEAMS_ERR_UNSUPPORTED, "a light path crossed more bricks than the exchange keeps (64)");
-  return BEAMS_OK;
-}
-
 // RenderOnDevice phases 1-4 for every brick + GlobalComposite + BlendBackground.
 // Learned bricks: every brick's final map from the whole volume's coarse optical depth and
 // the network, a per-brick sparse check against the whole volume's light march, and that
@@ -711,7 +698,7 @@ int build_bricks_learned(beams_renderer* h, const BrickFrame& f)
 {
   const int nb = f.nbricks;
   const size_t nc = (size_t)(kCoarseCells + 1) * (kCoarseCells + 1) * (kCoarseCells + 1);
-  HIP_TRY(h, h->coarse.ensure(2 * nc * sizeof(float)));
+  HIP_TRY(h, h->coarse.ensure(2 * nc * sizeof(float) + 6 * (kCoarseCells + 1) * sizeof(int32_t)));
   HIP_TRY(h, h->lerr.ensure(sizeof(float) * (size_t)nb));
   HIP_TRY(h, h->bredo.ensure(sizeof(int32_t) * (size_t)nb));
   const GridParams g = grid_params(h);
@@ -757,8 +744,7 @@ int build_bricks_learned(beams_renderer* h, const BrickFrame& f)
 int render_bricks(beams_renderer* h, const CameraParams& c, const float* background, float* rgba,
                   float* depth, int32_t mem)
 {
-  int rc = check_overflow(h);
-  if (rc) return rc;
+  int rc;
   if (h->use_shadow && h->shadow_mode == BEAMS_SHADOW_SWEEP)
     return h->fail(BEAMS_ERR_UNSUPPORTED,
                    "bricks build their maps with the reference march or the learned closure");
@@ -816,14 +802,11 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
   key.put(1).put(f).put(c).put(o).put(h->btab_version).put(shadows).put(learned).put(h->dtype)
      .put(h->map_dtype).put(ntf);
   if ((rc = run_frame(h, key.bytes, issue))) return rc;
-  if (shadows) {
-    h->map_valid = true;
-    h->overflow_checked = false;
-  }
+  if (shadows) h->map_valid = true;
   if (mem == BEAMS_MEM_HOST) {
     HIP_TRY(h, hipMemcpyAsync(rgba, o.rgba, sizeof(float) * 4 * npix, hipMemcpyDeviceToHost, h->stream));
     if (depth) HIP_TRY(h, hipMemcpyAsync(depth, o.depth, sizeof(float) * npix, hipMemcpyDeviceToHost, h->stream));
-    if ((rc = check_overflow(h))) return rc;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
   }
   return BEAMS_OK;
 }
@@ -887,7 +870,7 @@ int beams_synchronize(beams_renderer* h)
 {
   CHECK_HANDLE(h);
   HIP_TRY(h, hipStreamSynchronize(h->stream));
-  return check_overflow(h);
+  return BEAMS_OK;
 }
 
 int beams_set_field(beams_renderer* h, const void* values, int32_t dtype, int32_t assoc,

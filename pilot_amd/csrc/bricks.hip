@@ -9,7 +9,7 @@
 //   phase 4  k_bricks_march (footprint tiles)        persistent, all bricks
 //   phase 5  k_bricks_composite (depth-ordered over) per pixel
 // The reference routes 32-byte hit records through rank 0 with MPI; here the hits of a
-// vertex live in LDS for the duration of its loop and are never written to memory.
+// vertex live in registers for the duration of its loop and are never written to memory.
 #include "bricks.hpp"
 #include "camera.hpp"
 #include "march_dev.hpp"
@@ -140,10 +140,41 @@ __device__ __forceinline__ bool may_cross(const MapParams& m, const float* c, fl
 // of the item (conservative; the per-vertex FP32 slab cull and the exact FP64 test follow).
 constexpr int kMaxCandidates = 1024;
 
+// One light segment (o -> p) against brick j: the conservative FP32 slab cull on the brick's
+// inflated box, then the exact test (FP64 slab terms); key = (exit distance bits, brick).
+__device__ __forceinline__ bool exchange_hit(const BrickFrame& f, int j, int b, const float* o,
+                                             const float* p, const float* dd, const float* inv,
+                                             unsigned long long& key)
+{
+  if (j == b) return false;
+  const float* bx = f.bricks[j].box32;
+  float t0 = 0.0f, t1 = 1.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float ta = (bx[2 * a] - o[a]) * inv[a], tb = (bx[2 * a + 1] - o[a]) * inv[a];
+    if (dd[a] == 0.0f) {
+      const bool in = o[a] >= bx[2 * a] && o[a] <= bx[2 * a + 1];
+      ta = in ? -__builtin_inff() : __builtin_inff();
+      tb = in ? __builtin_inff() : -__builtin_inff();
+    }
+    t0 = fmaxf(t0, fminf(ta, tb));
+    t1 = fminf(t1, fmaxf(ta, tb));
+  }
+  if (!(t0 <= t1 + 1e-3f)) return false;
+  float tmin, tmax;
+  if (!segment_aabb(o, p, f.bricks[j].g.bounds, (float)b, tmin, tmax)) return false;
+  // tmax >= tmin >= 0; adding +0 turns a -0 into +0 so the bit order is the float order
+  key = ((unsigned long long)f32_bits(tmax + 0.0f) << 32) | (unsigned long long)(unsigned)j;
+  return true;
+}
+
+// Hits kept sorted in registers per vertex; a path crossing more bricks is composited by
+// repeated selection of the next key over the candidates (same order, no storage).
+constexpr int kRegHits = 16;
+
 template <int MDT>
 __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
 {
-  extern __shared__ unsigned long long hits[];  // [kMaxPathHits][64], (t bits << 32) | brick
   __shared__ int slot;
   __shared__ int cand[kMaxCandidates];
   const int lane = threadIdx.x;
@@ -187,50 +218,62 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
       const float dd[3] = { p[0] - o[0], p[1] - o[1], p[2] - o[2] };
       const float rs = 1.0f / __builtin_sqrtf(dot3(dd, dd));  // vtkm::Normalize
       const float dir[3] = { dd[0] * rs, dd[1] * rs, dd[2] * rs };
-      // conservative FP32 cull in the slab form, exact test (FP64 slab terms) for survivors
       const float inv[3] = { 1.0f / dd[0], 1.0f / dd[1], 1.0f / dd[2] };
-      int nh = 0;
-      for (int q = 0; q < nloop; ++q) {
-        const int j = all ? q : cand[q];
-        if (j == b) continue;
-        const float* bx = f.bricks[j].box32;
-        float t0 = 0.0f, t1 = 1.0f;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          float ta = (bx[2 * a] - o[a]) * inv[a], tb = (bx[2 * a + 1] - o[a]) * inv[a];
-          if (dd[a] == 0.0f) {
-            const bool in = o[a] >= bx[2 * a] && o[a] <= bx[2 * a + 1];
-            ta = in ? -__builtin_inff() : __builtin_inff();
-            tb = in ? __builtin_inff() : -__builtin_inff();
-          }
-          t0 = fmaxf(t0, fminf(ta, tb));
-          t1 = fminf(t1, fmaxf(ta, tb));
-        }
-        if (!(t0 <= t1 + 1e-3f)) continue;
-        float tmin, tmax;
-        if (!segment_aabb(o, p, f.bricks[j].g.bounds, (float)b, tmin, tmax)) continue;
-        if (nh >= kMaxPathHits) {
-          atomicOr(f.overflow, 1u);
-          break;
-        }
-        // tmax >= tmin >= 0; adding +0 turns a -0 into +0 so the bit order is the float order
-        const unsigned long long key =
-          ((unsigned long long)f32_bits(tmax + 0.0f) << 32) | (unsigned long long)(unsigned)j;
-        int k = nh++;
-        while (k > 0 && hits[(k - 1) * 64 + lane] > key) {  // insertion by (RayT, brick)
-          hits[k * 64 + lane] = hits[(k - 1) * 64 + lane];
-          --k;
-        }
-        hits[k * 64 + lane] = key;
-      }
-      float op = 0.0f;
-      for (int k = 0; k < nh; ++k) {
-        const unsigned long long key = hits[k * 64 + lane];
+      auto composite = [&](float op, unsigned long long key) {
         const int j = (int)(key & 0xffffffffull);
         const float t = bits_f32((uint32_t)(key >> 32));
         const float q[3] = { o[0] + dir[0] * t, o[1] + dir[1] * t, o[2] + dir[2] * t };
         const float local = map_opacity<MDT>(f.bricks[j].m, q);
-        op = op + (1.0f - op) * local;
+        return op + (1.0f - op) * local;
+      };
+      unsigned long long keys[kRegHits];
+#pragma unroll
+      for (int m = 0; m < kRegHits; ++m) keys[m] = ~0ull;
+      int nh = 0;
+      bool spill = false;
+      for (int q = 0; q < nloop; ++q) {
+        unsigned long long key;
+        if (!exchange_hit(f, all ? q : cand[q], b, o, p, dd, inv, key)) continue;
+        if (nh == kRegHits) {
+          spill = true;
+          break;
+        }
+        // insertion by (RayT, brick): the larger keys move up one slot
+        unsigned long long carry = key;
+#pragma unroll
+        for (int m = 0; m < kRegHits; ++m) {
+          const bool swap = m < nh && keys[m] > carry;
+          const unsigned long long t = keys[m];
+          keys[m] = swap ? carry : t;
+          carry = swap ? t : carry;
+        }
+#pragma unroll
+        for (int m = 0; m < kRegHits; ++m) keys[m] = m == nh ? carry : keys[m];
+        ++nh;
+      }
+      float op = 0.0f;
+      if (!spill) {
+        for (int k = 0; k < nh; ++k) {
+          unsigned long long key = keys[0];
+#pragma unroll
+          for (int m = 1; m < kRegHits; ++m) key = k == m ? keys[m] : key;
+          op = composite(op, key);
+        }
+      } else {
+        bool started = false;
+        unsigned long long last = 0;
+        for (;;) {
+          unsigned long long best = ~0ull;
+          for (int q = 0; q < nloop; ++q) {
+            unsigned long long key;
+            if (!exchange_hit(f, all ? q : cand[q], b, o, p, dd, inv, key)) continue;
+            if ((!started || key > last) && key < best) best = key;
+          }
+          if (best == ~0ull) break;
+          op = composite(op, best);
+          last = best;
+          started = true;
+        }
       }
       f.seeds[bd.seed_off + v] = op;
     }
@@ -365,10 +408,9 @@ hipError_t launch_bricks_exchange(const BrickFrame& f, int32_t map_dtype, int64_
 {
   (void)total_verts;
   if (f.n_map_items == 0) return hipSuccess;
-  const size_t lds = sizeof(unsigned long long) * kMaxPathHits * 64;
   const dim3 grid((unsigned)(f.persistent_groups * 4));
   BRICK_DISPATCH_MAP(map_dtype, {
-    hipLaunchKernelGGL((k_bricks_exchange<MDT>), grid, dim3(64), lds, st, f);
+    hipLaunchKernelGGL((k_bricks_exchange<MDT>), grid, dim3(64), 0, st, f);
   });
   return hipGetLastError();
 }

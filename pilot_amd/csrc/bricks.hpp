@@ -17,9 +17,6 @@ struct BrickDesc {
   float box32[6];      // bounds inflated for the conservative FP32 cull
 };
 
-// Upper bound on light-path hits per map vertex kept in LDS by the exchange kernel.
-constexpr int kMaxPathHits = 64;
-
 struct BrickFrame {
   const BrickDesc* bricks;
   int32_t nbricks;
@@ -31,7 +28,6 @@ struct BrickFrame {
   int32_t n_px_items;
   const int32_t* depth_order;     // bricks by ascending camera-to-centre distance
   unsigned int* queue;            // work-queue counters: [0] map-1, [1] exchange, [2] map-3, [3] march
-  unsigned int* overflow;         // set when a light path crosses more than kMaxPathHits bricks
   float* seeds;                   // non-local opacity per map vertex (phase 2 -> 3)
   float4* partial;                // per-brick footprint RGBA
   float* partial_depth;           // per-brick footprint entry distance

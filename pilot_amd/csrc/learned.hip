@@ -37,13 +37,13 @@ __device__ __forceinline__ int64_t nearest_index(const float* x, int64_t n, floa
   return (v - x[i] <= x[i + 1] - v) ? i : i + 1;
 }
 
-__device__ __forceinline__ int64_t tf_index(const ShadeParams& s, float val)
+// The TF entry of a scalar as the reference indexes it (truncate, then clamp to the table);
+// clamping in float first gives the same index for every finite value and keeps it 32-bit.
+__device__ __forceinline__ int tf_index(const ShadeParams& s, float val)
 {
-  const float sc = (val - s.min_scalar) * s.inv_delta;
-  const int64_t tfmax = s.ntf - 1;
-  int64_t idx = (int64_t)(sc * (float)tfmax);
-  idx = idx < tfmax ? idx : tfmax;
-  return idx > 0 ? idx : 0;
+  const float tfmax = (float)(s.ntf - 1);
+  const float t = ((val - s.min_scalar) * s.inv_delta) * tfmax;
+  return (int)fminf(fmaxf(t, 0.0f), tfmax);
 }
 
 __device__ __forceinline__ void stage_alpha(const ShadeParams& s, float* a_tab)
@@ -61,18 +61,18 @@ __device__ __forceinline__ void stage_patch(const ShadeParams& s, float* d_tab)
 // Trilinear interpolation in a cn^3 vertex grid (x fastest), p clamped to the grid.
 __device__ __forceinline__ float coarse_lerp(const LearnedParams& p, const float* grid, const float* v)
 {
-  int64_t c[3];
+  int c[3];
   float t[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     float u = (v[d] - p.corg[d]) * p.cinv[d];
     u = fminf(fmaxf(u, 0.0f), (float)(p.cn - 1));
-    int64_t i = (int64_t)u;
+    int i = (int)u;
     i = i > p.cn - 2 ? p.cn - 2 : i;
     c[d] = i;
     t[d] = u - (float)i;
   }
-  const int64_t n = p.cn, b = c[0] + n * (c[1] + n * c[2]);
+  const int n = p.cn, b = c[0] + n * (c[1] + n * c[2]);
   const float v00 = grid[b] + t[0] * (grid[b + 1] - grid[b]);
   const float v10 = grid[b + n] + t[0] * (grid[b + n + 1] - grid[b + n]);
   const float v01 = grid[b + n * n] + t[0] * (grid[b + n * n + 1] - grid[b + n * n]);
@@ -107,27 +107,21 @@ __device__ __forceinline__ float features(const LearnedParams& p, const float* d
   const GridParams& g = p.g;
   const float tc = fminf(coarse_lerp(p, p.coarse_tau, v), kTauMax);
   f[0] = tc * 0.25f;
-  int64_t c[3];
+  // element offsets of the clamped neighbour indices per axis
+  int64_t o[3][3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) c[d] = nearest_index(g.x[d], g.n[d], v[d], g.gx0[d], g.ginv[d]);
-  int q = 1;
-#pragma unroll
-  for (int dz = -1; dz <= 1; ++dz) {
-    int64_t k = c[2] + dz;
-    k = k < 0 ? 0 : (k > g.n[2] - 1 ? g.n[2] - 1 : k);
-#pragma unroll
-    for (int dy = -1; dy <= 1; ++dy) {
-      int64_t j = c[1] + dy;
-      j = j < 0 ? 0 : (j > g.n[1] - 1 ? g.n[1] - 1 : j);
-#pragma unroll
-      for (int dx = -1; dx <= 1; ++dx) {
-        int64_t i = c[0] + dx;
-        i = i < 0 ? 0 : (i > g.n[0] - 1 ? g.n[0] - 1 : i);
-        const float val = load_t<FDT>(g.field, i * g.field_stride[0] + j * g.field_stride[1] + k * g.field_stride[2]);
-        f[q++] = d_tab[tf_index(p.s, val)];
-      }
-    }
+  for (int d = 0; d < 3; ++d) {
+    const int64_t c = nearest_index(g.x[d], g.n[d], v[d], g.gx0[d], g.ginv[d]);
+    const int64_t lo = c > 0 ? c - 1 : 0, hi = c < g.n[d] - 1 ? c + 1 : g.n[d] - 1;
+    o[d][0] = lo * g.field_stride[d];
+    o[d][1] = c * g.field_stride[d];
+    o[d][2] = hi * g.field_stride[d];
   }
+  float val[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) val[q] = load_t<FDT>(g.field, o[2][q / 9] + o[1][(q / 3) % 3] + o[0][q % 3]);
+#pragma unroll
+  for (int q = 0; q < 27; ++q) f[1 + q] = d_tab[tf_index(p.s, val[q])];
   light_dir(p.m, v, f + 28);
   f[31] = 0.0f;
   return tc;
@@ -139,6 +133,28 @@ __device__ __forceinline__ float sigma_of(float a, float inv_h)
 }
 
 // ---- coarse mip: mean extinction over the field vertices nearest each coarse vertex -------
+// Per axis and coarse index, the field indices whose coordinate lies within half a coarse
+// spacing (one thread each).
+__global__ void k_coarse_ranges(LearnedParams p)
+{
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= 3 * p.cn) return;
+  const int d = t / p.cn, ci = t % p.cn;
+  const GridParams& g = p.g;
+  const float c = p.corg[d] + p.csp[d] * (float)ci;
+  const float half = 0.5f * p.csp[d];
+  int64_t lo = lower_index(g.x[d], g.n[d], c - half);
+  const int64_t hi = ci == p.cn - 1 ? g.n[d] : lower_index(g.x[d], g.n[d], c + half);
+  int64_t cnt = hi - lo;
+  if (cnt <= 0) {  // coarser field than mip: the nearest field vertex alone
+    lo = nearest_index(g.x[d], g.n[d], c, g.gx0[d], g.ginv[d]);
+    cnt = 1;
+  }
+  p.coarse_ranges[2 * t] = (int32_t)lo;
+  p.coarse_ranges[2 * t + 1] = (int32_t)cnt;
+}
+
+// One workgroup per coarse vertex; each wave takes rows of its box, lanes along x.
 template <int FDT>
 __global__ void __launch_bounds__(kBlock) k_coarse_sigma(LearnedParams p)
 {
@@ -148,29 +164,24 @@ __global__ void __launch_bounds__(kBlock) k_coarse_sigma(LearnedParams p)
     sig_tab[i] = sigma_of(p.s.tf[4 * i + 3] / p.s.max_density, p.inv_h);
   __syncthreads();
   const GridParams& g = p.g;
-  const int64_t cn = p.cn, id = blockIdx.x;
-  const int64_t ci[3] = { id % cn, (id / cn) % cn, id / (cn * cn) };
-  int64_t lo[3], cnt[3];
+  const int cn = p.cn, id = (int)blockIdx.x;
+  const int ci[3] = { id % cn, (id / cn) % cn, id / (cn * cn) };
+  int lo[3], cnt[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const float c = p.corg[d] + p.csp[d] * (float)ci[d];
-    const float half = 0.5f * p.csp[d];
-    lo[d] = lower_index(g.x[d], g.n[d], c - half);
-    const int64_t hi = ci[d] == cn - 1 ? g.n[d] : lower_index(g.x[d], g.n[d], c + half);
-    cnt[d] = hi - lo[d];
-    if (cnt[d] <= 0) {  // coarser field than mip: the nearest field vertex alone
-      lo[d] = nearest_index(g.x[d], g.n[d], c, g.gx0[d], g.ginv[d]);
-      cnt[d] = 1;
-    }
+    lo[d] = p.coarse_ranges[2 * (d * cn + ci[d])];
+    cnt[d] = p.coarse_ranges[2 * (d * cn + ci[d]) + 1];
   }
-  const int64_t total = cnt[0] * cnt[1] * cnt[2];
+  const int total = cnt[0] * cnt[1] * cnt[2];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float acc = 0.0f;
-  for (int64_t q = threadIdx.x; q < total; q += blockDim.x) {
-    const int64_t i = lo[0] + q % cnt[0];
-    const int64_t j = lo[1] + (q / cnt[0]) % cnt[1];
-    const int64_t k = lo[2] + q / (cnt[0] * cnt[1]);
-    const float val = load_t<FDT>(g.field, i * g.field_stride[0] + j * g.field_stride[1] + k * g.field_stride[2]);
-    acc += sig_tab[tf_index(p.s, val)];
+  for (int row = wave; row < cnt[1] * cnt[2]; row += kBlock / 64) {
+    const int64_t j = lo[1] + row % cnt[1], k = lo[2] + row / cnt[1];
+    const int64_t base = j * g.field_stride[1] + k * g.field_stride[2];
+    for (int i = lane; i < cnt[0]; i += 64) {
+      const float val = load_t<FDT>(g.field, (int64_t)(lo[0] + i) * g.field_stride[0] + base);
+      acc += sig_tab[tf_index(p.s, val)];
+    }
   }
 #pragma unroll
   for (int mk = 32; mk >= 1; mk >>= 1) acc += __shfl_xor(acc, mk, 64);
@@ -234,7 +245,11 @@ struct MapLds {
 };
 
 template <int FDT, int MDT>
-__global__ void __launch_bounds__(kBlock) k_learned_map(LearnedParams p, int64_t nv)
+// Work unit: a chunk of 64 consecutive vertices of one map row (x), one per wave; the
+// vertex position is map_vertex's, computed from the row and the column without divisions
+// of the 64-bit vertex index.
+__global__ void __launch_bounds__(kBlock) k_learned_map(LearnedParams p, int32_t chunks_per_row,
+                                                        int32_t nchunks)
 {
   __shared__ MapLds s;
   extern __shared__ float d_tab[];
@@ -242,16 +257,20 @@ __global__ void __launch_bounds__(kBlock) k_learned_map(LearnedParams p, int64_t
   stage_patch(p.s, d_tab);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t base = (int64_t)blockIdx.x * kBlock + wave * 64; base < nv; base += stride) {
-    const int64_t id = base + lane;
-    const bool valid = id < nv;
+  const int mp0 = (int)p.m.mp[0], mp1 = (int)p.m.mp[1];
+  for (int chunk = blockIdx.x * (kBlock / 64) + wave; chunk < nchunks; chunk += gridDim.x * (kBlock / 64)) {
+    const int row = chunk / chunks_per_row;
+    const int i0 = (chunk - row * chunks_per_row) * 64;
+    const int j = row % mp1, k = row / mp1;
+    const int i = i0 + lane < mp0 ? i0 + lane : mp0 - 1;
     float v[3], f[kFeat];
-    map_vertex(p.m, valid ? id : nv - 1, v);
+    v[0] = p.m.morg[0] + p.m.msp[0] * (float)i;
+    v[1] = p.m.morg[1] + p.m.msp[1] * (float)j;
+    v[2] = p.m.morg[2] + p.m.msp[2] * (float)k;
     const float tc = features<FDT>(p, d_tab, v, f);
     const float y = p.params ? mlp_forward_fast(s.w, s.buf[wave], f) : 0.0f;
     const float tau = fmaxf(tc - y, 0.0f);
-    if (valid) store_t<MDT>(p.m.map, id, 1.0f - expf(-tau));
+    if (i0 + lane < mp0) store_t<MDT>(p.m.map, (int64_t)row * mp0 + i, 1.0f - expf(-tau));
   }
 }
 
@@ -467,6 +486,7 @@ hipError_t launch_coarse(const LearnedParams& p, hipStream_t st)
 {
   if (p.s.ntf > 1024) return hipErrorInvalidValue;
   const int64_t nc = (int64_t)p.cn * p.cn * p.cn;
+  hipLaunchKernelGGL(k_coarse_ranges, dim3((unsigned)((3 * p.cn + 63) / 64)), dim3(64), 0, st, p);
   BEAMS_DISPATCH_FIELD(p.g.dtype, {
     hipLaunchKernelGGL((k_coarse_sigma<FDT>), dim3((unsigned)nc), dim3(kBlock), 0, st, p);
   });
@@ -480,11 +500,14 @@ hipError_t launch_coarse(const LearnedParams& p, hipStream_t st)
 hipError_t launch_learned_map(const LearnedParams& p, hipStream_t st)
 {
   if (p.s.ntf > 1024) return hipErrorInvalidValue;
-  const int64_t nv = p.m.mp[0] * p.m.mp[1] * p.m.mp[2];
-  const int64_t groups = std::min<int64_t>((nv + kBlock - 1) / kBlock, 256 * 8);
+  // the map's vertices come from MapParams (mp <= 4097 per axis), so rows and chunks fit 32 bits
+  const int64_t cpr = (p.m.mp[0] + 63) / 64;
+  const int64_t nchunks = cpr * p.m.mp[1] * p.m.mp[2];
+  if (nchunks > INT32_MAX) return hipErrorInvalidValue;
+  const int64_t groups = std::min<int64_t>((nchunks + 3) / 4, 256 * 12);
   BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(p.m.dtype, {
     hipLaunchKernelGGL((k_learned_map<FDT, MDT>), dim3((unsigned)groups), dim3(kBlock), sizeof(float) * p.s.ntf,
-                       st, p, nv);
+                       st, p, (int32_t)cpr, (int32_t)nchunks);
   }));
   return hipGetLastError();
 }

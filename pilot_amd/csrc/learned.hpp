@@ -34,6 +34,7 @@ struct LearnedParams {
   ShadeParams s;
   float* coarse_sigma;  // cn^3 mean extinction around each coarse vertex, x fastest
   float* coarse_tau;    // cn^3 optical depth from each coarse vertex to the light
+  int32_t* coarse_ranges;  // [3][cn][2]: first field index and count nearest each coarse index
   int32_t cn;           // coarse vertices per axis
   float corg[3], csp[3], cinv[3];
   float inv_h;          // 1 / map step

@@ -75,6 +75,10 @@ __device__ __forceinline__ bf16x8 pack_frag(const float* v)
   return __builtin_bit_cast(bf16x8, u);
 }
 
+// Round to BF16 (nearest even) with the hardware conversion (v_cvt_pk_bf16_f32); equal to
+// f32_to_bf16 for every finite value.
+__device__ __forceinline__ uint16_t bf16_hw(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
 {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -176,7 +180,7 @@ __device__ __forceinline__ float mlp_forward_fast(const MlpLds& w, uint16_t* buf
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) a[q][i] = f32_to_bf16(feat[8 * q + i]);
+      for (int i = 0; i < 8; ++i) a[q][i] = bf16_hw(feat[8 * q + i]);
       *(u16x8*)(buf + lane * kLd + 8 * q) = a[q];
     }
   }
@@ -199,7 +203,7 @@ __device__ __forceinline__ float mlp_forward_fast(const MlpLds& w, uint16_t* buf
       typedef __attribute__((__vector_size__(4 * sizeof(uint16_t)))) uint16_t u16x4;
       u16x4 h;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) h[i] = f32_to_bf16(fmaxf(acc[i] + bias[i], 0.0f));
+      for (int i = 0; i < 4; ++i) h[i] = bf16_hw(fmaxf(acc[i] + bias[i], 0.0f));
       *(u16x4*)(buf + (16 * vt + r) * kLd + 16 * nt + 4 * g) = h;
     }
   }
