@@ -408,6 +408,8 @@ LearnedParams learned_params(beams_renderer* h, const GridParams& g, const Shade
   p.params = h->net ? mlp_device_params(h->net) : nullptr;
   p.err = (unsigned int*)h->lerr.p;
   p.check_stride = h->check_stride;
+  const double bytes = (double)dtype_size(g.dtype) * (double)g.n[0] * (double)g.n[1] * (double)g.n[2];
+  p.field_small = bytes < 4294967296.0 ? 1 : 0;
   return p;
 }
 

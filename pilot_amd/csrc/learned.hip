@@ -118,8 +118,29 @@ __device__ __forceinline__ float features(const LearnedParams& p, const float* d
     o[d][2] = hi * g.field_stride[d];
   }
   float val[27];
+  if (p.field_small) {
+    // byte offsets below 4 GiB: one 32-bit add per gather from the uniform field base
+    constexpr uint32_t es = FDT == kF32 ? 4u : 2u;
+    uint32_t b[3][3];
 #pragma unroll
-  for (int q = 0; q < 27; ++q) val[q] = load_t<FDT>(g.field, o[2][q / 9] + o[1][(q / 3) % 3] + o[0][q % 3]);
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int t = 0; t < 3; ++t) b[d][t] = (uint32_t)o[d][t] * es;
+    const char* base = (const char*)g.field;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      const uint32_t off = b[2][q / 9] + b[1][(q / 3) % 3] + b[0][q % 3];
+      if constexpr (FDT == kF32)
+        val[q] = *(const float*)(base + off);
+      else if constexpr (FDT == kF16)
+        val[q] = f16_to_f32(*(const uint16_t*)(base + off));
+      else
+        val[q] = bf16_to_f32(*(const uint16_t*)(base + off));
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 27; ++q) val[q] = load_t<FDT>(g.field, o[2][q / 9] + o[1][(q / 3) % 3] + o[0][q % 3]);
+  }
 #pragma unroll
   for (int q = 0; q < 27; ++q) f[1 + q] = d_tab[tf_index(p.s, val[q])];
   light_dir(p.m, v, f + 28);

@@ -41,6 +41,7 @@ struct LearnedParams {
   const float* params;  // network parameters (FP32 packed); null: coarse optical depth only
   unsigned int* err;    // [0] max |learned - exact|, [1] max |coarse only - exact| (float bits)
   int32_t check_stride; // check vertices: every check_stride-th map index per axis + the last
+  int32_t field_small;  // the field's storage spans less than 4 GiB (32-bit byte offsets)
 };
 
 hipError_t launch_coarse(const LearnedParams& p, hipStream_t st);

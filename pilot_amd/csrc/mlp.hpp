@@ -34,7 +34,9 @@ typedef __attribute__((__vector_size__(8 * sizeof(uint16_t)))) uint16_t u16x8;
 struct MlpLds {
   uint16_t w1t[kHid][kFeat + 8];
   uint16_t w2t[kHid][kLd];  // w2t[n][k] = W2[k][n]
-  float b1[kHid], b2[kHid], w3[kHid];
+  alignas(16) float b1[kHid];
+  alignas(16) float b2[kHid];
+  alignas(16) float w3[kHid];
   float b3;
 };
 
@@ -78,6 +80,13 @@ __device__ __forceinline__ bf16x8 pack_frag(const float* v)
 // Round to BF16 (nearest even) with the hardware conversion (v_cvt_pk_bf16_f32); equal to
 // f32_to_bf16 for every finite value.
 __device__ __forceinline__ uint16_t bf16_hw(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+
+// ReLU as an integer max on the bits: negative floats (sign bit set) and -0 become +0, every
+// other value passes unchanged; one instruction, no NaN canonicalisation.
+__device__ __forceinline__ float relu(float x)
+{
+  return __builtin_bit_cast(float, max(__builtin_bit_cast(int32_t, x), 0));
+}
 
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
 {
@@ -174,6 +183,7 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpLds& w, uint16_t* buf
 // a reduction over the four lane groups.  Same contract as mlp_forward_wave (no ybuf).
 __device__ __forceinline__ float mlp_forward_fast(const MlpLds& w, uint16_t* buf, const float* feat)
 {
+#pragma clang fp contract(fast)
   const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
   {
     u16x8 a[4];
@@ -193,17 +203,15 @@ __device__ __forceinline__ float mlp_forward_fast(const MlpLds& w, uint16_t* buf
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
     const bf16x8 a = lds_frag(&w.w1t[16 * nt + r][8 * g]);
-    float bias[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) bias[i] = w.b1[16 * nt + 4 * g + i];
+    // the bias enters as the accumulator input (rounded once, after the products)
+    const f32x4 bias = *(const f32x4*)&w.b1[16 * nt + 4 * g];
 #pragma unroll
     for (int vt = 0; vt < 4; ++vt) {
-      f32x4 acc = { 0.0f, 0.0f, 0.0f, 0.0f };
-      acc = mfma(a, xb[vt], acc);
+      const f32x4 acc = mfma(a, xb[vt], bias);
       typedef __attribute__((__vector_size__(4 * sizeof(uint16_t)))) uint16_t u16x4;
       u16x4 h;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) h[i] = bf16_hw(fmaxf(acc[i] + bias[i], 0.0f));
+      for (int i = 0; i < 4; ++i) h[i] = bf16_hw(relu(acc[i]));
       *(u16x4*)(buf + (16 * vt + r) * kLd + 16 * nt + 4 * g) = h;
     }
   }
@@ -219,19 +227,14 @@ __device__ __forceinline__ float mlp_forward_fast(const MlpLds& w, uint16_t* buf
   for (int nt = 0; nt < 4; ++nt) {
     const bf16x8 a0 = lds_frag(&w.w2t[16 * nt + r][8 * g]);
     const bf16x8 a1 = lds_frag(&w.w2t[16 * nt + r][32 + 8 * g]);
-    float bias[4], w3[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      bias[i] = w.b2[16 * nt + 4 * g + i];
-      w3[i] = w.w3[16 * nt + 4 * g + i];
-    }
+    const f32x4 bias = *(const f32x4*)&w.b2[16 * nt + 4 * g];
+    const f32x4 w3 = *(const f32x4*)&w.w3[16 * nt + 4 * g];
 #pragma unroll
     for (int vt = 0; vt < 4; ++vt) {
-      f32x4 acc = { 0.0f, 0.0f, 0.0f, 0.0f };
-      acc = mfma(a0, hb[vt][0], acc);
+      f32x4 acc = mfma(a0, hb[vt][0], bias);
       acc = mfma(a1, hb[vt][1], acc);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) part[vt] += fmaxf(acc[i] + bias[i], 0.0f) * w3[i];
+      for (int i = 0; i < 4; ++i) part[vt] += relu(acc[i]) * w3[i];
     }
   }
   // row 16 vt + r's output: sum over the four lane groups; lane 16 g + r keeps row tile g

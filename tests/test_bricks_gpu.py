@@ -77,3 +77,19 @@ def test_single_brick_equals_one_block(oracle_mod):
         scenes.configure(r, sc)
         ref, _ = r.RenderImage(sc.camera, sc.width, sc.height)
     assert np.array_equal(img, ref.reshape(-1, 4))
+
+
+def test_bricks_long_light_paths_match_oracle(oracle_mod):
+    """24 z-slabs under a light travelling along +z: a top vertex's light segment crosses 23
+    other bricks, past the 16 hits the exchange keeps in registers, so those vertices take its
+    rescan path; maps and image stay bit-exact."""
+    sc = scenes.make_scene("slabs", 49, seed=7, nblobs=8, width=48, height=40, light="directional")
+    d = np.array([0.12, 0.07, 1.0])
+    sc.light_pos = (d / np.linalg.norm(d)).astype(np.float32)
+    ranges = np.array([[0, 48, 0, 48, 2 * k, 2 * k + 2] for k in range(24)])
+    print("[bricks slabs x24]", flush=True)
+    img, depth, maps, t = _gpu(sc, ranges)
+    (oimg, local, final, partial), o, d = _oracle(oracle_mod, sc, ranges)
+    for b in range(len(ranges)):
+        assert np.array_equal(maps[b][1], final[b]), f"brick {b} final map"
+    assert np.array_equal(img, oimg)
