@@ -55,6 +55,23 @@ def test_null_handle_calls_fail_cleanly():
     assert L.beams_synchronize(None) == 1
     assert L.beams_set_sample_distance(None, 0.1) == 1
     assert L.beams_last_error(None) == b"null handle"
+    assert L.beams_set_learned(None, None, 0, 1e-3, 8) == 1
+    assert L.beams_get_learned_stats(None, None, None) == 1
+    assert L.beams_mlp_destroy(None) == 1
+    assert L.beams_mlp_last_error(None) == b"null handle"
+
+
+def test_mlp_parameter_count_and_missing_device():
+    """6,337 packed parameters: W1 32x64, b1 64, W2 64x64, b2 64, W3 64, b3."""
+    import learned_ref as R
+
+    L = B.lib()
+    assert L.beams_mlp_num_params() == R.NPARAMS == 32 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+    if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK | os.W_OK):
+        pytest.skip("a GPU device node is present")
+    h = C.c_void_p()
+    assert L.beams_mlp_create(C.byref(h), 1) == 5  # BEAMS_ERR_NO_DEVICE
+    assert L.beams_mlp_create(None, 1) == 1
 
 
 def test_generator_field_properties():
