@@ -4,10 +4,14 @@
 //                   RenderImage on the same camera rays (bit equality)
 #include "../../include/beams_mi355x.h"
 #include "beams/SceneFactory.h"
+#include "beams/io/DataSetUtils.h"
+
+#include <unistd.h>
 
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <string>
 
 static int failures = 0;
@@ -44,6 +48,123 @@ dtype = "bf16"
 map_enabled = false
 )";
 
+static std::string TempPath(const std::string& tag)
+{
+  return "/tmp/beams_io_" + std::to_string(getpid()) + "_" + tag;
+}
+
+static bool SameDataSet(const beams::DataSet& a, const beams::DataSet& b)
+{
+  if (a.CellSet.PointDims != b.CellSet.PointDims || a.Fields.size() != b.Fields.size())
+    return false;
+  for (int d = 0; d < 3; ++d)
+    if (a.Coords.Axes[d] != b.Coords.Axes[d])
+      return false;
+  for (size_t k = 0; k < a.Fields.size(); ++k)
+    if (a.Fields[k].Name != b.Fields[k].Name || a.Fields[k].Assoc != b.Fields[k].Assoc ||
+        a.Fields[k].Values != b.Fields[k].Values)
+      return false;
+  return true;
+}
+
+// VTK legacy files and chunking (pilot/io/DataSetUtils.cxx, convert/Chunk.cxx)
+static void test_io(const beams::DataSet& ds, const beams::DataSet& bf16)
+{
+  namespace io = beams::io;
+  for (auto ft : { io::FileType::BINARY, io::FileType::ASCII }) {
+    const std::string fn = TempPath(ft == io::FileType::BINARY ? "bin.vtk" : "asc.vtk");
+    CHECK(io::DataSetUtils::Write(ds, fn, ft).Success);
+    const io::ReadResult r = io::DataSetUtils::Read(fn);
+    CHECK(r.Success);
+    CHECK(SameDataSet(ds, r.Value));
+    std::remove(fn.c_str());
+  }
+  // rectilinear coordinates and a cell field
+  beams::DataSet rect = ds;
+  for (size_t i = 0; i < rect.Coords.Axes[1].size(); ++i)
+    rect.Coords.Axes[1][i] = 0.5f * (float)(i * i) / 23.0f;
+  beams::Field cf;
+  cf.Name = "cellid";
+  cf.Assoc = beams::Association::Cells;
+  for (int i = 0; i < 23 * 23 * 23; ++i) cf.Values.push_back((float)i * 0.25f);
+  rect.Fields.push_back(cf);
+  const std::string fr = TempPath("rect.vtk");
+  CHECK(io::DataSetUtils::Write(rect, fr).Success);
+  {
+    std::ifstream in(fr);
+    std::string l1, l2, l3, l4;
+    std::getline(in, l1), std::getline(in, l2), std::getline(in, l3), std::getline(in, l4);
+    CHECK(l4 == "DATASET RECTILINEAR_GRID");
+  }
+  const io::ReadResult rr = io::DataSetUtils::Read(fr);
+  CHECK(rr.Success && SameDataSet(rect, rr.Value));
+  std::remove(fr.c_str());
+  // BF16 storage is written as its decoded values
+  const std::string fb = TempPath("bf16.vtk");
+  CHECK(io::DataSetUtils::Write(bf16, fb).Success);
+  const io::ReadResult rb = io::DataSetUtils::Read(fb);
+  CHECK(rb.Success && rb.Value.Fields.size() == 1);
+  bool same = rb.Success;
+  for (size_t i = 0; same && i < bf16.Fields[0].Half.size(); ++i) {
+    const uint32_t bits = (uint32_t)bf16.Fields[0].Half[i] << 16;
+    float v;
+    std::memcpy(&v, &bits, 4);
+    same = rb.Value.Fields[0].Values[i] == v;
+  }
+  CHECK(same);
+  std::remove(fb.c_str());
+  // a file written by hand: ASCII, SPACING, FIELD data skipped, unsigned_char scalars
+  const std::string fh = TempPath("hand.vtk");
+  {
+    std::ofstream os(fh);
+    os << "# vtk DataFile Version 2.0\nhand\nASCII\nDATASET STRUCTURED_POINTS\nDIMENSIONS 2 2 2\n"
+          "ORIGIN 1 2 3\nSPACING 0.5 0.5 2\nPOINT_DATA 8\nFIELD FieldData 1\nlabels 1 8 int\n"
+          "0 1 2 3 4 5 6 7\nSCALARS density unsigned_char\nLOOKUP_TABLE default\n"
+          "0 10 20 30 40 50 60 255\n";
+  }
+  const io::ReadResult rh = io::DataSetUtils::Read(fh);
+  CHECK(rh.Success);
+  CHECK(rh.Success && rh.Value.Coords.Axes[2][1] == 5.0f && rh.Value.Fields[0].Values[7] == 255.0f);
+  std::remove(fh.c_str());
+  // errors
+  CHECK(!io::DataSetUtils::Read(TempPath("missing.vtk")).Success);
+  const std::string fe = TempPath("poly.vtk");
+  {
+    std::ofstream os(fe);
+    os << "# vtk DataFile Version 3.0\nx\nASCII\nDATASET POLYDATA\nPOINTS 0 float\n";
+  }
+  const io::ReadResult re = io::DataSetUtils::Read(fe);
+  CHECK(!re.Success && re.Error.find("POLYDATA") != std::string::npos);
+  std::remove(fe.c_str());
+  // chunking: 2 x 3 x 1 blocks sharing their boundary planes, values copied exactly
+  const auto ranges = io::ChunkPointRanges(ds.CellSet.PointDims, 2, 3, 1);
+  CHECK(ranges.size() == 6);
+  CHECK(ranges[0][0] == 0 && ranges[0][1] == 11 && ranges[1][0] == 11 && ranges[1][1] == 23);
+  CHECK(ranges[5][2] == 15 && ranges[5][3] == 23 && ranges[5][5] == 23);
+  const auto chunks = io::ChunkDataSet(ds, 2, 3, 1);
+  CHECK(chunks.size() == 6);
+  const auto& c5 = chunks[5];
+  const auto& src = ds.Fields[0].Values;
+  bool ok = c5.CellSet.PointDims[0] == 13 && c5.CellSet.PointDims[1] == 9;
+  for (int64_t k = 0; ok && k < c5.CellSet.PointDims[2]; ++k)
+    for (int64_t j = 0; ok && j < c5.CellSet.PointDims[1]; ++j)
+      for (int64_t i = 0; ok && i < c5.CellSet.PointDims[0]; ++i)
+        ok = c5.Fields[0].Values[(size_t)((k * 9 + j) * 13 + i)] == src[(size_t)((k * 24 + 15 + j) * 24 + 11 + i)];
+  CHECK(ok);
+  const std::string prefix = TempPath("chunk");
+  CHECK(io::SaveChunksToDisk(chunks, prefix).Success);
+  const io::ReadResult rc = io::DataSetUtils::Read(prefix + "_5.vtk");
+  CHECK(rc.Success && SameDataSet(c5, rc.Value));
+  for (size_t i = 0; i < chunks.size(); ++i) std::remove((prefix + "_" + std::to_string(i) + ".vtk").c_str());
+  bool threw = false;
+  try {
+    io::ChunkPointRanges(ds.CellSet.PointDims, 0, 1, 1);
+  } catch (const beams::ErrorBadValue&) {
+    threw = true;
+  }
+  CHECK(threw);
+}
+
 static void test_cpu()
 {
   beams::Config cfg;
@@ -78,6 +199,7 @@ static void test_cpu()
   beams::DataSet ds2;
   CHECK(beams::DataSetFactory::CreateDataSetFromPreset(cfg.Presets.at("half"), ds2, fname).Success);
   CHECK(ds2.Fields[0].Type == beams::DType::BF16 && ds2.Fields[0].Half.size() == 16 * 16 * 16);
+  test_io(ds, ds2);
 
   beams::rendering::LightedVolumeRenderer lvr;
   bool threw = false;
