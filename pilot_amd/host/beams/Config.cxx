@@ -80,7 +80,8 @@ Result Apply(Preset& p, const std::string& key, const std::string& raw, int line
   if (key == "dataset")
     p.DataSet.Factory = str;
   else if (key == "size" || key == "seed" || key == "blobs" || key == "noise_octaves" ||
-           key == "noise_amp" || key == "dtype" || key == "time" || key == "drift_speed")
+           key == "noise_amp" || key == "dtype" || key == "time" || key == "drift_speed" ||
+           key == "file_pattern" || key == "field")
     p.DataSet.Params[key] = str;
   else if (key == "map_enabled")
   {

@@ -1,5 +1,7 @@
 #include "SceneFactory.h"
 
+#include "io/DataSetUtils.h"
+
 #include "../../../include/beams_mi355x.h"
 
 #include <cstdlib>
@@ -15,9 +17,39 @@ std::string Param(const Preset& p, const std::string& key, const std::string& df
 }
 } // namespace
 
+// FileSceneBase::GetDataSet (FileSceneBase.cxx:33-49): the file of this rank, "%s" standing
+// for the number of ranks and "%r" for the rank; one device renders all bricks, so rank 0 of 1.
+Result ReadFileDataSet(const Preset& preset, DataSet& dataSet, std::string& fieldName)
+{
+  std::string name = Param(preset, "file_pattern", "");
+  if (name.empty())
+    return Result::Failed("file dataset needs file_pattern");
+  for (const auto& rep : { std::make_pair(std::string("%s"), std::string("1")),
+                           std::make_pair(std::string("%r"), std::string("0")) })
+    for (size_t p = name.find(rep.first); p != std::string::npos; p = name.find(rep.first, p))
+      name.replace(p, rep.first.size(), rep.second);
+  io::ReadResult r = io::DataSetUtils::Read(name);
+  if (!r.Success)
+    return Result::Failed(r.Error);
+  const std::string want = Param(preset, "field", "");
+  fieldName.clear();
+  for (const auto& f : r.Value.Fields)
+    if ((want.empty() && f.IsPointField()) || f.Name == want)
+    {
+      fieldName = f.Name;
+      break;
+    }
+  if (fieldName.empty())
+    return Result::Failed(name + ": no " + (want.empty() ? std::string("point field") : "field " + want));
+  dataSet = std::move(r.Value);
+  return Result::Succeeded();
+}
+
 Result DataSetFactory::CreateDataSetFromPreset(const Preset& preset, DataSet& dataSet,
                                                std::string& fieldName)
 {
+  if (preset.DataSet.Factory == "file")
+    return ReadFileDataSet(preset, dataSet, fieldName);
   if (preset.DataSet.Factory != "blobs")
     return Result::Failed("unknown dataset factory: " + preset.DataSet.Factory);
   const int64_t n = std::atoll(Param(preset, "size", "64").c_str());

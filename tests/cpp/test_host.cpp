@@ -156,6 +156,18 @@ static void test_io(const beams::DataSet& ds, const beams::DataSet& bf16)
   const io::ReadResult rc = io::DataSetUtils::Read(prefix + "_5.vtk");
   CHECK(rc.Success && SameDataSet(c5, rc.Value));
   for (size_t i = 0; i < chunks.size(); ++i) std::remove((prefix + "_" + std::to_string(i) + ".vtk").c_str());
+  // a file scene: preset dataset = "file" with the per-rank pattern
+  const std::string fs = TempPath("scene_r0_of_1.vtk");
+  CHECK(io::DataSetUtils::Write(ds, fs).Success);
+  beams::Config cfg;
+  const std::string toml = "[presets.f]\ndataset = \"file\"\nfile_pattern = \"" +
+                           TempPath("scene_r%r_of_%s.vtk") + "\"\n";
+  CHECK(cfg.Deserialize(toml).Success);
+  beams::DataSet fromFile;
+  std::string fname;
+  CHECK(beams::DataSetFactory::CreateDataSetFromPreset(cfg.Presets.at("f"), fromFile, fname).Success);
+  CHECK(fname == ds.Fields[0].Name && SameDataSet(ds, fromFile));
+  std::remove(fs.c_str());
   bool threw = false;
   try {
     io::ChunkPointRanges(ds.CellSet.PointDims, 0, 1, 1);
