@@ -175,43 +175,73 @@ __global__ void k_coarse_ranges(LearnedParams p)
   p.coarse_ranges[2 * t + 1] = (int32_t)cnt;
 }
 
-// One workgroup per coarse vertex; each wave takes rows of its box, lanes along x.
+// One workgroup per coarse (y, z) index pair: it reads the field rows of that pair's boxes
+// once, coalesced along x, and spreads each column's sum over the (one or two) coarse x boxes
+// holding that column.  A field coarser than the mip (a column in three or more boxes) sums
+// each box directly instead.  Field x extent up to kCoarseMaxX points.
+constexpr int kCoarseMaxX = 4097;
+
 template <int FDT>
 __global__ void __launch_bounds__(kBlock) k_coarse_sigma(LearnedParams p)
 {
   __shared__ float sig_tab[1024];
-  __shared__ float red[kBlock / 64];
+  __shared__ int16_t first_box[kCoarseMaxX], second_box[kCoarseMaxX];
+  __shared__ float sums[64];
+  __shared__ int crowded;
+  const GridParams& g = p.g;
+  const int cn = p.cn, n0 = (int)g.n[0];
   for (int i = threadIdx.x; i < p.s.ntf; i += blockDim.x)
     sig_tab[i] = sigma_of(p.s.tf[4 * i + 3] / p.s.max_density, p.inv_h);
-  __syncthreads();
-  const GridParams& g = p.g;
-  const int cn = p.cn, id = (int)blockIdx.x;
-  const int ci[3] = { id % cn, (id / cn) % cn, id / (cn * cn) };
-  int lo[3], cnt[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    lo[d] = p.coarse_ranges[2 * (d * cn + ci[d])];
-    cnt[d] = p.coarse_ranges[2 * (d * cn + ci[d]) + 1];
+  for (int i = threadIdx.x; i < n0; i += blockDim.x) {
+    first_box[i] = -1;
+    second_box[i] = -1;
   }
-  const int total = cnt[0] * cnt[1] * cnt[2];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  float acc = 0.0f;
-  for (int row = wave; row < cnt[1] * cnt[2]; row += kBlock / 64) {
-    const int64_t j = lo[1] + row % cnt[1], k = lo[2] + row / cnt[1];
-    const int64_t base = j * g.field_stride[1] + k * g.field_stride[2];
-    for (int i = lane; i < cnt[0]; i += 64) {
-      const float val = load_t<FDT>(g.field, (int64_t)(lo[0] + i) * g.field_stride[0] + base);
-      acc += sig_tab[tf_index(p.s, val)];
-    }
-  }
-#pragma unroll
-  for (int mk = 32; mk >= 1; mk >>= 1) acc += __shfl_xor(acc, mk, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  for (int i = threadIdx.x; i < cn; i += blockDim.x) sums[i] = 0.0f;
   __syncthreads();
   if (threadIdx.x == 0) {
-    float sum = 0.0f;
-    for (int w = 0; w < kBlock / 64; ++w) sum += red[w];
-    p.coarse_sigma[id] = sum / (float)total;
+    crowded = 0;
+    for (int cx = 0; cx < cn; ++cx) {
+      const int lo = p.coarse_ranges[2 * cx], cnt = p.coarse_ranges[2 * cx + 1];
+      for (int i = lo; i < lo + cnt; ++i) {
+        if (first_box[i] < 0)
+          first_box[i] = (int16_t)cx;
+        else if (second_box[i] < 0)
+          second_box[i] = (int16_t)cx;
+        else
+          crowded = 1;
+      }
+    }
+  }
+  __syncthreads();
+  const int cj = (int)blockIdx.x % cn, ck = (int)blockIdx.x / cn;
+  const int lo1 = p.coarse_ranges[2 * (cn + cj)], cnt1 = p.coarse_ranges[2 * (cn + cj) + 1];
+  const int lo2 = p.coarse_ranges[2 * (2 * cn + ck)], cnt2 = p.coarse_ranges[2 * (2 * cn + ck) + 1];
+  if (crowded) {
+    for (int cx = 0; cx < cn; ++cx) {
+      const int lo0 = p.coarse_ranges[2 * cx], cnt0 = p.coarse_ranges[2 * cx + 1];
+      float acc = 0.0f;
+      for (int q = threadIdx.x; q < cnt0 * cnt1 * cnt2; q += blockDim.x) {
+        const int64_t i = lo0 + q % cnt0, j = lo1 + (q / cnt0) % cnt1, k = lo2 + q / (cnt0 * cnt1);
+        const float val = load_t<FDT>(g.field, i * g.field_stride[0] + j * g.field_stride[1] + k * g.field_stride[2]);
+        acc += sig_tab[tf_index(p.s, val)];
+      }
+      atomicAdd(&sums[cx], acc);
+    }
+  }
+  for (int i = threadIdx.x; i < (crowded ? 0 : n0); i += blockDim.x) {
+    float acc = 0.0f;
+    for (int row = 0; row < cnt1 * cnt2; ++row) {
+      const int64_t j = lo1 + row % cnt1, k = lo2 + row / cnt1;
+      const float val = load_t<FDT>(g.field, (int64_t)i * g.field_stride[0] + j * g.field_stride[1] + k * g.field_stride[2]);
+      acc += sig_tab[tf_index(p.s, val)];
+    }
+    if (first_box[i] >= 0) atomicAdd(&sums[first_box[i]], acc);
+    if (second_box[i] >= 0) atomicAdd(&sums[second_box[i]], acc);
+  }
+  __syncthreads();
+  for (int cx = threadIdx.x; cx < cn; cx += blockDim.x) {
+    const int cnt0 = p.coarse_ranges[2 * cx + 1];
+    p.coarse_sigma[cx + (int64_t)cn * (cj + (int64_t)cn * ck)] = sums[cx] / (float)(cnt0 * cnt1 * cnt2);
   }
 }
 
@@ -505,11 +535,11 @@ int64_t learned_check_count(const MapParams& m, int32_t stride)
 
 hipError_t launch_coarse(const LearnedParams& p, hipStream_t st)
 {
-  if (p.s.ntf > 1024) return hipErrorInvalidValue;
+  if (p.s.ntf > 1024 || p.g.n[0] > kCoarseMaxX || p.cn > 64) return hipErrorInvalidValue;
   const int64_t nc = (int64_t)p.cn * p.cn * p.cn;
   hipLaunchKernelGGL(k_coarse_ranges, dim3((unsigned)((3 * p.cn + 63) / 64)), dim3(64), 0, st, p);
   BEAMS_DISPATCH_FIELD(p.g.dtype, {
-    hipLaunchKernelGGL((k_coarse_sigma<FDT>), dim3((unsigned)nc), dim3(kBlock), 0, st, p);
+    hipLaunchKernelGGL((k_coarse_sigma<FDT>), dim3((unsigned)(p.cn * p.cn)), dim3(kBlock), 0, st, p);
   });
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

@@ -89,10 +89,11 @@ def _ref_scene(sc):
                    sc.light_pos, sc.map_cells)
 
 
-@pytest.mark.parametrize("light", ["point", "directional"])
-def test_samples_match_restatement(oracle_mod, light):
+@pytest.mark.parametrize("light,n", [("point", 33), ("directional", 33), ("point", 20)])
+def test_samples_match_restatement(oracle_mod, light, n):
+    """n = 20: the field is coarser than the 32^3 mip, so coarse boxes share field columns."""
     O = oracle_mod
-    sc = _scene(41, light=light)
+    sc = _scene(41, n=n, cells=16, light=light)
     with B.LightedVolumeRenderer() as r:
         scenes.configure(r, sc)
         x, t = r.LearnedSamples(400, seed=9, mode=B.SHADOW_REF_MARCH)
