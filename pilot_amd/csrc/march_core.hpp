@@ -122,6 +122,47 @@ __device__ __forceinline__ int64_t walk_cell(const float* x, int64_t ncell, floa
   return c;
 }
 
+// The map cell of the previous lookup and its corner opacities: consecutive samples of a ray
+// usually share a map cell, whose corners need not be gathered again.  Same result as
+// map_opacity (the values are the same loads; the interpolation is recomputed each time).
+struct MapCellCache {
+  int c[3] = { -1, -1, -1 };
+  float v[8];
+};
+
+template <int MDT>
+__device__ __forceinline__ float map_opacity_cached(const MapParams& m, const float* p, MapCellCache& mc)
+{
+  float pc[3];
+  int cell[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float t = (p[d] - m.morg[d]) * m.minv[d];
+    if (t < 0.0f) t = 0.0f;
+    if (t >= (float)(m.mp[d] - 1)) t = (float)(m.mp[d] - 2);
+    cell[d] = (int)t;
+    float lo = m.morg[d] + m.msp[d] * (float)cell[d];
+    float hi = m.morg[d] + m.msp[d] * (float)(cell[d] + 1);
+    pc[d] = (p[d] - lo) / (hi - lo);
+  }
+  if (cell[0] != mc.c[0] || cell[1] != mc.c[1] || cell[2] != mc.c[2]) {
+    const int64_t s0 = m.ms[0], sx = m.ms[1], sxy = m.ms[2];
+    const int64_t i0 = cell[2] * sxy + cell[1] * sx + cell[0] * s0;
+    mc.v[0] = load_t<MDT>(m.map, i0);
+    mc.v[1] = load_t<MDT>(m.map, i0 + s0);
+    mc.v[2] = load_t<MDT>(m.map, i0 + s0 + sx);
+    mc.v[3] = load_t<MDT>(m.map, i0 + sx);
+    mc.v[4] = load_t<MDT>(m.map, i0 + sxy);
+    mc.v[5] = load_t<MDT>(m.map, i0 + sxy + s0);
+    mc.v[6] = load_t<MDT>(m.map, i0 + sxy + s0 + sx);
+    mc.v[7] = load_t<MDT>(m.map, i0 + sxy + sx);
+    mc.c[0] = cell[0];
+    mc.c[1] = cell[1];
+    mc.c[2] = cell[2];
+  }
+  return interp_hex(mc.v, pc);
+}
+
 // TransmittanceMap.h:93-112, 166-229: trilinear map opacity at p.
 template <int MDT>
 __device__ __forceinline__ float map_opacity(const MapParams& m, const float* p)

@@ -120,6 +120,7 @@ __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapPa
   const int64_t cms = s.ntf - 1;
   const int64_t sx = g.field_stride[0], sy = g.field_stride[1], sz = g.field_stride[2];
   const float sd = s.sample_dist;
+  MapCellCache mcache;
   while (inside(g, p) && dist < maxD && ++steps < s.max_steps) {
     const float mint = fminf(tx, fminf(ty, tz));
     const float maxt = fmaxf(tx, fmaxf(ty, tz));
@@ -171,7 +172,7 @@ __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapPa
       float4 c = tf[ci];
       c.w *= (1.0f - color[3]);
       if (USE_MAP) {
-        const float tr = 1.0f - map_opacity<MDT>(m, p);
+        const float tr = 1.0f - map_opacity_cached<MDT>(m, p, mcache);
         c.x = c.x * (tr * m.light_color[0]);
         c.y = c.y * (tr * m.light_color[1]);
         c.z = c.z * (tr * m.light_color[2]);
