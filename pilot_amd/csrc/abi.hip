@@ -903,6 +903,9 @@ int beams_set_field(beams_renderer* h, const void* values, int32_t dtype, int32_
       if (!(x[d][(size_t)i] > x[d][(size_t)i - 1]))
         return h->fail(BEAMS_ERR_INVALID, "coordinates must be strictly increasing");
   }
+  // the kernels index cells and strides in 32 bits within an xy plane
+  if (dims[0] * dims[1] >= ((int64_t)1 << 31))
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "a field's xy plane must hold fewer than 2^31 points");
   const int64_t nvals = (assoc == BEAMS_ASSOC_CELLS) ? (dims[0] - 1) * (dims[1] - 1) * (dims[2] - 1)
                                                      : dims[0] * dims[1] * dims[2];
   const size_t bytes = (size_t)nvals * dtype_size(dtype);

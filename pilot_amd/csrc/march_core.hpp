@@ -110,6 +110,17 @@ __device__ __forceinline__ int64_t oracle_cell(const float* x, int64_t ncell, fl
   return c;
 }
 
+// The same cell as oracle_cell in 32-bit indices (clamping the guess before truncating it
+// gives the same index): for per-step use where the axis has fewer than 2^31 points.
+__device__ __forceinline__ int oracle_cell32(const float* x, int ncell, float v, float x0, float ginv)
+{
+  const float gf = (v - x0) * ginv;
+  int c = (gf > 0.0f) ? (int)fminf(gf, (float)(ncell - 1)) : 0;
+  while (c > 0 && v <= x[c]) --c;
+  while (c < ncell - 1 && v > x[c + 1]) ++c;
+  return c;
+}
+
 // LightedVolumeRenderer.cxx:123-164: unique c with x[c] <= v < x[c+1] (v != max).
 __device__ __forceinline__ int64_t walk_cell(const float* x, int64_t ncell, float v, float x0,
                                              float ginv)
@@ -161,6 +172,23 @@ __device__ __forceinline__ float map_opacity_cached(const MapParams& m, const fl
     mc.c[2] = cell[2];
   }
   return interp_hex(mc.v, pc);
+}
+
+// walk_cell in 32-bit indices.
+__device__ __forceinline__ int walk_cell32(const float* x, int ncell, float v, float x0, float ginv)
+{
+  const float gf = (v - x0) * ginv;
+  int c = (gf > 0.0f) ? (int)fminf(gf, (float)(ncell - 1)) : 0;
+  while (c > 0 && v < x[c]) --c;
+  while (c < ncell - 1 && v >= x[c + 1]) ++c;
+  return c;
+}
+
+// The reference's TF index (truncate toward zero, then clamp to the table): clamping in float
+// first gives the same index for every finite product and keeps it 32-bit.
+__device__ __forceinline__ int tf_clamped_index(float t, float tfmax)
+{
+  return (int)fminf(fmaxf(t, 0.0f), tfmax);
 }
 
 // TransmittanceMap.h:93-112, 166-229: trilinear map opacity at p.

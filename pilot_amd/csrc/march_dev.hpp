@@ -48,9 +48,10 @@ __device__ __forceinline__ float march_vertex(const GridParams& g, const MapPara
     p[d] = start[d];
   }
   const float len2 = (tmax - tmin) * (tmax - tmin);
-  const int64_t tfmax = s.ntf - 1;
-  const int64_t c0 = g.n[0] - 1, c1 = g.n[1] - 1, c2 = g.n[2] - 1;
-  const int64_t sx = g.field_stride[0], sy = g.field_stride[1], sz = g.field_stride[2];
+  const float tfmax = (float)(s.ntf - 1);
+  // 32-bit cells and strides: set_field keeps a field's xy plane below 2^31 points
+  const int c0 = (int)(g.n[0] - 1), c1 = (int)(g.n[1] - 1), c2 = (int)(g.n[2] - 1);
+  const int sx = (int)g.field_stride[0], sy = (int)g.field_stride[1], sz = (int)g.field_stride[2];
   float op = seed_value;
   int64_t taken = 0;
   for (int64_t it = 0; it < kMapMaxSteps; ++it) {
@@ -61,14 +62,14 @@ __device__ __forceinline__ float march_vertex(const GridParams& g, const MapPara
     p[2] = p[2] + m.step * dir[2];
     if (!inside(g, p)) break;
     ++taken;
-    const int64_t ci = oracle_cell(x[0], c0, p[0], g.gx0[0], g.ginv[0]);
-    const int64_t cj = oracle_cell(x[1], c1, p[1], g.gx0[1], g.ginv[1]);
-    const int64_t ck = oracle_cell(x[2], c2, p[2], g.gx0[2], g.ginv[2]);
+    const int ci = oracle_cell32(x[0], c0, p[0], g.gx0[0], g.ginv[0]);
+    const int cj = oracle_cell32(x[1], c1, p[1], g.gx0[1], g.ginv[1]);
+    const int ck = oracle_cell32(x[2], c2, p[2], g.gx0[2], g.ginv[2]);
     float pc[3];
     pc[0] = (p[0] - x[0][ci]) / (x[0][ci + 1] - x[0][ci]);
     pc[1] = (p[1] - x[1][cj]) / (x[1][cj + 1] - x[1][cj]);
     pc[2] = (p[2] - x[2][ck]) / (x[2][ck + 1] - x[2][ck]);
-    const int64_t b = ci * sx + cj * sy + ck * sz;
+    const int64_t b = (int64_t)ci * sx + (int64_t)cj * sy + (int64_t)ck * sz;
     float val[8];
     val[0] = load_t<FDT>(g.field, b);
     val[1] = load_t<FDT>(g.field, b + sx);
@@ -80,10 +81,7 @@ __device__ __forceinline__ float march_vertex(const GridParams& g, const MapPara
     val[7] = load_t<FDT>(g.field, b + sz + sy);
     float sc = interp_hex(val, pc);
     sc = (sc - s.min_scalar) * s.inv_delta;
-    int64_t idx = (int64_t)(sc * (float)tfmax);
-    idx = idx < tfmax ? idx : tfmax;
-    idx = idx > 0 ? idx : 0;
-    op = op + (1.0f - op) * a_tab[idx];
+    op = op + (1.0f - op) * a_tab[tf_clamped_index(sc * tfmax, tfmax)];
     if (op == 1.0f) break;  // every later step leaves an opacity of exactly 1 unchanged
   }
   *taken_out = taken;
@@ -115,10 +113,11 @@ __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapPa
   float tx = 0.0f, ty = 0.0f, tz = 0.0f;
   float s0 = 0.0f, s10 = 0.0f, s23 = 0.0f, s3 = 0.0f, s4 = 0.0f, s54 = 0.0f, s67 = 0.0f,
         s7 = 0.0f;
-  int64_t cell[3] = { 0, 0, 0 };
+  int cell[3] = { 0, 0, 0 };
   float invsp[3] = { 0.0f, 0.0f, 0.0f };
-  const int64_t cms = s.ntf - 1;
-  const int64_t sx = g.field_stride[0], sy = g.field_stride[1], sz = g.field_stride[2];
+  const int cms = s.ntf - 1;
+  // 32-bit cells and strides: set_field keeps a field's xy plane below 2^31 points
+  const int sx = (int)g.field_stride[0], sy = (int)g.field_stride[1], sz = (int)g.field_stride[2];
   const float sd = s.sample_dist;
   MapCellCache mcache;
   while (inside(g, p) && dist < maxD && ++steps < s.max_steps) {
@@ -129,14 +128,14 @@ __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapPa
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         if (p[d] == g.maxp[d]) {
-          cell[d] = g.n[d] - 2;  // invSpacing keeps its previous value
+          cell[d] = (int)g.n[d] - 2;  // invSpacing keeps its previous value
         } else {
-          const int64_t c = walk_cell(x[d], g.n[d] - 1, p[d], g.gx0[d], g.ginv[d]);
+          const int c = walk_cell32(x[d], (int)g.n[d] - 1, p[d], g.gx0[d], g.ginv[d]);
           invsp[d] = 1.0f / (x[d][c + 1] - x[d][c]);
           cell[d] = c;
         }
       }
-      const int64_t b = cell[0] * sx + cell[1] * sy + cell[2] * sz;
+      const int64_t b = (int64_t)cell[0] * sx + (int64_t)cell[1] * sy + (int64_t)cell[2] * sz;
       bl[0] = x[0][cell[0]];
       bl[1] = x[1][cell[1]];
       bl[2] = x[2][cell[2]];
@@ -165,7 +164,10 @@ __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapPa
     const float bot = l01 + ty * (l32 - l01);
     float fs = bot + tz * (top - bot);
     fs = (fs - s.min_scalar) * s.inv_delta;
-    const int64_t ci = (int64_t)(fs * (float)cms);
+    // truncation toward zero as the reference's static_cast<Id>, after clamping to
+    // [-1, cms + 1]: a product out of the table's range stays out of it, in 32 bits
+    const float fci = fs * (float)cms;
+    const int ci = (int)fminf(fmaxf(fci, -1.0f), (float)cms + 1.0f);
     // The reference re-evaluates the same sample forever for an index outside the
     // table; here the sample is skipped and the ray advances.
     if (ci >= 0 && ci <= cms) {
