@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--learned-size", type=int, default=512)
     ap.add_argument("--learned-samples", type=int, default=4 << 20)
     ap.add_argument("--learned-epochs", type=int, default=12)
+    ap.add_argument("--learned-eval-seeds", type=int, default=4, help="held-out fields evaluated (1..4)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device 5 times (the byte-counter calibration run)")
     return ap.parse_args()
@@ -529,7 +530,8 @@ def main():
         net, train = train_closure(args.learned_size, seeds, args.learned_samples // len(seeds),
                                    args.learned_epochs)
         print(f"[bench] closure trained: {train}", file=sys.stderr, flush=True)
-        wl = learned_workload(max(2, args.steps // 4), 1, net, n=args.learned_size)
+        wl = learned_workload(max(2, args.steps // 4), 1, net, n=args.learned_size,
+                              seeds=(2001, 2002, 2003, 2004)[:max(1, min(4, args.learned_eval_seeds))])
         wl["training"] = train
         line.setdefault("workloads", {})["learned"] = wl
         if not args.no_series:
