@@ -288,28 +288,37 @@ __global__ void __launch_bounds__(kBlock) k_coarse_tau(LearnedParams p)
 }
 
 // ---- the learned map: 64 vertices per wave, four waves per workgroup -----------------------
-// Static part of the learned-map kernels' LDS (51 KB); the density-patch table (ntf floats)
-// is dynamic, so that three workgroups fit a CU for the usual 256-entry TF.
+// Static part of the learned-map kernels' LDS (38 KB: the waves' activation tiles and the
+// biases; the weights live in registers); the density-patch table (ntf floats) is dynamic.
 struct MapLds {
-  MlpLds w;
+  MlpBias w;
   uint16_t buf[kBlock / 64][64 * kLd];
 };
 
 template <int FDT, int MDT>
 // Work unit: a chunk of 64 consecutive vertices of one map row (x), one per wave; the
 // vertex position is map_vertex's, computed from the row and the column without divisions
-// of the 64-bit vertex index.
+// of the 64-bit vertex index.  Workgroups are dealt to the 8 XCDs round-robin, so each XCD
+// takes one contiguous eighth of the chunks: its waves walk neighbouring rows together and
+// the 3 x 3 field rows of their patches stay in that XCD's L2.  The grid is a multiple of 8.
 __global__ void __launch_bounds__(kBlock) k_learned_map(LearnedParams p, int32_t chunks_per_row,
                                                         int32_t nchunks)
 {
   __shared__ MapLds s;
   extern __shared__ float d_tab[];
-  if (p.params) mlp_stage(p.params, s.w);
+  MlpFrags fr;
+  if (p.params) {
+    mlp_load_frags(p.params, fr);
+    mlp_stage_bias(p.params, s.w);
+  }
   stage_patch(p.s, d_tab);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int mp0 = (int)p.m.mp[0], mp1 = (int)p.m.mp[1];
-  for (int chunk = blockIdx.x * (kBlock / 64) + wave; chunk < nchunks; chunk += gridDim.x * (kBlock / 64)) {
+  const int xcd = (int)(blockIdx.x % 8), slot = (int)(blockIdx.x / 8);
+  const int waves_per_xcd = (int)(gridDim.x / 8) * (kBlock / 64);
+  const int first = (int)((int64_t)nchunks * xcd / 8), last = (int)((int64_t)nchunks * (xcd + 1) / 8);
+  for (int chunk = first + slot * (kBlock / 64) + wave; chunk < last; chunk += waves_per_xcd) {
     const int row = chunk / chunks_per_row;
     const int i0 = (chunk - row * chunks_per_row) * 64;
     const int j = row % mp1, k = row / mp1;
@@ -319,7 +328,7 @@ __global__ void __launch_bounds__(kBlock) k_learned_map(LearnedParams p, int32_t
     v[1] = p.m.morg[1] + p.m.msp[1] * (float)j;
     v[2] = p.m.morg[2] + p.m.msp[2] * (float)k;
     const float tc = features<FDT>(p, d_tab, v, f);
-    const float y = p.params ? mlp_forward_fast(s.w, s.buf[wave], f) : 0.0f;
+    const float y = p.params ? mlp_forward_fast(fr, s.w, s.buf[wave], f) : 0.0f;
     const float tau = fmaxf(tc - y, 0.0f);
     if (i0 + lane < mp0) store_t<MDT>(p.m.map, (int64_t)row * mp0 + i, 1.0f - expf(-tau));
   }
@@ -396,7 +405,11 @@ __global__ void __launch_bounds__(kBlock) k_bricks_learned_map(LearnedParams p, 
 {
   __shared__ MapLds s;
   extern __shared__ float d_tab[];
-  if (p.params) mlp_stage(p.params, s.w);
+  MlpFrags fr;
+  if (p.params) {
+    mlp_load_frags(p.params, fr);
+    mlp_stage_bias(p.params, s.w);
+  }
   stage_patch(p.s, d_tab);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -408,7 +421,7 @@ __global__ void __launch_bounds__(kBlock) k_bricks_learned_map(LearnedParams p, 
     float v[3], feat[kFeat];
     map_vertex(d.m, valid ? id : nv - 1, v);
     const float tc = features<FDT>(p, d_tab, v, feat);
-    const float y = p.params ? mlp_forward_fast(s.w, s.buf[wave], feat) : 0.0f;
+    const float y = p.params ? mlp_forward_fast(fr, s.w, s.buf[wave], feat) : 0.0f;
     if (valid) store_t<MDT>(d.map_final, id, 1.0f - expf(-fmaxf(tc - y, 0.0f)));
   }
 }
@@ -555,7 +568,7 @@ hipError_t launch_learned_map(const LearnedParams& p, hipStream_t st)
   const int64_t cpr = (p.m.mp[0] + 63) / 64;
   const int64_t nchunks = cpr * p.m.mp[1] * p.m.mp[2];
   if (nchunks > INT32_MAX) return hipErrorInvalidValue;
-  const int64_t groups = std::min<int64_t>((nchunks + 3) / 4, 256 * 12);
+  const int64_t groups = std::max<int64_t>(8, std::min<int64_t>((nchunks + 3) / 4, 256 * 12) / 8 * 8);
   BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(p.m.dtype, {
     hipLaunchKernelGGL((k_learned_map<FDT, MDT>), dim3((unsigned)groups), dim3(kBlock), sizeof(float) * p.s.ntf,
                        st, p, (int32_t)cpr, (int32_t)nchunks);

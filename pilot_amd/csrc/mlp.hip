@@ -188,7 +188,7 @@ __global__ void k_zero_f32(float* p, int32_t n)
 }
 
 struct InferLds {
-  MlpLds w;
+  MlpBias w;
   uint16_t buf[4][64 * kLd];
 };
 
@@ -196,7 +196,9 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float* params, const fl
                                                    float* y)
 {
   __shared__ InferLds s;
-  mlp_stage(params, s.w);
+  MlpFrags fr;
+  mlp_load_frags(params, fr);
+  mlp_stage_bias(params, s.w);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int64_t chunk = (int64_t)blockIdx.x * 4 + wave; chunk * 64 < n; chunk += (int64_t)gridDim.x * 4) {
@@ -204,7 +206,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float* params, const fl
     float feat[kFeat];
 #pragma unroll
     for (int k = 0; k < kFeat; ++k) feat[k] = row < n ? x[row * kFeat + k] : 0.0f;
-    const float out = mlp_forward_fast(s.w, s.buf[wave], feat);
+    const float out = mlp_forward_fast(fr, s.w, s.buf[wave], feat);
     if (row < n) y[row] = out;
   }
 }

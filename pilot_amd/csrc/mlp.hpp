@@ -177,11 +177,56 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpLds& w, uint16_t* buf
   return y;
 }
 
+// Inference weights: each lane keeps its A-operand fragments of W1^T and W2^T in registers
+// (48 VGPRs) and the workgroup shares the biases and W3 in LDS, so the LDS per workgroup is
+// the per-wave activation tiles and little else.
+struct MlpBias {
+  alignas(16) float b1[kHid];
+  alignas(16) float b2[kHid];
+  alignas(16) float w3[kHid];
+  float b3;
+};
+
+struct MlpFrags {
+  bf16x8 w1[4];     // W1^T rows 16 nt + r, features 8g .. 8g+7
+  bf16x8 w2[4][2];  // W2^T rows 16 nt + r, units 32 ks + 8g .. +7
+};
+
+__device__ __forceinline__ void mlp_load_frags(const float* params, MlpFrags& fr)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    u16x8 u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) u[j] = bf16_hw(params[kOffW1 + (8 * g + j) * kHid + 16 * nt + r]);
+    fr.w1[nt] = __builtin_bit_cast(bf16x8, u);
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) u[j] = bf16_hw(params[kOffW2 + (32 * ks + 8 * g + j) * kHid + 16 * nt + r]);
+      fr.w2[nt][ks] = __builtin_bit_cast(bf16x8, u);
+    }
+  }
+}
+
+__device__ __forceinline__ void mlp_stage_bias(const float* params, MlpBias& b)
+{
+  for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
+    b.b1[i] = params[kOffB1 + i];
+    b.b2[i] = params[kOffB2 + i];
+    b.w3[i] = bf16_to_f32(f32_to_bf16(params[kOffW3 + i]));
+  }
+  if (threadIdx.x == 0) b.b3 = params[kOffB3];
+}
+
 // Inference forward pass, computed transposed (H^T = W^T X^T): the weights are the A
 // operands and the rows the B operands, so a lane's accumulator holds four consecutive hidden
 // units of one row.  Layer 1 leaves one 8-byte LDS store per tile, and the output needs only
-// a reduction over the four lane groups.  Same contract as mlp_forward_wave (no ybuf).
-__device__ __forceinline__ float mlp_forward_fast(const MlpLds& w, uint16_t* buf, const float* feat)
+// a reduction over the four lane groups.  buf: a per-wave LDS tile of 64 * kLd bf16; every lane
+// of the wave calls it with its row's 32 features and receives that row's output.
+__device__ __forceinline__ float mlp_forward_fast(const MlpFrags& fr, const MlpBias& w, uint16_t* buf,
+                                                  const float* feat)
 {
 #pragma clang fp contract(fast)
   const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
@@ -202,7 +247,7 @@ __device__ __forceinline__ float mlp_forward_fast(const MlpLds& w, uint16_t* buf
   // layer 1: unit tile nt x row tile vt
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
-    const bf16x8 a = lds_frag(&w.w1t[16 * nt + r][8 * g]);
+    const bf16x8 a = fr.w1[nt];
     // the bias enters as the accumulator input (rounded once, after the products)
     const f32x4 bias = *(const f32x4*)&w.b1[16 * nt + 4 * g];
 #pragma unroll
@@ -225,8 +270,8 @@ __device__ __forceinline__ float mlp_forward_fast(const MlpLds& w, uint16_t* buf
   float part[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
-    const bf16x8 a0 = lds_frag(&w.w2t[16 * nt + r][8 * g]);
-    const bf16x8 a1 = lds_frag(&w.w2t[16 * nt + r][32 + 8 * g]);
+    const bf16x8 a0 = fr.w2[nt][0];
+    const bf16x8 a1 = fr.w2[nt][1];
     const f32x4 bias = *(const f32x4*)&w.b2[16 * nt + 4 * g];
     const f32x4 w3 = *(const f32x4*)&w.w3[16 * nt + 4 * g];
 #pragma unroll
