@@ -79,9 +79,10 @@ def test_mlp_training_fits_a_smooth_target():
     assert abs(loss - l1) < 0.5 * l0
 
 
-def _scene(seed, n=33, cells=16, light="point"):
+def _scene(seed, n=33, cells=16, light="point", field_dtype=B.F32, map_dtype=B.F32):
     return scenes.make_scene("learned_small", n, seed=seed, nblobs=12, noise_octaves=2, noise_amp=0.2,
-                             light=light, map_cells=(cells, cells, cells), width=96, height=80)
+                             light=light, map_cells=(cells, cells, cells), width=96, height=80,
+                             field_dtype=field_dtype, map_dtype=map_dtype)
 
 
 def _ref_scene(sc):
@@ -113,15 +114,19 @@ def test_samples_match_restatement(oracle_mod, light, n):
     assert np.allclose(t, L.target(tc, omap[ids]), rtol=1e-4, atol=1e-4)
 
 
-def test_learned_map_and_check(oracle_mod):
+@pytest.mark.parametrize("fdt,mdt", [(B.F32, B.F32), (B.BF16, B.F16)])
+def test_learned_map_and_check(oracle_mod, fdt, mdt):
+    """The learned map against the restated formula (features from the decoded storage
+    values; the FP16 map rounds the opacity), the check against the oracle's light march, and
+    the march over the learned map against the oracle's sampler over the same map."""
     O = oracle_mod
-    sc = _scene(51)
+    sc = _scene(51, field_dtype=fdt, map_dtype=mdt)
     with B.ShadowMLP(seed=5) as net:
         p = _random_params(52) * 0.2
         net.set_params(p)
         with B.LightedVolumeRenderer() as r:
             scenes.configure(r, sc)
-            r.SetShadow(True, sc.map_cells, B.F32, mode=B.SHADOW_LEARNED)
+            r.SetShadow(True, sc.map_cells, sc.map_dtype, mode=B.SHADOW_LEARNED)
             r.SetLearned(net, fallback=B.SHADOW_REF_MARCH, error_budget=1.0, check_stride=4)
             o, d = B.camera_rays(sc.camera, sc.width, sc.height)
             rays = r.Render(B.Rays(o, d))
