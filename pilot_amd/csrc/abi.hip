@@ -169,10 +169,16 @@ GridParams grid_params(const beams_renderer* h)
   g.field_stride[0] = 1;
   g.field_stride[1] = sx;
   g.field_stride[2] = sx * sy;
+  const int64_t nvals = (h->assoc == 1) ? (h->dims[0] - 1) * (h->dims[1] - 1) * (h->dims[2] - 1)
+                                        : h->dims[0] * h->dims[1] * h->dims[2];
+  const uint64_t fbytes = (uint64_t)nvals * dtype_size(h->dtype);
+  g.bytes32 = fbytes < ((uint64_t)1 << 32) ? (uint32_t)fbytes : 0u;
   const float* base = (const float*)h->coords.p;
+  const int64_t nc = h->dims[0] + h->dims[1] + h->dims[2];
   g.x[0] = base;
   g.x[1] = base + h->dims[0];
   g.x[2] = base + h->dims[0] + h->dims[1];
+  for (int d = 0; d < 3; ++d) g.r[d] = g.x[d] + nc;
   for (int d = 0; d < 3; ++d) {
     const std::vector<float>& x = h->hx[d];
     g.minp[d] = x.front();
@@ -525,6 +531,7 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
   int32_t max_coords = 0;
   const float* xdev = (const float*)h->coords.p;
   const int64_t xoff[3] = { 0, h->dims[0], h->dims[0] + h->dims[1] };
+  const int64_t ncoord = h->dims[0] + h->dims[1] + h->dims[2];
   for (int b = 0; b < nb; ++b) {
     const auto& br = h->bricks[(size_t)b];
     BrickDesc& d = desc[(size_t)b];
@@ -532,8 +539,10 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
     g.dtype = h->dtype;
     g.assoc = 0;
     const int64_t sx = br.sdims[0], sxy = br.sdims[0] * br.sdims[1];
-    g.field = (const unsigned char*)br.field.p +
-              esz * (size_t)((br.lo[2] - br.slo[2]) * sxy + (br.lo[1] - br.slo[1]) * sx + (br.lo[0] - br.slo[0]));
+    const size_t foff = esz * (size_t)((br.lo[2] - br.slo[2]) * sxy + (br.lo[1] - br.slo[1]) * sx + (br.lo[0] - br.slo[0]));
+    g.field = (const unsigned char*)br.field.p + foff;
+    const uint64_t fbytes = (uint64_t)esz * (uint64_t)(br.sdims[0] * br.sdims[1] * br.sdims[2]) - foff;
+    g.bytes32 = fbytes < ((uint64_t)1 << 32) ? (uint32_t)fbytes : 0u;
     g.field_stride[0] = 1;
     g.field_stride[1] = sx;
     g.field_stride[2] = sxy;
@@ -541,6 +550,7 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
       const std::vector<float>& x = h->hx[k];
       g.n[k] = br.hi[k] - br.lo[k] + 1;
       g.x[k] = xdev + xoff[k] + br.lo[k];
+      g.r[k] = g.x[k] + ncoord;
       g.minp[k] = x[(size_t)br.lo[k]];
       g.maxp[k] = x[(size_t)br.hi[k]];
       g.gx0[k] = g.minp[k];
@@ -902,7 +912,19 @@ int beams_set_field(beams_renderer* h, const void* values, int32_t dtype, int32_
     for (int64_t i = 1; i < dims[d]; ++i)
       if (!(x[d][(size_t)i] > x[d][(size_t)i - 1]))
         return h->fail(BEAMS_ERR_INVALID, "coordinates must be strictly increasing");
+    // the kernels divide by a cell width through its reciprocal and one correction step,
+    // which gives the IEEE quotient for widths in [2^-60, 2^61] and coordinates below 2^60
+    for (int64_t i = 0; i < dims[d]; ++i) {
+      const float v = x[d][(size_t)i];
+      if (!(std::fabs(v) <= 0x1p60f))
+        return h->fail(BEAMS_ERR_INVALID, "coordinates must be finite and below 2^60 in magnitude");
+      if (i > 0 && !(v - x[d][(size_t)i - 1] >= 0x1p-60f))
+        return h->fail(BEAMS_ERR_INVALID, "cell widths must be at least 2^-60");
+    }
   }
+  // the kernels stage the coordinates and reciprocal cell widths in LDS
+  if (dims[0] + dims[1] + dims[2] > 16384)
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "the three point dims must sum to at most 16384");
   // the kernels index cells and strides in 32 bits within an xy plane
   if (dims[0] * dims[1] >= ((int64_t)1 << 31))
     return h->fail(BEAMS_ERR_UNSUPPORTED, "a field's xy plane must hold fewer than 2^31 points");
@@ -911,12 +933,21 @@ int beams_set_field(beams_renderer* h, const void* values, int32_t dtype, int32_
   const size_t bytes = (size_t)nvals * dtype_size(dtype);
   HIP_TRY(h, h->field.ensure(bytes));
   HIP_TRY(h, hipMemcpyAsync(h->field.p, values, bytes, copy_kind(mem, true), h->stream));
+  // coordinates of the three axes, then their reciprocal cell widths (IEEE FP32 quotients,
+  // the last entry of each axis 0)
   const size_t nc = (size_t)(dims[0] + dims[1] + dims[2]);
   std::vector<float> flat;
-  flat.reserve(nc);
+  flat.reserve(2 * nc);
   for (int d = 0; d < 3; ++d) flat.insert(flat.end(), x[d].begin(), x[d].end());
-  HIP_TRY(h, h->coords.ensure(sizeof(float) * nc));
-  HIP_TRY(h, hipMemcpyAsync(h->coords.p, flat.data(), sizeof(float) * nc, hipMemcpyHostToDevice, h->stream));
+  for (int d = 0; d < 3; ++d) {
+    for (size_t i = 0; i + 1 < x[d].size(); ++i) {
+      const float w = x[d][i + 1] - x[d][i];
+      flat.push_back(1.0f / w);
+    }
+    flat.push_back(0.0f);
+  }
+  HIP_TRY(h, h->coords.ensure(sizeof(float) * 2 * nc));
+  HIP_TRY(h, hipMemcpyAsync(h->coords.p, flat.data(), sizeof(float) * 2 * nc, hipMemcpyHostToDevice, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   for (int d = 0; d < 3; ++d) {
     h->hx[d] = std::move(x[d]);

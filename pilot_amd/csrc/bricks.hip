@@ -27,12 +27,6 @@ __device__ __forceinline__ int next_item(unsigned int* counter, int* slot)
   return *slot;
 }
 
-__device__ __forceinline__ void stage_brick_coords(const GridParams& g, float* xs)
-{
-  const int64_t n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-  for (int64_t i = threadIdx.x; i < n0 + n1 + n2; i += blockDim.x)
-    xs[i] = (i < n0) ? g.x[0][i] : ((i < n0 + n1) ? g.x[1][i - n0] : g.x[2][i - n0 - n1]);
-}
 
 // ---- phases 1 and 3: per-brick TransmittanceMapGenerator -------------------------------------
 template <int FDT, int MDT>
@@ -52,7 +46,7 @@ __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
     const BrickDesc& bd = f.bricks[b];
     if (staged < 0)
       for (int i = threadIdx.x; i < ntf; i += blockDim.x) a_tab[i] = bd.s.tf[4 * i + 3] / bd.s.max_density;
-    if (b != staged) stage_brick_coords(bd.g, xs);
+    if (b != staged) stage_coords(bd.g, xs);
     staged = b;
     __syncthreads();
     const float* x[3] = { xs, xs + bd.g.n[0], xs + bd.g.n[0] + bd.g.n[1] };
@@ -296,7 +290,7 @@ __global__ void __launch_bounds__(256) k_bricks_march(BrickFrame f, CameraParams
     const int b = f.px_item_brick[item];
     const BrickDesc& bd = f.bricks[b];
     if (staged < 0) stage_tf(bd.s, tf);
-    if (b != staged) stage_brick_coords(bd.g, xs);
+    if (b != staged) stage_coords(bd.g, xs);
     staged = b;
     __syncthreads();
     const float* x[3] = { xs, xs + bd.g.n[0], xs + bd.g.n[0] + bd.g.n[1] };
@@ -395,7 +389,7 @@ hipError_t launch_bricks_map(const BrickFrame& f, int32_t field_dtype, int32_t m
                              int32_t ntf, bool seeded, hipStream_t st)
 {
   if (f.n_map_items == 0) return hipSuccess;
-  const size_t lds = sizeof(float) * (size_t)(ntf + f.max_coords);
+  const size_t lds = sizeof(float) * (size_t)(ntf + 2 * f.max_coords);
   const dim3 grid((unsigned)f.persistent_groups);
   BRICK_DISPATCH_FIELD(field_dtype, BRICK_DISPATCH_MAP(map_dtype, {
     hipLaunchKernelGGL((k_bricks_map<FDT, MDT>), grid, dim3(256), lds, st, f, seeded ? 1 : 0);
@@ -419,7 +413,7 @@ hipError_t launch_bricks_march(const BrickFrame& f, int32_t field_dtype, int32_t
                                int32_t ntf, bool use_map, const CameraParams& c, hipStream_t st)
 {
   if (f.n_px_items == 0) return hipSuccess;
-  const size_t lds = sizeof(float4) * (size_t)ntf + sizeof(float) * (size_t)f.max_coords;
+  const size_t lds = sizeof(float4) * (size_t)ntf + sizeof(float) * 2 * (size_t)f.max_coords;
   const dim3 grid((unsigned)f.persistent_groups);
   BRICK_DISPATCH_FIELD(field_dtype, BRICK_DISPATCH_MAP(map_dtype, {
     if (use_map)

@@ -31,7 +31,7 @@ struct BrickFrame {
   float* seeds;                   // non-local opacity per map vertex (phase 2 -> 3)
   float4* partial;                // per-brick footprint RGBA
   float* partial_depth;           // per-brick footprint entry distance
-  int32_t max_coords;             // LDS floats for the largest brick's coordinates
+  int32_t max_coords;             // the largest brick's n0 + n1 + n2 (LDS: coordinates + reciprocal widths)
   int32_t persistent_groups;      // workgroups of the persistent launches
 };
 

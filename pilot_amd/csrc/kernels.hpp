@@ -16,10 +16,12 @@ struct GridParams {
   int64_t n[3];        // point dims
   int64_t field_stride[3];  // element strides of the stored array (x, y, z)
   const float* x[3];   // coordinate arrays (device)
+  const float* r[3];   // reciprocal cell widths 1 / (x[c+1] - x[c]) in FP32 (device; r[n-1] = 0)
   float minp[3], maxp[3];
   float gx0[3], ginv[3];    // cell-guess helpers: x[0] and (n-1)/(x[n-1]-x[0])
   double bounds[6];         // FP64 bounds of the coordinates
   float ext_mag;            // |extent| in FP32 (BoundsMagnitude)
+  uint32_t bytes32;         // bytes from field to the end of its array when below 2^32, else 0
 };
 
 // Transmittance (opacity) map over the block bounds.

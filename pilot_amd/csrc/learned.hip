@@ -517,7 +517,7 @@ hipError_t launch_bricks_learned_map(const LearnedParams& p, const BrickFrame& f
 hipError_t launch_bricks_learned_check(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
                                        float* err, hipStream_t st)
 {
-  const size_t lds = sizeof(float) * (size_t)(p.s.ntf + p.g.n[0] + p.g.n[1] + p.g.n[2]);
+  const size_t lds = sizeof(float) * (size_t)(p.s.ntf + 2 * (p.g.n[0] + p.g.n[1] + p.g.n[2]));
   BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(map_dtype, {
     hipLaunchKernelGGL((k_bricks_learned_check<FDT, MDT>), dim3((unsigned)f.nbricks), dim3(kBlock), lds,
                        st, p, f, err);
@@ -528,7 +528,7 @@ hipError_t launch_bricks_learned_check(const LearnedParams& p, const BrickFrame&
 hipError_t launch_bricks_exact_map(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
                                    const int32_t* redo, hipStream_t st)
 {
-  const size_t lds = sizeof(float) * (size_t)(p.s.ntf + p.g.n[0] + p.g.n[1] + p.g.n[2]);
+  const size_t lds = sizeof(float) * (size_t)(p.s.ntf + 2 * (p.g.n[0] + p.g.n[1] + p.g.n[2]));
   const dim3 grid((unsigned)std::max(1, std::min(f.n_map_items, f.persistent_groups)));
   BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(map_dtype, {
     hipLaunchKernelGGL((k_bricks_exact_map<FDT, MDT>), grid, dim3(kBlock), lds, st, p, f, redo);
@@ -579,7 +579,7 @@ hipError_t launch_learned_map(const LearnedParams& p, hipStream_t st)
 hipError_t launch_learned_check(const LearnedParams& p, hipStream_t st)
 {
   const int64_t nc = learned_check_count(p.m, p.check_stride);
-  const size_t lds = sizeof(float) * (size_t)(p.s.ntf + p.g.n[0] + p.g.n[1] + p.g.n[2]);
+  const size_t lds = sizeof(float) * (size_t)(p.s.ntf + 2 * (p.g.n[0] + p.g.n[1] + p.g.n[2]));
   BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(p.m.dtype, {
     hipLaunchKernelGGL((k_learned_check<FDT, MDT>), dim3((unsigned)((nc + kBlock - 1) / kBlock)),
                        dim3(kBlock), lds, st, p, nc);

@@ -101,7 +101,8 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
-size_t lds_bytes_coords(const GridParams& g) { return sizeof(float) * (g.n[0] + g.n[1] + g.n[2]); }
+// coordinates and reciprocal cell widths (stage_coords)
+size_t lds_bytes_coords(const GridParams& g) { return 2 * sizeof(float) * (g.n[0] + g.n[1] + g.n[2]); }
 }  // namespace
 
 // ---- launchers -----------------------------------------------------------------------------

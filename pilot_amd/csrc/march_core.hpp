@@ -42,6 +42,71 @@ template <> __device__ __forceinline__ void store_t<kF16>(void* p, int64_t i, fl
   ((uint16_t*)p)[i] = f32_to_f16(v);
 }
 
+// The 8 corner scalars of a field cell (hex order 0:(i,j,k) 1:(i+1,j,k) 2:(i+1,j+1,k)
+// 3:(i,j+1,k), 4-7 the same at k+1).  A field view of fewer than 2^32 bytes is read through a
+// raw buffer resource: the cell's byte offset is the only per-lane address and the corners'
+// offsets from it are uniform scalar operands; larger views take 64-bit addresses.  Every
+// lane of a wave reads the same view (the kernels march one grid, or one brick per
+// workgroup), which the constructor states to the compiler so that the resource stays in
+// scalar registers.
+template <int FDT>
+struct CornerFetch {
+  __amdgpu_buffer_rsrc_t rsrc;
+  int so[8];  // corner byte offsets
+  int esz;
+  const void* field;
+  int sx, sy, sz;
+  bool w32;
+  __device__ __forceinline__ explicit CornerFetch(const void* f, uint32_t bytes32, int sx_, int sy_, int sz_)
+  {
+    const uint64_t fp = (uint64_t)f;
+    const uint32_t plo = __builtin_amdgcn_readfirstlane((uint32_t)fp);
+    const uint32_t phi = __builtin_amdgcn_readfirstlane((uint32_t)(fp >> 32));
+    field = (const void*)(((uint64_t)phi << 32) | plo);
+    bytes32 = __builtin_amdgcn_readfirstlane(bytes32);
+    sx = __builtin_amdgcn_readfirstlane(sx_);
+    sy = __builtin_amdgcn_readfirstlane(sy_);
+    sz = __builtin_amdgcn_readfirstlane(sz_);
+    w32 = bytes32 != 0;
+    esz = FDT == kF32 ? 4 : 2;
+    rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(field), (short)0, (int)bytes32, 0x00020000);
+    // element offsets below 2^31, byte offsets below 2^32 when w32 (unsigned products)
+    const uint32_t e = (uint32_t)esz;
+    so[0] = 0;
+    so[1] = (int)((uint32_t)sx * e);
+    so[2] = (int)((uint32_t)(sx + sy) * e);
+    so[3] = (int)((uint32_t)sy * e);
+    so[4] = (int)((uint32_t)sz * e);
+    so[5] = (int)((uint32_t)(sz + sx) * e);
+    so[6] = (int)((uint32_t)(sz + sx + sy) * e);
+    so[7] = (int)((uint32_t)(sz + sy) * e);
+  }
+  __device__ __forceinline__ float buf(int voff, int soff) const
+  {
+    if (FDT == kF32) return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, soff, 0));
+    const uint16_t h = __builtin_amdgcn_raw_buffer_load_b16(rsrc, voff, soff, 0);
+    return FDT == kF16 ? f16_to_f32(h) : bf16_to_f32(h);
+  }
+  __device__ __forceinline__ void load(int ci, int cj, int ck, float* v) const
+  {
+    if (w32) {
+      const int off = (int)((uint32_t)(ci * sx + cj * sy + ck * sz) * (uint32_t)esz);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = buf(off, so[q]);
+    } else {
+      const int64_t b = (int64_t)ci * sx + (int64_t)cj * sy + (int64_t)ck * sz;
+      v[0] = load_t<FDT>(field, b);
+      v[1] = load_t<FDT>(field, b + sx);
+      v[2] = load_t<FDT>(field, b + sx + sy);
+      v[3] = load_t<FDT>(field, b + sy);
+      v[4] = load_t<FDT>(field, b + sz);
+      v[5] = load_t<FDT>(field, b + sz + sx);
+      v[6] = load_t<FDT>(field, b + sz + sx + sy);
+      v[7] = load_t<FDT>(field, b + sz + sy);
+    }
+  }
+};
+
 __device__ __forceinline__ float lerp_lcl(float a, float b, float t)
 {
   return __builtin_fmaf(t, b, __builtin_fmaf(-t, a, a));
@@ -121,6 +186,70 @@ __device__ __forceinline__ int oracle_cell32(const float* x, int ncell, float v,
   return c;
 }
 
+// a / b from y = 1 / b rounded to FP32: q = a y corrected by one residual step,
+// q + (a - q b) y with fused products (Markstein's theorem: y within half an ulp of 1 / b and
+// q within one ulp of a / b make the corrected q the correctly rounded quotient).  The
+// theorem needs y normal and no intermediate below the normal range: widths b lie in
+// [2^-125, 2^125] (set_field keeps field cells in [2^-60, 2^61]; map cells split the field's
+// extent at most 4096 ways), the quotients here are parametric coordinates near [0, 1], and
+// numerators or first quotients below 2^-100 in magnitude (zeros among them) divide.
+__device__ __forceinline__ float div_by_recip(float a, float b, float y)
+{
+  const float q = a * y;
+  if (__builtin_expect(__builtin_fabsf(a) >= 0x1p-100f && __builtin_fabsf(q) >= 0x1p-100f, 1))
+    return __builtin_fmaf(__builtin_fmaf(-q, b, a), y, q);
+  return a / b;
+}
+
+// One axis's cell of a point, with its bounding coordinates and reciprocal width
+// (x[rofs + c]): axis_guess reads the uniform-spacing guess's cell, oracle_settle /
+// walk_settle keep it when the point lies in it (a walk from there would not move) and walk
+// otherwise.  The three axes' guesses are read before any is checked.
+struct AxisCell {
+  int c;
+  float lo, hi, rw;
+};
+__device__ __forceinline__ AxisCell axis_guess(const float* x, int rofs, int ncell, float v,
+                                               float x0, float ginv)
+{
+  AxisCell a;
+  a.c = (int)fminf(fmaxf((v - x0) * ginv, 0.0f), (float)(ncell - 1));
+  a.lo = x[a.c];
+  a.hi = x[a.c + 1];
+  a.rw = x[rofs + a.c];
+  return a;
+}
+
+// oracle_cell32's cell: v in (x[c], x[c+1]] (lower cell on an interior plane).
+__device__ __forceinline__ bool oracle_in(const AxisCell& a, float v) { return v > a.lo && v <= a.hi; }
+__device__ __forceinline__ void oracle_settle(const float* x, int rofs, int ncell, float v, AxisCell& a)
+{
+  int c = a.c;
+  while (c > 0 && v <= x[c]) --c;
+  while (c < ncell - 1 && v > x[c + 1]) ++c;
+  a.c = c;
+  a.lo = x[c];
+  a.hi = x[c + 1];
+  a.rw = x[rofs + c];
+}
+
+// walk_cell32's cell (the camera march): v in [x[c], x[c+1]).
+__device__ __forceinline__ AxisCell walk_axis(const float* x, int rofs, int ncell, float v,
+                                              float x0, float ginv)
+{
+  AxisCell a = axis_guess(x, rofs, ncell, v, x0, ginv);
+  if (__builtin_expect(!(v >= a.lo && v < a.hi), 0)) {
+    int c = a.c;
+    while (c > 0 && v < x[c]) --c;
+    while (c < ncell - 1 && v >= x[c + 1]) ++c;
+    a.c = c;
+    a.lo = x[c];
+    a.hi = x[c + 1];
+    a.rw = x[rofs + c];
+  }
+  return a;
+}
+
 // LightedVolumeRenderer.cxx:123-164: unique c with x[c] <= v < x[c+1] (v != max).
 __device__ __forceinline__ int64_t walk_cell(const float* x, int64_t ncell, float v, float x0,
                                              float ginv)
@@ -138,6 +267,7 @@ __device__ __forceinline__ int64_t walk_cell(const float* x, int64_t ncell, floa
 // map_opacity (the values are the same loads; the interpolation is recomputed each time).
 struct MapCellCache {
   int c[3] = { -1, -1, -1 };
+  float lo[3], w[3], rw[3];  // the cell's lower coordinates, widths and reciprocal widths
   float v[8];
 };
 
@@ -152,11 +282,16 @@ __device__ __forceinline__ float map_opacity_cached(const MapParams& m, const fl
     if (t < 0.0f) t = 0.0f;
     if (t >= (float)(m.mp[d] - 1)) t = (float)(m.mp[d] - 2);
     cell[d] = (int)t;
-    float lo = m.morg[d] + m.msp[d] * (float)cell[d];
-    float hi = m.morg[d] + m.msp[d] * (float)(cell[d] + 1);
-    pc[d] = (p[d] - lo) / (hi - lo);
   }
   if (cell[0] != mc.c[0] || cell[1] != mc.c[1] || cell[2] != mc.c[2]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float lo = m.morg[d] + m.msp[d] * (float)cell[d];
+      const float hi = m.morg[d] + m.msp[d] * (float)(cell[d] + 1);
+      mc.lo[d] = lo;
+      mc.w[d] = hi - lo;
+      mc.rw[d] = 1.0f / mc.w[d];
+    }
     const int64_t s0 = m.ms[0], sx = m.ms[1], sxy = m.ms[2];
     const int64_t i0 = cell[2] * sxy + cell[1] * sx + cell[0] * s0;
     mc.v[0] = load_t<MDT>(m.map, i0);
@@ -171,6 +306,8 @@ __device__ __forceinline__ float map_opacity_cached(const MapParams& m, const fl
     mc.c[1] = cell[1];
     mc.c[2] = cell[2];
   }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) pc[d] = div_by_recip(p[d] - mc.lo[d], mc.w[d], mc.rw[d]);
   return interp_hex(mc.v, pc);
 }
 

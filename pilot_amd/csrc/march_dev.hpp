@@ -6,21 +6,29 @@
 
 namespace beams {
 
-// Load the block's rectilinear coordinates into LDS (xs[0..n0+n1+n2)).
+// Load the block's rectilinear coordinates into LDS, xs[0..nc) with nc = n0 + n1 + n2,
+// followed by the reciprocal cell widths, xs[nc..2 nc) (coords_rofs(g) past each coordinate).
 __device__ __forceinline__ void stage_coords(const GridParams& g, float* xs)
 {
-  const int64_t n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-  for (int64_t i = threadIdx.x; i < n0 + n1 + n2; i += blockDim.x) {
-    float v;
-    if (i < n0)
+  const int64_t n0 = g.n[0], n1 = g.n[1], n2 = g.n[2], nc = n0 + n1 + n2;
+  for (int64_t i = threadIdx.x; i < nc; i += blockDim.x) {
+    float v, w;
+    if (i < n0) {
       v = g.x[0][i];
-    else if (i < n0 + n1)
+      w = g.r[0][i];
+    } else if (i < n0 + n1) {
       v = g.x[1][i - n0];
-    else
+      w = g.r[1][i - n0];
+    } else {
       v = g.x[2][i - n0 - n1];
+      w = g.r[2][i - n0 - n1];
+    }
     xs[i] = v;
+    xs[nc + i] = w;
   }
 }
+
+__device__ __forceinline__ int coords_rofs(const GridParams& g) { return (int)(g.n[0] + g.n[1] + g.n[2]); }
 
 // ---- TransmittanceMap.h:279-331: one map vertex's light march ----------------------------
 // a_tab: alpha / max alpha per TF entry; x: the block's coordinates (LDS).  Returns the
@@ -52,6 +60,8 @@ __device__ __forceinline__ float march_vertex(const GridParams& g, const MapPara
   // 32-bit cells and strides: set_field keeps a field's xy plane below 2^31 points
   const int c0 = (int)(g.n[0] - 1), c1 = (int)(g.n[1] - 1), c2 = (int)(g.n[2] - 1);
   const int sx = (int)g.field_stride[0], sy = (int)g.field_stride[1], sz = (int)g.field_stride[2];
+  const int rofs = coords_rofs(g);
+  const CornerFetch<FDT> corners(g.field, g.bytes32, sx, sy, sz);
   float op = seed_value;
   int64_t taken = 0;
   for (int64_t it = 0; it < kMapMaxSteps; ++it) {
@@ -62,23 +72,22 @@ __device__ __forceinline__ float march_vertex(const GridParams& g, const MapPara
     p[2] = p[2] + m.step * dir[2];
     if (!inside(g, p)) break;
     ++taken;
-    const int ci = oracle_cell32(x[0], c0, p[0], g.gx0[0], g.ginv[0]);
-    const int cj = oracle_cell32(x[1], c1, p[1], g.gx0[1], g.ginv[1]);
-    const int ck = oracle_cell32(x[2], c2, p[2], g.gx0[2], g.ginv[2]);
+    // FindCell and the parametric coordinates (p - x[c]) / (x[c+1] - x[c])
+    AxisCell ax = axis_guess(x[0], rofs, c0, p[0], g.gx0[0], g.ginv[0]);
+    AxisCell ay = axis_guess(x[1], rofs, c1, p[1], g.gx0[1], g.ginv[1]);
+    AxisCell az = axis_guess(x[2], rofs, c2, p[2], g.gx0[2], g.ginv[2]);
+    const bool settled = oracle_in(ax, p[0]) & oracle_in(ay, p[1]) & oracle_in(az, p[2]);
+    if (__builtin_expect(!settled, 0)) {
+      oracle_settle(x[0], rofs, c0, p[0], ax);
+      oracle_settle(x[1], rofs, c1, p[1], ay);
+      oracle_settle(x[2], rofs, c2, p[2], az);
+    }
     float pc[3];
-    pc[0] = (p[0] - x[0][ci]) / (x[0][ci + 1] - x[0][ci]);
-    pc[1] = (p[1] - x[1][cj]) / (x[1][cj + 1] - x[1][cj]);
-    pc[2] = (p[2] - x[2][ck]) / (x[2][ck + 1] - x[2][ck]);
-    const int64_t b = (int64_t)ci * sx + (int64_t)cj * sy + (int64_t)ck * sz;
+    pc[0] = div_by_recip(p[0] - ax.lo, ax.hi - ax.lo, ax.rw);
+    pc[1] = div_by_recip(p[1] - ay.lo, ay.hi - ay.lo, ay.rw);
+    pc[2] = div_by_recip(p[2] - az.lo, az.hi - az.lo, az.rw);
     float val[8];
-    val[0] = load_t<FDT>(g.field, b);
-    val[1] = load_t<FDT>(g.field, b + sx);
-    val[2] = load_t<FDT>(g.field, b + sx + sy);
-    val[3] = load_t<FDT>(g.field, b + sy);
-    val[4] = load_t<FDT>(g.field, b + sz);
-    val[5] = load_t<FDT>(g.field, b + sz + sx);
-    val[6] = load_t<FDT>(g.field, b + sz + sx + sy);
-    val[7] = load_t<FDT>(g.field, b + sz + sy);
+    corners.load(ax.c, ay.c, az.c, val);
     float sc = interp_hex(val, pc);
     sc = (sc - s.min_scalar) * s.inv_delta;
     op = op + (1.0f - op) * a_tab[tf_clamped_index(sc * tfmax, tfmax)];
@@ -118,6 +127,8 @@ __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapPa
   const int cms = s.ntf - 1;
   // 32-bit cells and strides: set_field keeps a field's xy plane below 2^31 points
   const int sx = (int)g.field_stride[0], sy = (int)g.field_stride[1], sz = (int)g.field_stride[2];
+  const int rofs = coords_rofs(g);
+  const CornerFetch<FDT> corners(g.field, g.bytes32, sx, sy, sz);
   const float sd = s.sample_dist;
   MapCellCache mcache;
   while (inside(g, p) && dist < maxD && ++steps < s.max_steps) {
@@ -129,24 +140,23 @@ __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapPa
       for (int d = 0; d < 3; ++d) {
         if (p[d] == g.maxp[d]) {
           cell[d] = (int)g.n[d] - 2;  // invSpacing keeps its previous value
+          bl[d] = x[d][cell[d]];
         } else {
-          const int c = walk_cell32(x[d], (int)g.n[d] - 1, p[d], g.gx0[d], g.ginv[d]);
-          invsp[d] = 1.0f / (x[d][c + 1] - x[d][c]);
-          cell[d] = c;
+          // the cell and 1 / (x[c+1] - x[c]) (the staged IEEE quotient)
+          const AxisCell a = walk_axis(x[d], rofs, (int)g.n[d] - 1, p[d], g.gx0[d], g.ginv[d]);
+          invsp[d] = a.rw;
+          cell[d] = a.c;
+          bl[d] = a.lo;
         }
       }
-      const int64_t b = (int64_t)cell[0] * sx + (int64_t)cell[1] * sy + (int64_t)cell[2] * sz;
-      bl[0] = x[0][cell[0]];
-      bl[1] = x[1][cell[1]];
-      bl[2] = x[2][cell[2]];
-      s0 = load_t<FDT>(g.field, b);
-      const float s1 = load_t<FDT>(g.field, b + sx);
-      const float s2 = load_t<FDT>(g.field, b + sx + sy);
-      s3 = load_t<FDT>(g.field, b + sy);
-      s4 = load_t<FDT>(g.field, b + sz);
-      const float s5 = load_t<FDT>(g.field, b + sz + sx);
-      const float s6 = load_t<FDT>(g.field, b + sz + sx + sy);
-      s7 = load_t<FDT>(g.field, b + sz + sy);
+      float cv[8];
+      corners.load(cell[0], cell[1], cell[2], cv);
+      s0 = cv[0];
+      const float s1 = cv[1], s2 = cv[2];
+      s3 = cv[3];
+      s4 = cv[4];
+      const float s5 = cv[5], s6 = cv[6];
+      s7 = cv[7];
       s67 = s6 - s7;
       s54 = s5 - s4;
       s10 = s1 - s0;

@@ -176,6 +176,50 @@ def test_edge_rays(oracle_mod):
     _assert_same(rays, gmap, rb, omap, "edge rays")
 
 
+@pytest.mark.parametrize("light", ["directional", "point"])
+def test_rectilinear_coordinates(oracle_mod, light):
+    """Unevenly spaced coordinates (widths varying 6x, shifted origin): the uniform-spacing
+    cell guess misses often, so the map march's and the camera march's cell walks and the
+    staged reciprocal widths of uneven cells all take part."""
+    import dataclasses
+
+    sc = scenes.make_scene("rect", 24, seed=17, nblobs=8, light=light, width=64, height=48)
+    rng = np.random.default_rng(3)
+    coords = []
+    for d, (lo, hi) in enumerate([(-0.3, 1.1), (0.0, 1.0), (0.2, 1.7)]):
+        w = rng.uniform(0.3, 1.8, size=sc.dims[d] - 1)
+        x = lo + (hi - lo) * np.concatenate([[0.0], np.cumsum(w) / w.sum()])
+        coords.append(x.astype(np.float32))
+    sc = dataclasses.replace(sc, coords=coords)
+    o, d = B.camera_rays(B.orbit_camera(sc.bounds, 35.0, 25.0, 50.0), sc.width, sc.height)
+    rays, gmap, rb, omap = _render_both(oracle_mod, sc, o, d)
+    _assert_same(rays, gmap, rb, omap, f"rectilinear {light}")
+    assert rays.rgba[:, 3].max() > 0.1
+
+
+def test_set_field_rejects_grids_outside_the_kernels_domain():
+    """Coordinates beyond 2^60, cells narrower than 2^-60 (the reciprocal-width division's
+    range) and coordinate tables too large for LDS are refused with BEAMS_ERR_INVALID /
+    BEAMS_ERR_UNSUPPORTED; a grid at the limits is accepted."""
+    n = 4
+    vals = np.zeros(n ** 3, np.float32)
+    ok = [np.linspace(0, 1, n, dtype=np.float32)] * 3
+    big = [ok[0], ok[1], np.float32([0, 1, 2, 2.0 ** 61])]
+    thin = [ok[0], ok[1], np.float32([0, 2.0 ** -62, 0.5, 1])]
+    with B.LightedVolumeRenderer() as r:
+        r.SetData(vals, (n, n, n), coords=ok)
+        for coords, status in ((big, 1), (thin, 1)):
+            with pytest.raises(B.BeamsError) as ei:
+                r.SetData(vals, (n, n, n), coords=coords)
+            assert ei.value.status == status
+        with pytest.raises(B.BeamsError) as ei:
+            r.SetData(np.zeros(2 * 2 * 16383, np.float32), (2, 2, 16383), origin=(0, 0, 0),
+                      spacing=(1, 1, 1))
+        assert ei.value.status == 2
+        r.SetData(np.zeros(2 * 2 * 16380, np.float32), (2, 2, 16380), origin=(0, 0, 0),
+                  spacing=(1, 1, 1))
+
+
 def test_device_generator_matches_host():
     from pilot_amd.devmem import DeviceArray
 
