@@ -137,11 +137,21 @@ constexpr int kMaxCandidates = 1024;
 // One light segment (o -> p) against brick j: the conservative FP32 slab cull on the brick's
 // inflated box, then the exact test (FP64 slab terms); key = (exit distance bits, brick).
 __device__ __forceinline__ bool exchange_hit(const BrickFrame& f, int j, int b, const float* o,
-                                             const float* p, const float* dd, const float* inv,
+                                             const Segment& sg, const float* dd, const float* inv,
                                              unsigned long long& key)
 {
   if (j == b) return false;
-  const float* bx = f.bricks[j].box32;
+  // j is the same for every lane and the brick table is read-only during the frame: reading
+  // it through the constant address space makes these scalar loads, issued together
+  typedef const __attribute__((address_space(4))) BrickDesc ConstBrick;
+  ConstBrick* bk = (ConstBrick*)(const void*)(f.bricks + j);
+  float bx[6];
+  double bb[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    bx[k] = bk->box32[k];
+    bb[k] = bk->g.bounds[k];
+  }
   float t0 = 0.0f, t1 = 1.0f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -156,7 +166,7 @@ __device__ __forceinline__ bool exchange_hit(const BrickFrame& f, int j, int b, 
   }
   if (!(t0 <= t1 + 1e-3f)) return false;
   float tmin, tmax;
-  if (!segment_aabb(o, p, f.bricks[j].g.bounds, (float)b, tmin, tmax)) return false;
+  if (!segment_box(sg, bb, (float)b, tmin, tmax)) return false;
   // tmax >= tmin >= 0; adding +0 turns a -0 into +0 so the bit order is the float order
   key = ((unsigned long long)f32_bits(tmax + 0.0f) << 32) | (unsigned long long)(unsigned)j;
   return true;
@@ -213,6 +223,8 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
       const float rs = 1.0f / __builtin_sqrtf(dot3(dd, dd));  // vtkm::Normalize
       const float dir[3] = { dd[0] * rs, dd[1] * rs, dd[2] * rs };
       const float inv[3] = { 1.0f / dd[0], 1.0f / dd[1], 1.0f / dd[2] };
+      Segment sg;  // SegmentAABB's per-segment terms, shared by every candidate brick
+      segment_prepare(o, p, sg);
       auto composite = [&](float op, unsigned long long key) {
         const int j = (int)(key & 0xffffffffull);
         const float t = bits_f32((uint32_t)(key >> 32));
@@ -227,7 +239,9 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
       bool spill = false;
       for (int q = 0; q < nloop; ++q) {
         unsigned long long key;
-        if (!exchange_hit(f, all ? q : cand[q], b, o, p, dd, inv, key)) continue;
+        // the same candidate for every lane of the wave
+        const int j = __builtin_amdgcn_readfirstlane(all ? q : cand[q]);
+        if (!exchange_hit(f, j, b, o, sg, dd, inv, key)) continue;
         if (nh == kRegHits) {
           spill = true;
           break;
@@ -260,7 +274,8 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
           unsigned long long best = ~0ull;
           for (int q = 0; q < nloop; ++q) {
             unsigned long long key;
-            if (!exchange_hit(f, all ? q : cand[q], b, o, p, dd, inv, key)) continue;
+            const int j = __builtin_amdgcn_readfirstlane(all ? q : cand[q]);
+            if (!exchange_hit(f, j, b, o, sg, dd, inv, key)) continue;
             if ((!started || key > last) && key < best) best = key;
           }
           if (best == ~0ull) break;

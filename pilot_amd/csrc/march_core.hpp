@@ -186,6 +186,16 @@ __device__ __forceinline__ int oracle_cell32(const float* x, int ncell, float v,
   return c;
 }
 
+// a / d in FP64 from y = 1 / d rounded to FP64, corrected as div_by_recip (the same theorem
+// at 53 bits); zero or tiny numerators and quotients, and d = 0 (fast false), divide.
+__device__ __forceinline__ double div64_by_recip(double a, double d, double y, bool fast)
+{
+  const double q = a * y;
+  if (__builtin_expect(fast && __builtin_fabs(a) >= 0x1p-900 && __builtin_fabs(q) >= 0x1p-900, 1))
+    return __builtin_fma(__builtin_fma(-q, d, a), y, q);
+  return a / d;
+}
+
 // a / b from y = 1 / b rounded to FP32: q = a y corrected by one residual step,
 // q + (a - q b) y with fused products (Markstein's theorem: y within half an ulp of 1 / b and
 // q within one ulp of a / b make the corrected q the correctly rounded quotient).  The
@@ -199,6 +209,44 @@ __device__ __forceinline__ float div_by_recip(float a, float b, float y)
   if (__builtin_expect(__builtin_fabsf(a) >= 0x1p-100f && __builtin_fabsf(q) >= 0x1p-100f, 1))
     return __builtin_fmaf(__builtin_fmaf(-q, b, a), y, q);
   return a / b;
+}
+
+// segment_aabb for one segment against many boxes: the direction, length and the FP64
+// reciprocals of the direction are computed once (segment_prepare); segment_box then gives
+// segment_aabb's result for each box with the slab quotients from those reciprocals.
+struct Segment {
+  float s[3], d[3], len;
+  double rd[3];  // 1 / (double)d
+};
+__device__ __forceinline__ void segment_prepare(const float* s, const float* e, Segment& sg)
+{
+  float d[3] = { e[0] - s[0], e[1] - s[1], e[2] - s[2] };
+  sg.len = __builtin_sqrtf(dot3(d, d));
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    sg.s[a] = s[a];
+    sg.d[a] = d[a] / sg.len;
+    sg.rd[a] = 1.0 / (double)sg.d[a];
+  }
+}
+__device__ __forceinline__ bool segment_box(const Segment& sg, const double* bb, float teps,
+                                            float& tmin_o, float& tmax_o)
+{
+  float tmin = 0.0f, tmax = __builtin_inff();
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double da = (double)sg.d[a];
+    const bool fast = sg.d[a] != 0.0f;
+    float t0 = (float)div64_by_recip(bb[2 * a] - (double)sg.s[a], da, sg.rd[a], fast);
+    float t1 = (float)div64_by_recip(bb[2 * a + 1] - (double)sg.s[a], da, sg.rd[a], fast);
+    tmin = fmaxf(tmin, fminf(t0, t1));
+    tmax = fminf(tmax, fmaxf(t0, t1));
+  }
+  if (tmin < 0.0f) tmin = teps;
+  if (tmax > sg.len) tmax = sg.len;
+  tmin_o = tmin;
+  tmax_o = tmax;
+  return tmin <= tmax;
 }
 
 // One axis's cell of a point, with its bounding coordinates and reciprocal width
