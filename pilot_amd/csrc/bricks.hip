@@ -172,15 +172,28 @@ __device__ __forceinline__ bool exchange_hit(const BrickFrame& f, int j, int b, 
   return true;
 }
 
-// Hits kept sorted in registers per vertex; a path crossing more bricks is composited by
-// repeated selection of the next key over the candidates (same order, no storage).
+// Hits kept sorted per vertex in a lane-private LDS column (slot k of lane l at
+// hits[k][l]); a path crossing more bricks is composited by repeated selection of the next
+// key over the candidates (same order, no storage).  The item's candidates are ordered by
+// their distance along the light first, so hits mostly arrive in order and are appended.
 constexpr int kRegHits = 16;
+
+// Ordering key of a candidate brick: distance of its centre from the point light, or its
+// centre's position along the directional light's travel direction.
+__device__ __forceinline__ float cand_order(const MapParams& m, const float* bx)
+{
+  const float s[3] = { 0.5f * (bx[0] + bx[1]), 0.5f * (bx[2] + bx[3]), 0.5f * (bx[4] + bx[5]) };
+  if (m.light_kind == 1) return s[0] * m.light[0] + s[1] * m.light[1] + s[2] * m.light[2];
+  const float d[3] = { s[0] - m.light[0], s[1] - m.light[1], s[2] - m.light[2] };
+  return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
 
 template <int MDT>
 __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
 {
   __shared__ int slot;
   __shared__ int cand[kMaxCandidates];
+  __shared__ unsigned long long hits[kRegHits][64];
   const int lane = threadIdx.x;
   for (;;) {
     const int item = next_item(f.queue + 1, &slot);
@@ -210,6 +223,19 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
         ncand += __popcll(mask);
       }
       __syncthreads();
+      if (ncand <= 64) {
+        // rank sort of up to 64 candidates by cand_order (ties by list position)
+        const int j = lane < ncand ? cand[lane] : 0;
+        const float key = lane < ncand ? cand_order(bd.m, f.bricks[j].box32) : 0.0f;
+        int rank = 0;
+        for (int k = 0; k < ncand; ++k) {
+          const float kk = __shfl(key, k);
+          rank += (kk < key || (kk == key && k < lane)) ? 1 : 0;
+        }
+        __syncthreads();
+        if (lane < ncand) cand[rank] = j;
+        __syncthreads();
+      }
     }
     const bool all = ncand > kMaxCandidates;  // too many to list: test every brick
     const int nloop = all ? f.nbricks : ncand;
@@ -232,9 +258,7 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
         const float local = map_opacity<MDT>(f.bricks[j].m, q);
         return op + (1.0f - op) * local;
       };
-      unsigned long long keys[kRegHits];
-#pragma unroll
-      for (int m = 0; m < kRegHits; ++m) keys[m] = ~0ull;
+      unsigned long long top = 0;  // the largest key so far (hits[nh-1][lane])
       int nh = 0;
       bool spill = false;
       for (int q = 0; q < nloop; ++q) {
@@ -246,27 +270,24 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
           spill = true;
           break;
         }
-        // insertion by (RayT, brick): the larger keys move up one slot
-        unsigned long long carry = key;
-#pragma unroll
-        for (int m = 0; m < kRegHits; ++m) {
-          const bool swap = m < nh && keys[m] > carry;
-          const unsigned long long t = keys[m];
-          keys[m] = swap ? carry : t;
-          carry = swap ? t : carry;
+        if (nh == 0 || key > top) {
+          hits[nh][lane] = key;  // in order: append
+          top = key;
+        } else {
+          // insertion: the larger keys move up one slot
+          int k = nh;
+          for (; k > 0; --k) {
+            const unsigned long long prev = hits[k - 1][lane];
+            if (prev < key) break;
+            hits[k][lane] = prev;
+          }
+          hits[k][lane] = key;
         }
-#pragma unroll
-        for (int m = 0; m < kRegHits; ++m) keys[m] = m == nh ? carry : keys[m];
         ++nh;
       }
       float op = 0.0f;
       if (!spill) {
-        for (int k = 0; k < nh; ++k) {
-          unsigned long long key = keys[0];
-#pragma unroll
-          for (int m = 1; m < kRegHits; ++m) key = k == m ? keys[m] : key;
-          op = composite(op, key);
-        }
+        for (int k = 0; k < nh; ++k) op = composite(op, hits[k][lane]);
       } else {
         bool started = false;
         unsigned long long last = 0;

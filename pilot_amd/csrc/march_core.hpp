@@ -199,14 +199,15 @@ __device__ __forceinline__ double div64_by_recip(double a, double d, double y, b
 // a / b from y = 1 / b rounded to FP32: q = a y corrected by one residual step,
 // q + (a - q b) y with fused products (Markstein's theorem: y within half an ulp of 1 / b and
 // q within one ulp of a / b make the corrected q the correctly rounded quotient).  The
-// theorem needs y normal and no intermediate below the normal range: widths b lie in
-// [2^-125, 2^125] (set_field keeps field cells in [2^-60, 2^61]; map cells split the field's
+// theorem needs y normal and the residual a - q b representable: widths b lie in
+// [2^-72, 2^61] (set_field keeps field cells in [2^-60, 2^61]; map cells split the field's
 // extent at most 4096 ways), the quotients here are parametric coordinates near [0, 1], and
-// numerators or first quotients below 2^-100 in magnitude (zeros among them) divide.
+// a first quotient of at least 2^-28 then puts |a| above 2^-100, where the residual's
+// granularity is above the subnormal step.  Smaller quotients (zeros among them) divide.
 __device__ __forceinline__ float div_by_recip(float a, float b, float y)
 {
   const float q = a * y;
-  if (__builtin_expect(__builtin_fabsf(a) >= 0x1p-100f && __builtin_fabsf(q) >= 0x1p-100f, 1))
+  if (__builtin_expect(__builtin_fabsf(q) >= 0x1p-28f, 1))
     return __builtin_fmaf(__builtin_fmaf(-q, b, a), y, q);
   return a / b;
 }
@@ -261,7 +262,8 @@ __device__ __forceinline__ AxisCell axis_guess(const float* x, int rofs, int nce
                                                float x0, float ginv)
 {
   AxisCell a;
-  a.c = (int)fminf(fmaxf((v - x0) * ginv, 0.0f), (float)(ncell - 1));
+  // clamp (med3; v is finite inside the bounds, so the guess is never NaN)
+  a.c = (int)__builtin_amdgcn_fmed3f((v - x0) * ginv, 0.0f, (float)(ncell - 1));
   a.lo = x[a.c];
   a.hi = x[a.c + 1];
   a.rw = x[rofs + a.c];
@@ -326,9 +328,10 @@ __device__ __forceinline__ float map_opacity_cached(const MapParams& m, const fl
   int cell[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
+    const int np = (int)m.mp[d];
     float t = (p[d] - m.morg[d]) * m.minv[d];
     if (t < 0.0f) t = 0.0f;
-    if (t >= (float)(m.mp[d] - 1)) t = (float)(m.mp[d] - 2);
+    if (t >= (float)(np - 1)) t = (float)(np - 2);
     cell[d] = (int)t;
   }
   if (cell[0] != mc.c[0] || cell[1] != mc.c[1] || cell[2] != mc.c[2]) {
@@ -380,20 +383,23 @@ __device__ __forceinline__ int tf_clamped_index(float t, float tfmax)
 template <int MDT>
 __device__ __forceinline__ float map_opacity(const MapParams& m, const float* p)
 {
+  // 32-bit cells and conversions (a map axis has at most 4097 points; the same float values
+  // as 64-bit integers would give)
   float pc[3];
-  int64_t cell[3];
+  int cell[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
+    const int np = (int)m.mp[d];
     float t = (p[d] - m.morg[d]) * m.minv[d];
     if (t < 0.0f) t = 0.0f;
-    if (t >= (float)(m.mp[d] - 1)) t = (float)(m.mp[d] - 2);
-    cell[d] = (int64_t)t;
+    if (t >= (float)(np - 1)) t = (float)(np - 2);
+    cell[d] = (int)t;
     float lo = m.morg[d] + m.msp[d] * (float)cell[d];
     float hi = m.morg[d] + m.msp[d] * (float)(cell[d] + 1);
     pc[d] = (p[d] - lo) / (hi - lo);
   }
   const int64_t s0 = m.ms[0], sx = m.ms[1], sxy = m.ms[2];
-  const int64_t i0 = cell[2] * sxy + cell[1] * sx + cell[0] * s0;
+  const int64_t i0 = (int64_t)cell[2] * sxy + (int64_t)cell[1] * sx + (int64_t)cell[0] * s0;
   float v[8];
   v[0] = load_t<MDT>(m.map, i0);
   v[1] = load_t<MDT>(m.map, i0 + s0);
