@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--learned-samples", type=int, default=4 << 20)
     ap.add_argument("--learned-epochs", type=int, default=12)
     ap.add_argument("--learned-eval-seeds", type=int, default=4, help="held-out fields evaluated (1..4)")
+    ap.add_argument("--series-timesteps", type=int, default=8, help="timesteps of the time series (1..8)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device 5 times (the byte-counter calibration run)")
     return ap.parse_args()
@@ -535,7 +536,8 @@ def main():
         wl["training"] = train
         line.setdefault("workloads", {})["learned"] = wl
         if not args.no_series:
-            line["workloads"]["time_series"] = time_series_workload(2, 1, net)
+            line["workloads"]["time_series"] = time_series_workload(
+                2, 1, net, timesteps=max(1, min(8, args.series_timesteps)))
     if not args.no_precision:
         line["precision"] = precision_report(n, img, threads, net)
     if net is not None:

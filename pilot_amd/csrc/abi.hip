@@ -767,7 +767,7 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
   HIP_TRY(h, h->bmap_local.ensure((size_t)nv_total * msz));
   HIP_TRY(h, h->bmap_final.ensure((size_t)nv_total * msz));
   HIP_TRY(h, h->bseeds.ensure((size_t)nv_total * sizeof(float)));
-  HIP_TRY(h, h->bqueue.ensure(8 * sizeof(unsigned int)));
+  HIP_TRY(h, h->bqueue.ensure(4 * kQueueSlots * sizeof(unsigned int)));
   if ((rc = frame_begin(h))) return rc;
   BrickFrame f;
   int64_t total_verts = 0;
@@ -791,7 +791,7 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
   auto issue = [&](bool timed) -> int {
     int r;
     auto mark = [&](int p, bool end) { return timed ? phase_mark(h, p, end) : BEAMS_OK; };
-    HIP_TRY(h, launch_zero_u32(f.queue, 5, h->stream));
+    HIP_TRY(h, launch_zero_u32(f.queue, 4 * kQueueSlots, h->stream));
     if (shadows && !learned) {
       if ((r = mark(BEAMS_PHASE_MAP, false))) return r;
       HIP_TRY(h, launch_bricks_map(f, h->dtype, h->map_dtype, ntf, false, h->stream));

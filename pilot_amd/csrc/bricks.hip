@@ -18,13 +18,44 @@ namespace beams {
 
 namespace {
 
-// Fetch the next work item of a persistent workgroup (uniform across the group).
-__device__ __forceinline__ int next_item(unsigned int* counter, int* slot)
+// Work queue of a persistent workgroup.  Items are handed out in chunks of consecutive items
+// from eight counters (one per XCD: workgroups are dealt to the XCDs round-robin), the eight
+// counters' chunks interleaved: one device-scope atomic per chunk, spread over eight
+// addresses, instead of one per item on a single address (whose serialisation bounded the
+// phase's time).
+struct ItemQueue {
+  int next, end, item;
+};
+
+__device__ __forceinline__ void queue_init(ItemQueue* q)
+{
+  if (threadIdx.x == 0) q->next = q->end = 0;
+}
+
+// The next item of the workgroup (uniform across the group), nitems when the work is done.
+__device__ __forceinline__ int next_item(unsigned int* counters, ItemQueue* q, int nitems, int chunk)
 {
   __syncthreads();  // every thread is done with the previous item and its LDS
-  if (threadIdx.x == 0) *slot = (int)atomicAdd(counter, 1u);
+  if (threadIdx.x == 0) {
+    if (q->next >= q->end) {
+      const unsigned x = blockIdx.x & 7u;
+      const int64_t k = (int64_t)atomicAdd(&counters[x], 1u);
+      const int64_t first = (k * 8 + x) * chunk;
+      q->next = first < nitems ? (int)first : nitems;
+      q->end = first + chunk < nitems ? (int)(first + chunk) : nitems;
+    }
+    q->item = q->next < q->end ? q->next++ : nitems;
+  }
   __syncthreads();
-  return *slot;
+  return q->item;
+}
+
+// Chunk sizes: about sixteen fetches per workgroup for the map phases (hundreds of thousands
+// of items), single items for the march's few thousand tiles.
+__device__ __forceinline__ int map_chunk(const BrickFrame& f, int groups)
+{
+  const int c = f.n_map_items / (groups * 16);
+  return c < 1 ? 1 : (c > 16 ? 16 : c);
 }
 
 
@@ -33,14 +64,16 @@ template <int FDT, int MDT>
 __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
 {
   extern __shared__ float lds[];
-  __shared__ int slot;
+  __shared__ ItemQueue queue;
   const int ntf = f.bricks[0].s.ntf;
   float* a_tab = lds;
   float* xs = lds + ntf;
-  unsigned int* counter = f.queue + (seeded ? 2 : 0);
+  unsigned int* counters = f.queue + (seeded ? 2 : 0) * kQueueSlots;
+  const int chunk = map_chunk(f, (int)gridDim.x);
+  queue_init(&queue);
   int staged = -1;
   for (;;) {
-    const int item = next_item(counter, &slot);
+    const int item = next_item(counters, &queue, f.n_map_items, chunk);
     if (item >= f.n_map_items) break;
     const int b = f.map_item_brick[item];
     const BrickDesc& bd = f.bricks[b];
@@ -191,12 +224,14 @@ __device__ __forceinline__ float cand_order(const MapParams& m, const float* bx)
 template <int MDT>
 __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
 {
-  __shared__ int slot;
+  __shared__ ItemQueue queue;
   __shared__ int cand[kMaxCandidates];
   __shared__ unsigned long long hits[kRegHits][64];
   const int lane = threadIdx.x;
+  const int chunk = map_chunk(f, (int)gridDim.x);
+  queue_init(&queue);
   for (;;) {
-    const int item = next_item(f.queue + 1, &slot);
+    const int item = next_item(f.queue + 1 * kQueueSlots, &queue, f.n_map_items, chunk);
     if (item >= f.n_map_items) break;
     const int b = f.map_item_brick[item];
     const BrickDesc& bd = f.bricks[b];
@@ -315,13 +350,14 @@ template <int FDT, int MDT, bool USE_MAP>
 __global__ void __launch_bounds__(256) k_bricks_march(BrickFrame f, CameraParams cam)
 {
   extern __shared__ float4 lds4[];
-  __shared__ int slot;
+  __shared__ ItemQueue queue;
   const int ntf = f.bricks[0].s.ntf;
   float4* tf = lds4;
   float* xs = (float*)(lds4 + ntf);
+  queue_init(&queue);
   int staged = -1;
   for (;;) {
-    const int item = next_item(f.queue + 3, &slot);
+    const int item = next_item(f.queue + 3 * kQueueSlots, &queue, f.n_px_items, 1);
     if (item >= f.n_px_items) break;
     const int b = f.px_item_brick[item];
     const BrickDesc& bd = f.bricks[b];
