@@ -6,10 +6,10 @@
 //            light segment crosses (SegmentAABB), that brick's local opacity at the segment's
 //            exit, over-composited in exit-distance order -> seed opacity
 //   phase 3  k_bricks_map (seeded re-march)          persistent, all bricks
-//   phase 4  k_bricks_march (footprint tiles)        persistent, all bricks
+//   phase 4  k_bricks_march (footprint tiles)        one workgroup per (brick, tile) item
 //   phase 5  k_bricks_composite (depth-ordered over) per pixel
 // The reference routes 32-byte hit records through rank 0 with MPI; here the hits of a
-// vertex live in registers for the duration of its loop and are never written to memory.
+// vertex live in LDS for the duration of its loop and are never written to memory.
 #include "bricks.hpp"
 #include "camera.hpp"
 #include "march_dev.hpp"
@@ -345,45 +345,61 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
   }
 }
 
+// A copy of a struct that every lane reads at the same address and that no kernel writes
+// during the frame: word by word through the constant address space, i.e. scalar loads into
+// scalar registers (the words a kernel never uses are not loaded).
+template <class T>
+__device__ __forceinline__ T load_uniform(const T* p)
+{
+  static_assert(sizeof(T) % 4 == 0, "word-sized struct");
+  typedef const __attribute__((address_space(4))) uint32_t ConstWord;
+  ConstWord* q = (ConstWord*)(const void*)p;
+  T out;
+  uint32_t* o = (uint32_t*)&out;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) o[i] = q[i];
+  return out;
+}
+
 // ---- phase 4: CalcRayStart + Sampler per brick over its screen footprint ---------------------
 template <int FDT, int MDT, bool USE_MAP>
 __global__ void __launch_bounds__(256) k_bricks_march(BrickFrame f, CameraParams cam)
 {
+  // One workgroup per (brick, 16x16 screen tile) item.  Not persistent: a loop over items
+  // keeps the camera and brick parameters live across the march and doubles the registers.
   extern __shared__ float4 lds4[];
-  __shared__ ItemQueue queue;
-  const int ntf = f.bricks[0].s.ntf;
+  const int item = blockIdx.x;
   float4* tf = lds4;
-  float* xs = (float*)(lds4 + ntf);
-  queue_init(&queue);
-  int staged = -1;
-  for (;;) {
-    const int item = next_item(f.queue + 3 * kQueueSlots, &queue, f.n_px_items, 1);
-    if (item >= f.n_px_items) break;
-    const int b = f.px_item_brick[item];
-    const BrickDesc& bd = f.bricks[b];
-    if (staged < 0) stage_tf(bd.s, tf);
-    if (b != staged) stage_coords(bd.g, xs);
-    staged = b;
-    __syncthreads();
-    const float* x[3] = { xs, xs + bd.g.n[0], xs + bd.g.n[0] + bd.g.n[1] };
-    const int tiles_x = (bd.fp[2] + 15) / 16;
-    const int tile = f.px_item_tile[item];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int lx = (tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
-    const int ly = (tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
-    if (lx >= bd.fp[2] || ly >= bd.fp[3]) continue;
-    float org[3], dir[3];
-    camera_ray(cam, bd.fp[0] + lx, bd.fp[1] + ly, org, dir);
-    float minD = cam.near_d, maxD = cam.far_d, dist = 0.0f;
-    calc_ray_start(bd.g, org, dir, minD, dist, maxD);
-    float color[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
-    MapParams m = bd.m;
-    m.map = bd.map_final;
-    shade<FDT, MDT, USE_MAP>(bd.g, m, bd.s, tf, x, org, dir, minD, maxD, color);
-    const int64_t pix = bd.img_off + (int64_t)ly * bd.fp[2] + lx;
-    f.partial[pix] = make_float4(color[0], color[1], color[2], color[3]);
-    f.partial_depth[pix] = (minD == -1.0f) ? __builtin_inff() : minD;
-  }
+  float* xs = (float*)(lds4 + f.bricks[0].s.ntf);
+  // the item's brick is the same for the whole workgroup: its parameters are read with
+  // scalar loads (constant address space) into scalar registers, not into every lane
+  const int b = __builtin_amdgcn_readfirstlane(f.px_item_brick[item]);
+  const BrickDesc* bk = f.bricks + b;
+  const GridParams g = load_uniform(&bk->g);
+  const ShadeParams sp = load_uniform(&bk->s);
+  MapParams m = load_uniform(&bk->m);
+  m.map = bk->map_final;
+  const int fp0 = bk->fp[0], fp1 = bk->fp[1], fp2 = bk->fp[2], fp3 = bk->fp[3];
+  const int64_t img_off = bk->img_off;
+  stage_tf(sp, tf);
+  stage_coords(g, xs);
+  __syncthreads();
+  const float* x[3] = { xs, xs + g.n[0], xs + g.n[0] + g.n[1] };
+  const int tiles_x = (fp2 + 15) / 16;
+  const int tile = f.px_item_tile[item];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lx = (tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+  const int ly = (tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+  if (lx >= fp2 || ly >= fp3) return;
+  float org[3], dir[3];
+  camera_ray(cam, fp0 + lx, fp1 + ly, org, dir);
+  float minD = cam.near_d, maxD = cam.far_d, dist = 0.0f;
+  calc_ray_start(g, org, dir, minD, dist, maxD);
+  float color[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+  shade<FDT, MDT, USE_MAP>(g, m, sp, tf, x, org, dir, minD, maxD, color);
+  const int64_t pix = img_off + (int64_t)ly * fp2 + lx;
+  f.partial[pix] = make_float4(color[0], color[1], color[2], color[3]);
+  f.partial_depth[pix] = (minD == -1.0f) ? __builtin_inff() : minD;
 }
 
 // ---- phase 5: GlobalComposite, front-to-back over in camera-distance order -------------------
@@ -486,7 +502,7 @@ hipError_t launch_bricks_march(const BrickFrame& f, int32_t field_dtype, int32_t
 {
   if (f.n_px_items == 0) return hipSuccess;
   const size_t lds = sizeof(float4) * (size_t)ntf + sizeof(float) * 2 * (size_t)f.max_coords;
-  const dim3 grid((unsigned)f.persistent_groups);
+  const dim3 grid((unsigned)f.n_px_items);
   BRICK_DISPATCH_FIELD(field_dtype, BRICK_DISPATCH_MAP(map_dtype, {
     if (use_map)
       hipLaunchKernelGGL((k_bricks_march<FDT, MDT, true>), grid, dim3(256), lds, st, f, c);
