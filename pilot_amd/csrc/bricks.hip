@@ -59,6 +59,22 @@ __device__ __forceinline__ int map_chunk(const BrickFrame& f, int groups)
 }
 
 
+// A copy of a struct that every lane reads at the same address and that no kernel writes
+// during the frame: word by word through the constant address space, i.e. scalar loads into
+// scalar registers (the words a kernel never uses are not loaded).
+template <class T>
+__device__ __forceinline__ T load_uniform(const T* p)
+{
+  static_assert(sizeof(T) % 4 == 0, "word-sized struct");
+  typedef const __attribute__((address_space(4))) uint32_t ConstWord;
+  ConstWord* q = (ConstWord*)(const void*)p;
+  T out;
+  uint32_t* o = (uint32_t*)&out;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) o[i] = q[i];
+  return out;
+}
+
 // ---- phases 1 and 3: per-brick TransmittanceMapGenerator -------------------------------------
 template <int FDT, int MDT>
 __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
@@ -75,21 +91,25 @@ __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
   for (;;) {
     const int item = next_item(counters, &queue, f.n_map_items, chunk);
     if (item >= f.n_map_items) break;
-    const int b = f.map_item_brick[item];
-    const BrickDesc& bd = f.bricks[b];
+    // the brick's parameters in scalar registers (the same brick for the whole workgroup)
+    const int b = __builtin_amdgcn_readfirstlane(f.map_item_brick[item]);
+    const BrickDesc* bk = f.bricks + b;
+    const GridParams g = load_uniform(&bk->g);
+    const MapParams m = load_uniform(&bk->m);
+    const ShadeParams sp = load_uniform(&bk->s);
     if (staged < 0)
-      for (int i = threadIdx.x; i < ntf; i += blockDim.x) a_tab[i] = bd.s.tf[4 * i + 3] / bd.s.max_density;
-    if (b != staged) stage_coords(bd.g, xs);
+      for (int i = threadIdx.x; i < ntf; i += blockDim.x) a_tab[i] = sp.tf[4 * i + 3] / sp.max_density;
+    if (b != staged) stage_coords(g, xs);
     staged = b;
     __syncthreads();
-    const float* x[3] = { xs, xs + bd.g.n[0], xs + bd.g.n[0] + bd.g.n[1] };
+    const float* x[3] = { xs, xs + g.n[0], xs + g.n[0] + g.n[1] };
     const int64_t v = f.map_item_first[item] + threadIdx.x;
-    if (v < bd.nverts) {
+    if (v < bk->nverts) {
       int64_t taken = 0;
-      const float seed = seeded ? f.seeds[bd.seed_off + v] : 0.0f;
-      const float op = march_vertex<FDT>(bd.g, bd.m, bd.s, a_tab, x, v, seed, &taken);
-      store_t<MDT>(seeded ? bd.map_final : bd.m.map, v, op);
-      if (bd.s.counters && taken) atomicAdd(&bd.s.counters[1], (unsigned long long)taken);
+      const float seed = seeded ? f.seeds[bk->seed_off + v] : 0.0f;
+      const float op = march_vertex<FDT>(g, m, sp, a_tab, x, v, seed, &taken);
+      store_t<MDT>(seeded ? bk->map_final : m.map, v, op);
+      if (sp.counters && taken) atomicAdd(&sp.counters[1], (unsigned long long)taken);
     }
   }
 }
@@ -343,22 +363,6 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
       f.seeds[bd.seed_off + v] = op;
     }
   }
-}
-
-// A copy of a struct that every lane reads at the same address and that no kernel writes
-// during the frame: word by word through the constant address space, i.e. scalar loads into
-// scalar registers (the words a kernel never uses are not loaded).
-template <class T>
-__device__ __forceinline__ T load_uniform(const T* p)
-{
-  static_assert(sizeof(T) % 4 == 0, "word-sized struct");
-  typedef const __attribute__((address_space(4))) uint32_t ConstWord;
-  ConstWord* q = (ConstWord*)(const void*)p;
-  T out;
-  uint32_t* o = (uint32_t*)&out;
-#pragma unroll
-  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) o[i] = q[i];
-  return out;
 }
 
 // ---- phase 4: CalcRayStart + Sampler per brick over its screen footprint ---------------------
