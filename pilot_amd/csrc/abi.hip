@@ -250,6 +250,7 @@ MapParams map_params(const beams_renderer* h, const GridParams& g, int32_t mode)
     m.ms[a2] = m.mp[a1];
     m.ms[axis] = m.mp[a1] * m.mp[a2];
   }
+  m.bytes32 = map_bytes32(m.mp, m.dtype);
   m.step = std::sqrt(size[0] * size[0] + size[1] * size[1] + size[2] * size[2]) / 128.0f;
   m.dir_far = 4.0f * g.ext_mag;
   return m;
@@ -579,6 +580,7 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
     m.ms[0] = 1;
     m.ms[1] = m.mp[0];
     m.ms[2] = m.mp[0] * m.mp[1];
+    m.bytes32 = map_bytes32(m.mp, m.dtype);
     m.step = std::sqrt(size[0] * size[0] + size[1] * size[1] + size[2] * size[2]) / 128.0f;
     m.dir_far = 4.0f * whole.ext_mag;  // one light origin for all bricks
     m.map = (unsigned char*)h->bmap_local.p + msz * (size_t)br.seed_off;

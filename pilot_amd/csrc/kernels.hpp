@@ -36,7 +36,15 @@ struct MapParams {
   float light[3];      // position (point) or travel direction (directional)
   float light_color[3];
   float dir_far;       // directional-light origin distance
+  uint32_t bytes32;    // bytes of the map array when below 2^32, else 0
 };
+
+// Bytes of a map array of mp[0] mp[1] mp[2] values of dtype when below 2^32, else 0.
+inline uint32_t map_bytes32(const int64_t* mp, int32_t dtype)
+{
+  const uint64_t b = (uint64_t)mp[0] * (uint64_t)mp[1] * (uint64_t)mp[2] * (dtype == kF32 ? 4u : 2u);
+  return b < ((uint64_t)1 << 32) ? (uint32_t)b : 0u;
+}
 
 struct ShadeParams {
   const float* tf;     // ntf x RGBA (device)

@@ -321,10 +321,14 @@ struct MapCellCache {
   float v[8];
 };
 
+// map_opacity with a cache, in two parts: map_cell_update finds the map cell of p and, when it
+// is not the cached one, issues the corner loads (the camera march calls it before the field's
+// own loads, so both sets are in flight together); map_cell_opacity interpolates in the cell.
+// Same result as map_opacity: the same corner values and parametric coordinates.
 template <int MDT>
-__device__ __forceinline__ float map_opacity_cached(const MapParams& m, const float* p, MapCellCache& mc)
+__device__ __forceinline__ void map_cell_update(const MapParams& m, const CornerFetch<MDT>& mf,
+                                                const float* p, MapCellCache& mc)
 {
-  float pc[3];
   int cell[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -343,20 +347,16 @@ __device__ __forceinline__ float map_opacity_cached(const MapParams& m, const fl
       mc.w[d] = hi - lo;
       mc.rw[d] = 1.0f / mc.w[d];
     }
-    const int64_t s0 = m.ms[0], sx = m.ms[1], sxy = m.ms[2];
-    const int64_t i0 = cell[2] * sxy + cell[1] * sx + cell[0] * s0;
-    mc.v[0] = load_t<MDT>(m.map, i0);
-    mc.v[1] = load_t<MDT>(m.map, i0 + s0);
-    mc.v[2] = load_t<MDT>(m.map, i0 + s0 + sx);
-    mc.v[3] = load_t<MDT>(m.map, i0 + sx);
-    mc.v[4] = load_t<MDT>(m.map, i0 + sxy);
-    mc.v[5] = load_t<MDT>(m.map, i0 + sxy + s0);
-    mc.v[6] = load_t<MDT>(m.map, i0 + sxy + s0 + sx);
-    mc.v[7] = load_t<MDT>(m.map, i0 + sxy + sx);
+    mf.load(cell[0], cell[1], cell[2], mc.v);
     mc.c[0] = cell[0];
     mc.c[1] = cell[1];
     mc.c[2] = cell[2];
   }
+}
+
+__device__ __forceinline__ float map_cell_opacity(const float* p, const MapCellCache& mc)
+{
+  float pc[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) pc[d] = div_by_recip(p[d] - mc.lo[d], mc.w[d], mc.rw[d]);
   return interp_hex(mc.v, pc);

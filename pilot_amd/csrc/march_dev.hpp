@@ -129,9 +129,12 @@ __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapPa
   const int sx = (int)g.field_stride[0], sy = (int)g.field_stride[1], sz = (int)g.field_stride[2];
   const int rofs = coords_rofs(g);
   const CornerFetch<FDT> corners(g.field, g.bytes32, sx, sy, sz);
+  const CornerFetch<MDT> map_corners(m.map, m.bytes32, (int)m.ms[0], (int)m.ms[1], (int)m.ms[2]);
   const float sd = s.sample_dist;
   MapCellCache mcache;
   while (inside(g, p) && dist < maxD && ++steps < s.max_steps) {
+    // the shadow map's cell first: its corner loads, if any, overlap the field's
+    if (USE_MAP) map_cell_update<MDT>(m, map_corners, p, mcache);
     const float mint = fminf(tx, fminf(ty, tz));
     const float maxt = fmaxf(tx, fmaxf(ty, tz));
     if (maxt > 1.0f || mint < 0.0f) newCell = true;
@@ -184,7 +187,7 @@ __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapPa
       float4 c = tf[ci];
       c.w *= (1.0f - color[3]);
       if (USE_MAP) {
-        const float tr = 1.0f - map_opacity_cached<MDT>(m, p, mcache);
+        const float tr = 1.0f - map_cell_opacity(p, mcache);
         c.x = c.x * (tr * m.light_color[0]);
         c.y = c.y * (tr * m.light_color[1]);
         c.z = c.z * (tr * m.light_color[2]);
