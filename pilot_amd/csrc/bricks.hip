@@ -190,7 +190,7 @@ constexpr int kMaxCandidates = 1024;
 // One light segment (o -> p) against brick j: the conservative FP32 slab cull on the brick's
 // inflated box, then the exact test (FP64 slab terms); key = (exit distance bits, brick).
 __device__ __forceinline__ bool exchange_hit(const BrickFrame& f, int j, int b, const float* o,
-                                             const Segment& sg, const float* dd, const float* inv,
+                                             const Segment& sg, const float* inv, const float* lift,
                                              unsigned long long& key)
 {
   if (j == b) return false;
@@ -205,17 +205,16 @@ __device__ __forceinline__ bool exchange_hit(const BrickFrame& f, int j, int b, 
     bx[k] = bk->box32[k];
     bb[k] = bk->g.bounds[k];
   }
+  // On an axis the segment does not move along, inv[a] = 2^126 and lift[a] = 2: with o in
+  // the closed slab the interval then starts at or below 0 and (lifted) ends above 1; with o
+  // outside it lies beyond [0, 1], or (o within about 2^-126 of the slab) is kept.  The cull
+  // never drops a hit of the exact test and never meets a NaN.
   float t0 = 0.0f, t1 = 1.0f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    float ta = (bx[2 * a] - o[a]) * inv[a], tb = (bx[2 * a + 1] - o[a]) * inv[a];
-    if (dd[a] == 0.0f) {
-      const bool in = o[a] >= bx[2 * a] && o[a] <= bx[2 * a + 1];
-      ta = in ? -__builtin_inff() : __builtin_inff();
-      tb = in ? __builtin_inff() : -__builtin_inff();
-    }
+    const float ta = (bx[2 * a] - o[a]) * inv[a], tb = (bx[2 * a + 1] - o[a]) * inv[a];
     t0 = fmaxf(t0, fminf(ta, tb));
-    t1 = fminf(t1, fmaxf(ta, tb));
+    t1 = fminf(t1, fmaxf(ta, tb) + lift[a]);
   }
   if (!(t0 <= t1 + 1e-3f)) return false;
   float tmin, tmax;
@@ -303,7 +302,12 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
       const float dd[3] = { p[0] - o[0], p[1] - o[1], p[2] - o[2] };
       const float rs = 1.0f / __builtin_sqrtf(dot3(dd, dd));  // vtkm::Normalize
       const float dir[3] = { dd[0] * rs, dd[1] * rs, dd[2] * rs };
-      const float inv[3] = { 1.0f / dd[0], 1.0f / dd[1], 1.0f / dd[2] };
+      float inv[3], lift[3];  // the cull's slab terms (exchange_hit)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        inv[a] = dd[a] == 0.0f ? 0x1p126f : 1.0f / dd[a];
+        lift[a] = dd[a] == 0.0f ? 2.0f : 0.0f;
+      }
       Segment sg;  // SegmentAABB's per-segment terms, shared by every candidate brick
       segment_prepare(o, p, sg);
       auto composite = [&](float op, unsigned long long key) {
@@ -320,7 +324,7 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
         unsigned long long key;
         // the same candidate for every lane of the wave
         const int j = __builtin_amdgcn_readfirstlane(all ? q : cand[q]);
-        if (!exchange_hit(f, j, b, o, sg, dd, inv, key)) continue;
+        if (!exchange_hit(f, j, b, o, sg, inv, lift, key)) continue;
         if (nh == kRegHits) {
           spill = true;
           break;
@@ -351,7 +355,7 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
           for (int q = 0; q < nloop; ++q) {
             unsigned long long key;
             const int j = __builtin_amdgcn_readfirstlane(all ? q : cand[q]);
-            if (!exchange_hit(f, j, b, o, sg, dd, inv, key)) continue;
+            if (!exchange_hit(f, j, b, o, sg, inv, lift, key)) continue;
             if ((!started || key > last) && key < best) best = key;
           }
           if (best == ~0ull) break;

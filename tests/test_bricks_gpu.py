@@ -81,8 +81,8 @@ def test_single_brick_equals_one_block(oracle_mod):
 
 def test_bricks_long_light_paths_match_oracle(oracle_mod):
     """24 z-slabs under a light travelling along +z: a top vertex's light segment crosses 23
-    other bricks, past the 16 hits the exchange keeps in registers, so those vertices take its
-    rescan path; maps and image stay bit-exact."""
+    other bricks, past the 16 hits the exchange keeps in its hit list, so those vertices take
+    its rescan path; maps and image stay bit-exact."""
     sc = scenes.make_scene("slabs", 49, seed=7, nblobs=8, width=48, height=40, light="directional")
     d = np.array([0.12, 0.07, 1.0])
     sc.light_pos = (d / np.linalg.norm(d)).astype(np.float32)
@@ -91,5 +91,24 @@ def test_bricks_long_light_paths_match_oracle(oracle_mod):
     img, depth, maps, t = _gpu(sc, ranges)
     (oimg, local, final, partial), o, d = _oracle(oracle_mod, sc, ranges)
     for b in range(len(ranges)):
+        assert np.array_equal(maps[b][1], final[b]), f"brick {b} final map"
+    assert np.array_equal(img, oimg)
+
+
+@pytest.mark.parametrize("kind,pos", [("directional", (0.0, 0.0, 1.0)),
+                                      ("directional", (0.0, -1.0, 0.0)),
+                                      ("point", (0.5, 0.5, -1.5))])
+def test_bricks_axis_aligned_light_segments(oracle_mod, kind, pos):
+    """Light segments with zero direction components (a directional light along an axis; a
+    point light level with central map rows): the exchange's cull on an axis the segment does
+    not move along and the slab test's division by zero; maps and image stay bit-exact."""
+    sc = scenes.make_scene("axis", 49, seed=12, nblobs=10, width=64, height=48, light=kind)
+    sc.light_pos = np.float32(pos)
+    ranges = B.kd_bricks(sc.dims, 8, 3, 8, 32)
+    print(f"[bricks axis-aligned {kind} {pos}]", flush=True)
+    img, depth, maps, t = _gpu(sc, ranges)
+    (oimg, local, final, partial), o, d = _oracle(oracle_mod, sc, ranges)
+    for b in range(len(ranges)):
+        assert np.array_equal(maps[b][0], local[b]), f"brick {b} local map"
         assert np.array_equal(maps[b][1], final[b]), f"brick {b} final map"
     assert np.array_equal(img, oimg)
