@@ -262,8 +262,13 @@ ShadeParams shade_params(beams_renderer* h, const GridParams& g)
   s.tf = (const float*)h->tf.p;
   s.ntf = (int32_t)(h->htf.size() / 4);
   float ma = 0.0f;
-  for (int32_t i = 0; i < s.ntf; ++i) ma = std::fmax(ma, h->htf[4 * i + 3]);
+  bool finite = true;
+  for (int32_t i = 0; i < s.ntf; ++i) {
+    ma = std::fmax(ma, h->htf[4 * i + 3]);
+    finite = finite && std::isfinite(h->htf[4 * i + 3]);
+  }
   s.max_density = ma;
+  s.alpha_finite = (finite && std::isfinite(ma) && ma > 0.0f) ? 1 : 0;
   const float mn = (float)h->smin, mx = (float)h->smax;
   s.min_scalar = mn;
   s.inv_delta = ((mx - mn) != 0.0f) ? 1.0f / (mx - mn) : mn;

@@ -18,11 +18,12 @@ namespace beams {
 
 namespace {
 
-// Work queue of a persistent workgroup.  Items are handed out in chunks of consecutive items
-// from eight counters (one per XCD: workgroups are dealt to the XCDs round-robin), the eight
-// counters' chunks interleaved: one device-scope atomic per chunk, spread over eight
-// addresses, instead of one per item on a single address (whose serialisation bounded the
-// phase's time).
+// Work queue of a persistent workgroup.  Items are handed out one at a time from eight counters
+// (one per XCD: workgroups are dealt to the XCDs round-robin), counter x handing out items
+// x, x + 8, x + 16, ...: the workgroups always work on the next few thousand consecutive
+// items, so the bricks they read stay few and cached (larger chunks, or a fixed stride whose
+// workgroups drift apart, spread the reads over many bricks and slowed the 1024^3 maps by
+// half), while the device-scope atomics are spread over eight addresses instead of one.
 struct ItemQueue {
   int next, end, item;
 };
@@ -50,13 +51,6 @@ __device__ __forceinline__ int next_item(unsigned int* counters, ItemQueue* q, i
   return q->item;
 }
 
-// Chunk sizes: about sixteen fetches per workgroup for the map phases (hundreds of thousands
-// of items), single items for the march's few thousand tiles.
-__device__ __forceinline__ int map_chunk(const BrickFrame& f, int groups)
-{
-  const int c = f.n_map_items / (groups * 16);
-  return c < 1 ? 1 : (c > 16 ? 16 : c);
-}
 
 
 // A copy of a struct that every lane reads at the same address and that no kernel writes
@@ -85,7 +79,8 @@ __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
   float* a_tab = lds;
   float* xs = lds + ntf;
   unsigned int* counters = f.queue + (seeded ? 2 : 0) * kQueueSlots;
-  const int chunk = map_chunk(f, (int)gridDim.x);
+  // the seeded re-march has many cheap items (vertices already opaque): fetched four at a time
+  const int chunk = seeded ? 4 : 1;
   queue_init(&queue);
   int staged = -1;
   for (;;) {
@@ -107,7 +102,10 @@ __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
     if (v < bk->nverts) {
       int64_t taken = 0;
       const float seed = seeded ? f.seeds[bk->seed_off + v] : 0.0f;
-      const float op = march_vertex<FDT>(g, m, sp, a_tab, x, v, seed, &taken);
+      // a march seeded at opacity 1 returns 1 (each step adds (1 - 1) a = 0 for finite a);
+      // skipped unless steps are being counted
+      const bool opaque = seed == 1.0f && sp.alpha_finite && !sp.counters;
+      const float op = opaque ? 1.0f : march_vertex<FDT>(g, m, sp, a_tab, x, v, seed, &taken);
       store_t<MDT>(seeded ? bk->map_final : m.map, v, op);
       if (sp.counters && taken) atomicAdd(&sp.counters[1], (unsigned long long)taken);
     }
@@ -247,7 +245,7 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
   __shared__ int cand[kMaxCandidates];
   __shared__ unsigned long long hits[kRegHits][64];
   const int lane = threadIdx.x;
-  const int chunk = map_chunk(f, (int)gridDim.x);
+  const int chunk = 1;
   queue_init(&queue);
   for (;;) {
     const int item = next_item(f.queue + 1 * kQueueSlots, &queue, f.n_map_items, chunk);

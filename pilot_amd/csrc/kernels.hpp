@@ -53,6 +53,7 @@ struct ShadeParams {
   float min_scalar, inv_delta;
   float sample_dist, mesh_eps, alpha_stop;
   int32_t use_map;
+  int32_t alpha_finite;  // every alpha / max alpha is finite (a march seeded at 1 stays at 1)
   int64_t max_steps;   // bound on samples per camera ray (2 diagonals / step + 64)
   unsigned long long* counters;  // optional: [0] composited camera-ray samples, [1] light steps
 };

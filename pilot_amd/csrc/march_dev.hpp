@@ -76,7 +76,8 @@ __device__ __forceinline__ float march_vertex(const GridParams& g, const MapPara
     AxisCell ax = axis_guess(x[0], rofs, c0, p[0], g.gx0[0], g.ginv[0]);
     AxisCell ay = axis_guess(x[1], rofs, c1, p[1], g.gx0[1], g.ginv[1]);
     AxisCell az = axis_guess(x[2], rofs, c2, p[2], g.gx0[2], g.ginv[2]);
-    const bool settled = oracle_in(ax, p[0]) & oracle_in(ay, p[1]) & oracle_in(az, p[2]);
+    // all three tests evaluated (no short circuit: the reads above are issued together)
+    const bool settled = ((int)oracle_in(ax, p[0]) & (int)oracle_in(ay, p[1]) & (int)oracle_in(az, p[2])) != 0;
     if (__builtin_expect(!settled, 0)) {
       oracle_settle(x[0], rofs, c0, p[0], ax);
       oracle_settle(x[1], rofs, c1, p[1], ay);
