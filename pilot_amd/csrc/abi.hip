@@ -752,6 +752,7 @@ int build_bricks_learned(beams_renderer* h, const BrickFrame& f)
     HIP_TRY(h, hipMemcpyAsync(h->bredo.p, redo.data(), sizeof(int32_t) * (size_t)nb, hipMemcpyHostToDevice,
                               h->stream));
     if ((rc = phase_mark(h, BEAMS_PHASE_REMAP, false))) return rc;
+    HIP_TRY(h, launch_zero_u32(f.queue + 4 * kQueueSlots, kQueueSlots, h->stream));
     HIP_TRY(h, launch_bricks_exact_map(p, f, h->map_dtype, (const int32_t*)h->bredo.p, h->stream));
     if ((rc = phase_mark(h, BEAMS_PHASE_REMAP, true))) return rc;
     // the host vector must outlive the asynchronous copy
@@ -774,7 +775,7 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
   HIP_TRY(h, h->bmap_local.ensure((size_t)nv_total * msz));
   HIP_TRY(h, h->bmap_final.ensure((size_t)nv_total * msz));
   HIP_TRY(h, h->bseeds.ensure((size_t)nv_total * sizeof(float)));
-  HIP_TRY(h, h->bqueue.ensure(4 * kQueueSlots * sizeof(unsigned int)));
+  HIP_TRY(h, h->bqueue.ensure(5 * kQueueSlots * sizeof(unsigned int)));
   if ((rc = frame_begin(h))) return rc;
   BrickFrame f;
   int64_t total_verts = 0;

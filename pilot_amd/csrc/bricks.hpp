@@ -30,7 +30,7 @@ struct BrickFrame {
   const int32_t* px_item_tile;    // per march work item: 16x16 tile index in the footprint
   int32_t n_px_items;
   const int32_t* depth_order;     // bricks by ascending camera-to-centre distance
-  unsigned int* queue;            // work-queue counters, kQueueSlots per phase: map-1, exchange, map-3, march
+  unsigned int* queue;            // work-queue counters, kQueueSlots per phase: map-1, exchange, map-3, march, fallback
   float* seeds;                   // non-local opacity per map vertex (phase 2 -> 3)
   float4* partial;                // per-brick footprint RGBA
   float* partial_depth;           // per-brick footprint entry distance

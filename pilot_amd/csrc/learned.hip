@@ -4,6 +4,7 @@
 #include "learned.hpp"
 #include "march_dev.hpp"
 #include "mlp.hpp"
+#include "workqueue.hpp"
 
 #include <algorithm>
 
@@ -427,9 +428,9 @@ __global__ void __launch_bounds__(kBlock) k_bricks_learned_map(LearnedParams p, 
 }
 
 // The brick's map geometry with the whole volume's light-march step.
-__device__ __forceinline__ MapParams whole_volume_march(const BrickDesc& d, const MapParams& whole)
+__device__ __forceinline__ MapParams whole_volume_march(const MapParams& brick_map, const MapParams& whole)
 {
-  MapParams m = d.m;
+  MapParams m = brick_map;
   m.step = whole.step;
   m.dir_far = whole.dir_far;
   return m;
@@ -449,7 +450,7 @@ __global__ void __launch_bounds__(kBlock) k_bricks_learned_check(LearnedParams p
   __syncthreads();
   const float* x[3] = { xs, xs + p.g.n[0], xs + p.g.n[0] + p.g.n[1] };
   const BrickDesc& d = f.bricks[blockIdx.x];
-  const MapParams m = whole_volume_march(d, p.m);
+  const MapParams m = whole_volume_march(d.m, p.m);
   const int64_t n0 = check_axis(m.mp[0] - 1, p.check_stride);
   const int64_t n1 = check_axis(m.mp[1] - 1, p.check_stride);
   const int64_t n2 = check_axis(m.mp[2] - 1, p.check_stride);
@@ -489,15 +490,19 @@ __global__ void __launch_bounds__(kBlock) k_bricks_exact_map(LearnedParams p, Br
   stage_coords(p.g, xs);
   __syncthreads();
   const float* x[3] = { xs, xs + p.g.n[0], xs + p.g.n[0] + p.g.n[1] };
-  for (int item = blockIdx.x; item < f.n_map_items; item += gridDim.x) {
-    const int b = f.map_item_brick[item];
+  __shared__ ItemQueue queue;
+  queue_init(&queue);
+  for (;;) {
+    const int item = next_item(f.queue + 4 * kQueueSlots, &queue, f.n_map_items, 1);
+    if (item >= f.n_map_items) break;
+    const int b = __builtin_amdgcn_readfirstlane(f.map_item_brick[item]);
     if (!redo[b]) continue;
-    const BrickDesc& d = f.bricks[b];
+    const BrickDesc* bk = f.bricks + b;
     const int64_t id = f.map_item_first[item] + threadIdx.x;
-    if (id >= d.nverts) continue;
-    const MapParams m = whole_volume_march(d, p.m);
+    if (id >= bk->nverts) continue;
+    const MapParams m = whole_volume_march(load_uniform(&bk->m), p.m);
     int64_t taken = 0;
-    store_t<MDT>(d.map_final, id, march_vertex<FDT>(p.g, m, p.s, a_tab, x, id, 0.0f, &taken));
+    store_t<MDT>(bk->map_final, id, march_vertex<FDT>(p.g, m, p.s, a_tab, x, id, 0.0f, &taken));
   }
 }
 
