@@ -256,6 +256,15 @@ MapParams map_params(const beams_renderer* h, const GridParams& g, int32_t mode)
   return m;
 }
 
+// Bound on a camera ray's samples: two extents' worth of steps plus 64, at most 2^31 - 1 (the
+// kernels count samples in 32 bits).
+int64_t max_ray_steps(float ext_mag, float sample_dist)
+{
+  const float q = 2.0f * ext_mag / sample_dist;
+  if (!(q < 2147483520.0f)) return 2147483647;
+  return (int64_t)q + 64;
+}
+
 ShadeParams shade_params(beams_renderer* h, const GridParams& g)
 {
   ShadeParams s{};
@@ -278,7 +287,7 @@ ShadeParams shade_params(beams_renderer* h, const GridParams& g)
   s.mesh_eps = g.ext_mag * 0.0001f;  // LightedVolumeRenderer.cxx:1261
   s.alpha_stop = h->alpha_stop;
   s.use_map = h->use_shadow ? 1 : 0;
-  s.max_steps = (int64_t)(2.0f * g.ext_mag / s.sample_dist) + 64;
+  s.max_steps = max_ray_steps(g.ext_mag, s.sample_dist);
   s.counters = h->counting ? (unsigned long long*)h->counters.p : nullptr;
   return s;
 }
@@ -595,7 +604,7 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
     s = s0;
     s.sample_dist = user_sd > 0.0f ? user_sd : g.ext_mag / 200.0f;
     s.mesh_eps = g.ext_mag * 0.0001f;
-    s.max_steps = (int64_t)(2.0f * g.ext_mag / s.sample_dist) + 64;
+    s.max_steps = max_ray_steps(g.ext_mag, s.sample_dist);
     d.nverts = br.nverts;
     d.seed_off = br.seed_off;
     for (int k = 0; k < 3; ++k) {

@@ -107,13 +107,14 @@ __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapPa
                                              float* color)
 {
   if (minD == -1.0f) return 0;
-  int64_t composited = 0;
+  int composited = 0;  // per ray: below 2^31
   float dist = minD + s.mesh_eps;
   float p[3];
-  int64_t steps = 0;  // bounded by s.max_steps, which no finite in-box ray reaches
+  int steps = 0;  // bounded by s.max_steps (below 2^31), which no finite in-box ray reaches
+  const int max_steps = (int)s.max_steps;
 #pragma unroll
   for (int d = 0; d < 3; ++d) p[d] = org[d] + dist * dir[d];
-  while (!inside(g, p) && dist < maxD && ++steps < s.max_steps) {
+  while (!inside(g, p) && dist < maxD && ++steps < max_steps) {
     dist += s.sample_dist;
 #pragma unroll
     for (int d = 0; d < 3; ++d) p[d] = org[d] + dist * dir[d];
@@ -133,7 +134,7 @@ __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapPa
   const CornerFetch<MDT> map_corners(m.map, m.bytes32, (int)m.ms[0], (int)m.ms[1], (int)m.ms[2]);
   const float sd = s.sample_dist;
   MapCellCache mcache;
-  while (inside(g, p) && dist < maxD && ++steps < s.max_steps) {
+  while (inside(g, p) && dist < maxD && ++steps < max_steps) {
     // the shadow map's cell first: its corner loads, if any, overlap the field's
     if (USE_MAP) map_cell_update<MDT>(m, map_corners, p, mcache);
     const float mint = fminf(tx, fminf(ty, tz));
@@ -222,13 +223,14 @@ __device__ __forceinline__ int64_t sample_cell(const GridParams& g, const ShadeP
                                             float maxD, float* color)
 {
   if (minD == -1.0f) return 0;
-  int64_t composited = 0;
+  int composited = 0;  // per ray: below 2^31
   float dist = minD + s.mesh_eps;
   float p[3];
-  int64_t steps = 0;
+  int steps = 0;
+  const int max_steps = (int)s.max_steps;
 #pragma unroll
   for (int d = 0; d < 3; ++d) p[d] = org[d] + dist * dir[d];
-  while (!inside(g, p) && dist < maxD && ++steps < s.max_steps) {
+  while (!inside(g, p) && dist < maxD && ++steps < max_steps) {
     dist += s.sample_dist;
 #pragma unroll
     for (int d = 0; d < 3; ++d) p[d] = org[d] + dist * dir[d];
@@ -241,7 +243,7 @@ __device__ __forceinline__ int64_t sample_cell(const GridParams& g, const ShadeP
   int64_t cell[3] = { 0, 0, 0 };
   const int64_t cms = s.ntf - 1;
   const float sd = s.sample_dist;
-  while (inside(g, p) && dist < maxD && ++steps < s.max_steps) {
+  while (inside(g, p) && dist < maxD && ++steps < max_steps) {
     const float mint = fminf(tx, fminf(ty, tz));
     const float maxt = fmaxf(tx, fmaxf(ty, tz));
     if (maxt > 1.0f || mint < 0.0f) newCell = true;
