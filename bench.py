@@ -378,6 +378,27 @@ def measured_traffic(kernel, workload):
     return None if entry is None else entry["bytes_per_launch"]
 
 
+# MI355X vector issue: 256 CUs x 4 SIMDs, a wave64 VALU instruction occupies a SIMD for 4
+# cycles (64 FP32 FMA lanes per CU per clock), 2.4 GHz peak engine clock.
+SIMDS, VALU_CYCLES, CLOCK_GHZ = 1024, 4, 2.4
+
+
+def valu_issue(kernel, workload, kernel_ms):
+    """The kernel's VALU-issue bound from its instruction counters (profiles/instructions.json:
+    SQ_INSTS_VALU and SQ_WAVES per dispatch): the time the SIMDs need just to issue those
+    instructions, and its fraction of the measured kernel time; None when not counted."""
+    path = os.path.join(ROOT, "profiles", "instructions.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        e = json.load(f).get(workload, {}).get(kernel)
+    if e is None:
+        return None
+    issue_ms = e["sq_insts_valu"] * VALU_CYCLES / (SIMDS * CLOCK_GHZ * 1e9) * 1e3
+    return {"valu_per_wave": e["valu_per_wave"], "waves": e["sq_waves"], "issue_ms": issue_ms,
+            "kernel_ms": kernel_ms, "frac": issue_ms / kernel_ms}
+
+
 def cpu_baseline(sc, field32, gpu_map, threads, samples_per_frame):
     """Oracle (FP32 restatement of the reference) on a bounded sample of the same frame:
     a z-slab of the map and a centred ray tile, extrapolated to the full frame."""
@@ -517,6 +538,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(dom, f"{n}^3 f32/f32 {img}x{img}"),
             "algorithmic_bytes": dom_bytes,
+            # the bound that applies: instruction issue, not HBM
+            "valu_issue": valu_issue(dom, f"{n}^3 f32/f32 {img}x{img}", dom_ms),
         },
         "cpu_baseline": None,
     }
