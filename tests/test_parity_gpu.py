@@ -220,6 +220,46 @@ def test_set_field_rejects_grids_outside_the_kernels_domain():
                   spacing=(1, 1, 1))
 
 
+def test_field_over_4gib_takes_64bit_offsets(oracle_mod):
+    """A 1024 x 1024 x 1100 FP32 field (4.6 GB, past the 32-bit buffer offsets the kernels use
+    below 4 GiB): the map march and the camera march read its cell corners through 64-bit
+    addresses; map and rays bit-exact against the oracle."""
+    from pilot_amd.devmem import DeviceArray
+
+    O = oracle_mod
+    dims = (1024, 1024, 1100)
+    print("[field over 4 GiB] generating", flush=True)
+    dev = B.gen_field(dims, seed=77, nblobs=12, noise_octaves=2, noise_amp=0.2, dtype=B.F32,
+                      out=DeviceArray(dims[0] * dims[1] * dims[2], np.float32))
+    try:
+        coords = [scenes.unit_coords(n)[0] for n in dims]
+        tf32, _ = B.gen_colormap(78, 6, 256)
+        light = np.float32([1.9, -0.6, 0.4])
+        cam = B.orbit_camera([0.0, 1.0, 0.0, 1.0, 0.0, 1.0], 30.0, 15.0, 60.0)
+        o, d = B.camera_rays(cam, 32, 32)
+        cells = (16, 16, 16)
+        sd = float(np.float32(0.5 / 1023))
+        with B.LightedVolumeRenderer() as r:
+            r.SetData(dev, dims, coords=coords, scalar_range=(0.0, 1.0), dtype=B.F32)
+            r.SetColorMap(tf32)
+            r.AddLight(light, (2.0, 2.0, 2.0), 1.0, kind=B.LIGHT_POINT)
+            r.SetShadow(True, cells, B.F32)
+            r.SetSampleDistance(sd)
+            rays = r.Render(B.Rays(o, d))
+            gmap = r.GetMap(cells)
+        field = dev.numpy()
+    finally:
+        dev.free()
+    print("[field over 4 GiB] oracle", flush=True)
+    vol = O.Volume(dims, coords, field, tf32, (0.0, 1.0), B.LIGHT_POINT, [float(v) for v in light],
+                   [2.0, 2.0, 2.0], cells, True, sd, 1.0, twin="f32")
+    rb = O.RayBuf(o, d, "f32")
+    omap = O.render_rays(vol, rb)
+    del field, vol
+    _assert_same(rays, gmap, rb, omap, "field over 4 GiB")
+    assert rays.rgba[:, 3].max() > 0.1
+
+
 def test_device_generator_matches_host():
     from pilot_amd.devmem import DeviceArray
 
