@@ -99,7 +99,11 @@ int beams_synchronize(beams_renderer* h);
  * coords: three rectilinear coordinate arrays (ArrayHandleCartesianProduct), host memory,
  *   dims[i] increasing values each; or NULL for uniform coordinates given by origin and
  *   spacing (held as x_i = float(origin) + float(spacing) * i, what ArrayHandleCounting holds).
- * dtype: BEAMS_F32, BEAMS_F16 or BEAMS_BF16 storage; arithmetic is FP32. */
+ * dtype: BEAMS_F32, BEAMS_F16 or BEAMS_BF16 storage; arithmetic is FP32.
+ * Limits (BEAMS_ERR_INVALID / BEAMS_ERR_UNSUPPORTED otherwise): each dims[i] in [2, 2^20],
+ *   dims[0]*dims[1] below 2^31, dims[0]+dims[1]+dims[2] at most 16384 (the coordinate tables
+ *   are staged in LDS), coordinates finite with magnitude at most 2^60 and cell widths at
+ *   least 2^-60 (the range of the kernels' corrected-reciprocal quotients). */
 int beams_set_field(beams_renderer* h, const void* values, int32_t dtype, int32_t assoc,
                     const int64_t dims[3], const float* const coords[3], const double origin[3],
                     const double spacing[3], int32_t mem);
