@@ -157,6 +157,18 @@ Result Apply(Preset& p, const std::string& key, const std::string& raw, int line
     else
       p.ColorTable.NumEntries = static_cast<int>(d);
   }
+  else if (key == "color_table")
+  {
+    p.ColorTable.Name = str;
+  }
+  else if (key == "point_alphas")
+  {
+    if (!isArray || nums.size() % 2 != 0)
+      return fail("expected [x0, alpha0, x1, alpha1, ...]");
+    p.ColorTable.PointAlphas.clear();
+    for (size_t i = 0; i < nums.size(); i += 2)
+      p.ColorTable.PointAlphas.push_back({ static_cast<float>(nums[i]), static_cast<float>(nums[i + 1]) });
+  }
   else if (key == "scalar_range")
   {
     if (!isArray || nums.size() != 2)

@@ -63,7 +63,11 @@ struct LightOption
 
 struct ColorTableOptions
 {
+  // "seeded": beams_gen_colormap's seeded table (Seed, NumPoints, NumEntries); otherwise a
+  // ColorTable preset name ("cool to warm", "grayscale") with PointAlphas, sampled into
+  // 1024 8-bit entries as Mapper::SetActiveColorTable does (FileSceneBase.cxx:67-88)
   std::string Name = "seeded";
+  std::vector<std::array<float, 2>> PointAlphas;
   float RangeMin = 0.0f;
   float RangeMax = 1.0f;
   uint64_t Seed = 1;

@@ -135,15 +135,28 @@ std::shared_ptr<rendering::Scene> SceneFactory::CreateFromPreset(const Preset& p
     throw ErrorBadValue(r.Err);
   scene->Id = preset.Id;
   scene->ScalarRange = { preset.ColorTable.RangeMin, preset.ColorTable.RangeMax };
-  const int ne = preset.ColorTable.NumEntries;
-  std::vector<float> rgba(static_cast<size_t>(ne) * 4);
-  if (beams_gen_colormap(preset.ColorTable.Seed, preset.ColorTable.NumPoints, ne, rgba.data(),
-                         nullptr) != BEAMS_OK)
-    throw ErrorBadValue("colour map generation failed");
-  scene->ColorMap.resize(static_cast<size_t>(ne));
-  for (int i = 0; i < ne; ++i)
-    scene->ColorMap[static_cast<size_t>(i)] = { rgba[4 * i], rgba[4 * i + 1], rgba[4 * i + 2],
-                                                 rgba[4 * i + 3] };
+  if (preset.ColorTable.Name == "seeded")
+  {
+    const int ne = preset.ColorTable.NumEntries;
+    std::vector<float> rgba(static_cast<size_t>(ne) * 4);
+    if (beams_gen_colormap(preset.ColorTable.Seed, preset.ColorTable.NumPoints, ne, rgba.data(),
+                           nullptr) != BEAMS_OK)
+      throw ErrorBadValue("colour map generation failed");
+    scene->ColorMap.resize(static_cast<size_t>(ne));
+    for (int i = 0; i < ne; ++i)
+      scene->ColorMap[static_cast<size_t>(i)] = { rgba[4 * i], rgba[4 * i + 1], rgba[4 * i + 2],
+                                                   rgba[4 * i + 3] };
+  }
+  else
+  {
+    // a named table with alpha points, as the reference's scenes build theirs
+    rendering::ColorTable table(preset.ColorTable.Name);
+    if (!table.Valid())
+      throw ErrorBadValue("unknown colour table: " + preset.ColorTable.Name);
+    for (const auto& pa : preset.ColorTable.PointAlphas)
+      table.AddPointAlpha(pa[0], pa[1]);
+    scene->ColorMap = table.SampleColorMap(1024);
+  }
   const auto& pd = scene->Data.CellSet.PointDims;
   scene->ShadowMapSize = preset.OpacityMap.CalculateSize({ pd[0] - 1, pd[1] - 1, pd[2] - 1 });
   scene->UseShadowMap = preset.OpacityMap.Enabled;

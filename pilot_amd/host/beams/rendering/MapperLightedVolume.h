@@ -5,6 +5,8 @@
 #ifndef beams_rendering_mapper_lighted_volume_h
 #define beams_rendering_mapper_lighted_volume_h
 
+#include "../io/ImageUtils.h"
+#include "ColorTable.h"
 #include "LightedVolumeRenderer.h"
 
 namespace beams
@@ -42,6 +44,14 @@ struct Canvas
     std::fill(Color.begin(), Color.end(), 0.0f);
     std::fill(Depth.begin(), Depth.end(), 0.0f);
   }
+  // Canvas::SaveAs as Beams.cxx:292-301 calls it: ".png" (RGBA) or ".ppm"/".pnm" (RGB), 8
+  // bits per channel; an unknown extension or a failed write throws ErrorBadValue.
+  void SaveAs(const std::string& fileName) const
+  {
+    const io::WriteResult r = io::SaveImage(Color, Width, Height, fileName);
+    if (!r.Success)
+      throw ErrorBadValue(r.Error);
+  }
   int Width, Height;
   std::vector<float> Color;
   std::vector<float> Depth;
@@ -55,6 +65,8 @@ public:
   void SetCanvas(Canvas* canvas);
   Canvas* GetCanvas() const { return this->Canvas_; }
   void SetActiveColorMap(const std::vector<Vec4f>& colorMap) { this->ColorMap = colorMap; }
+  // Mapper::SetActiveColorTable: the table sampled into 1024 8-bit RGBA entries / 255
+  void SetActiveColorTable(const ColorTable& table) { this->ColorMap = table.SampleColorMap(1024); }
   void RenderCells(const CellSetStructured& cellset,
                    const CoordinateSystem& coords,
                    const Field& scalarField,

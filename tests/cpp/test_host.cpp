@@ -5,11 +5,14 @@
 #include "../../include/beams_mi355x.h"
 #include "beams/SceneFactory.h"
 #include "beams/io/DataSetUtils.h"
+#include "beams/io/ImageUtils.h"
+#include "beams/rendering/ColorTable.h"
 
 #include <unistd.h>
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <string>
@@ -177,8 +180,153 @@ static void test_io(const beams::DataSet& ds, const beams::DataSet& bf16)
   CHECK(threw);
 }
 
+// The synthetic frame written by test_images; tests/test_host_cpp.py decodes the files
+// with zlib and recomputes the same values.
+static float FrameValue(int x, int y, int c)
+{
+  if (x == 0 && y == 0) {
+    const float special[4] = { -0.5f, 1.7f, NAN, 0.5f };
+    return special[c];
+  }
+  return static_cast<float>((x * 37 + y * 101 + c * 53) % 256) / 255.0f;
+}
+
+static void test_images()
+{
+  namespace io = beams::io;
+  CHECK(io::QuantizeChannel(0.0f) == 0 && io::QuantizeChannel(1.0f) == 255);
+  CHECK(io::QuantizeChannel(0.5f) == 128 && io::QuantizeChannel(-3.0f) == 0);
+  CHECK(io::QuantizeChannel(2.0f) == 255 && io::QuantizeChannel(NAN) == 0);
+  CHECK(io::FrameFileName("blobs", 3, 7) == "blobs_3_007.png");
+  CHECK(io::FrameFileName("s", 0, 1234, ".ppm") == "s_0_1234.ppm");
+
+  const int w = 5, h = 3;
+  std::vector<float> rgba(static_cast<size_t>(w) * h * 4);
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x)
+      for (int c = 0; c < 4; ++c) rgba[(static_cast<size_t>(y) * w + x) * 4 + c] = FrameValue(x, y, c);
+  const char* dir = std::getenv("BEAMS_TEST_IMAGE_DIR");
+  const std::string base = dir ? std::string(dir) + "/frame" : TempPath("frame");
+  CHECK(io::SaveImage(rgba, w, h, base + ".png").Success);
+  CHECK(io::SaveImage(rgba, w, h, base + ".ppm").Success);
+  CHECK(!io::SaveImage(rgba, w, h, base + ".jpg").Success);
+  CHECK(!io::SaveImage(rgba, w, h + 1, base + ".png").Success);
+  beams::rendering::Canvas canvas(w, h);
+  canvas.Color = rgba;
+  bool threw = false;
+  try {
+    canvas.SaveAs(base + ".bmp");
+  } catch (const beams::ErrorBadValue& e) {
+    threw = std::strstr(e.what(), "unknown image type") != nullptr;
+  }
+  CHECK(threw);
+
+  std::ifstream pf(base + ".ppm", std::ios::binary);
+  std::string ppm((std::istreambuf_iterator<char>(pf)), std::istreambuf_iterator<char>());
+  const std::string header = "P6\n5 3\n255\n";
+  CHECK(ppm.size() == header.size() + 45 && ppm.compare(0, header.size(), header) == 0);
+  if (ppm.size() == header.size() + 45) {
+    bool same = true;
+    for (int i = 0; i < 15; ++i)
+      for (int c = 0; c < 3; ++c)
+        same &= static_cast<uint8_t>(ppm[header.size() + 3 * i + c]) ==
+                io::QuantizeChannel(FrameValue(i % w, i / w, c));
+    CHECK(same);
+  }
+  // PNG: signature, IHDR, and the IEND chunk whose CRC-32 is the well-known AE 42 60 82
+  std::ifstream nf(base + ".png", std::ios::binary);
+  std::string png((std::istreambuf_iterator<char>(nf)), std::istreambuf_iterator<char>());
+  const unsigned char sig[8] = { 0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n' };
+  CHECK(png.size() > 45 && std::memcmp(png.data(), sig, 8) == 0);
+  CHECK(png.compare(12, 4, "IHDR") == 0 && png[19] == 5 && png[23] == 3 && png[24] == 8 && png[25] == 6);
+  const unsigned char iend[12] = { 0, 0, 0, 0, 'I', 'E', 'N', 'D', 0xAE, 0x42, 0x60, 0x82 };
+  CHECK(png.size() > 12 && std::memcmp(png.data() + png.size() - 12, iend, 12) == 0);
+  // a frame of more than one 65535-byte stored deflate block
+  const int bw = 160, bh = 120;
+  std::vector<float> big(static_cast<size_t>(bw) * bh * 4);
+  for (int y = 0; y < bh; ++y)
+    for (int x = 0; x < bw; ++x)
+      for (int c = 0; c < 4; ++c) big[(static_cast<size_t>(y) * bw + x) * 4 + c] = FrameValue(x, y, c);
+  CHECK(io::SaveImage(big, bw, bh, base + "_big.png").Success);
+  if (!dir) {
+    std::remove((base + "_big.png").c_str());
+    std::remove((base + ".png").c_str());
+    std::remove((base + ".ppm").c_str());
+  }
+}
+
+static void test_color_tables()
+{
+  namespace r = beams::rendering;
+  // Moreland's Msh coordinates of the cool and warm endpoints: (80, 1.08, -1.10), (80, 1.08, 0.50)
+  const auto cool = r::RgbToMsh({ 0.230, 0.299, 0.754 });
+  const auto warm = r::RgbToMsh({ 0.706, 0.016, 0.150 });
+  CHECK(std::fabs(cool[0] - 80.0) < 0.1 && std::fabs(cool[1] - 1.08) < 0.005 && std::fabs(cool[2] + 1.10) < 0.005);
+  CHECK(std::fabs(warm[0] - 80.0) < 0.1 && std::fabs(warm[1] - 1.08) < 0.005 && std::fabs(warm[2] - 0.50) < 0.005);
+  // Msh -> RGB inverts RGB -> Msh
+  const auto back = r::MshToRgb(cool);
+  CHECK(std::fabs(back[0] - 0.230) < 1e-5 && std::fabs(back[1] - 0.299) < 1e-5 && std::fabs(back[2] - 0.754) < 1e-5);
+
+  // Moreland's published cool-to-warm byte table at eighths of the range (one count of
+  // rounding slack): the hue spin, the white midpoint and the Lab transforms all show here
+  const int table[9][3] = { { 59, 76, 192 },   { 98, 130, 234 },  { 141, 176, 254 },
+                            { 184, 208, 249 }, { 221, 221, 221 }, { 245, 196, 173 },
+                            { 244, 154, 123 }, { 222, 96, 77 },   { 180, 4, 38 } };
+  r::ColorTable ct("Cool to Warm");
+  CHECK(ct.Valid());
+  bool near = true;
+  for (int i = 0; i <= 8; ++i) {
+    const auto c = ct.MapColor(i / 8.0);
+    for (int k = 0; k < 3; ++k) near &= std::fabs(c[k] * 255.0 - table[i][k]) <= 1.0;
+  }
+  CHECK(near);
+  CHECK(!r::ColorTable("no such preset").Valid());
+
+  // alpha points: piecewise linear, constant outside, later point at the same x replaces
+  ct.AddPointAlpha(0.0, 0.0f);
+  ct.AddPointAlpha(1.0, 0.8f);
+  ct.AddPointAlpha(0.5, 0.1f);
+  ct.AddPointAlpha(0.5, 0.2f);
+  CHECK(std::fabs(ct.MapAlpha(0.25) - 0.1) < 1e-7 && std::fabs(ct.MapAlpha(0.75) - 0.5) < 1e-7);
+  CHECK(ct.MapAlpha(-1.0) == 0.0 && std::fabs(ct.MapAlpha(2.0) - 0.8) < 1e-7);
+  const auto s = ct.Sample(1024);
+  CHECK(s.size() == 1024);
+  CHECK(s[0][0] == 59 && s[0][1] == 76 && s[0][2] == 192 && s[0][3] == 0);
+  CHECK(s[1023][0] == 180 && s[1023][1] == 4 && s[1023][2] == 38 && s[1023][3] == 204);
+  const auto m = ct.SampleColorMap();
+  CHECK(m.size() == 1024 && m[1023].W == 204.0f / 255.0f);
+
+  // the grayscale preset is a linear ramp: entry i of n holds round(255 i / (n - 1))
+  const auto g = r::ColorTable("grayscale").Sample(256);
+  bool ramp = g.size() == 256;
+  for (int i = 0; ramp && i < 256; ++i) ramp = g[i][0] == i && g[i][1] == i && g[i][2] == i && g[i][3] == 255;
+  CHECK(ramp);
+
+  // a scene built from a named table with alpha points gets the 1024-entry sampled map
+  beams::Config cfg;
+  CHECK(cfg.Deserialize("[presets.t]\ndataset = \"blobs\"\nsize = 8\ncolor_table = \"cool to warm\"\n"
+                        "point_alphas = [0.0, 0.0, 1.0, 0.5]\n").Success);
+  CHECK(cfg.Presets.at("t").ColorTable.PointAlphas.size() == 2);
+  auto scene = beams::SceneFactory::CreateFromPreset(cfg.Presets.at("t"));
+  CHECK(scene->ColorMap.size() == 1024 && scene->ColorMap[1023].W == 128.0f / 255.0f);
+  CHECK(scene->ColorMap[0].X == 59.0f / 255.0f && scene->ColorMap[0].W == 0.0f);
+  CHECK(!cfg.Deserialize("[presets.t]\npoint_alphas = [0.0, 0.0, 1.0]\n").Success);
+  beams::Config named;
+  CHECK(named.Deserialize("[presets.u]\ndataset = \"blobs\"\nsize = 8\ncolor_table = \"nope\"\n").Success);
+  bool threw = false;
+  try {
+    beams::SceneFactory::CreateFromPreset(named.Presets.at("u"));
+  } catch (const beams::ErrorBadValue& e) {
+    threw = std::strstr(e.what(), "nope") != nullptr;
+  }
+  CHECK(threw);
+}
+
 static void test_cpu()
 {
+  test_images();
+  test_color_tables();
+
   beams::Config cfg;
   beams::Result r = cfg.Deserialize(kToml);
   CHECK(r.Success);
@@ -245,6 +393,12 @@ static void test_gpu()
   double amax = 0.0;
   for (size_t i = 3; i < canvas.Color.size(); i += 4) amax = std::fmax(amax, canvas.Color[i]);
   CHECK(amax > 0.5);  // the background is blended under the image (alpha -> 1 everywhere)
+  const std::string frame = TempPath(beams::io::FrameFileName(scene->Id, 0, 0));
+  canvas.SaveAs(frame);
+  std::ifstream ff(frame, std::ios::binary);
+  std::string pngHead(8, '\0');
+  CHECK(ff.read(&pngHead[0], 8) && pngHead.compare(1, 3, "PNG") == 0);
+  std::remove(frame.c_str());
 
   // Render(Ray<Float32>) on the camera's rays equals RenderImage without background.
   beams::rendering::LightedVolumeRenderer& lvr = scene->Mapper.GetRenderer();
