@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
       const bool opaque = seed == 1.0f && sp.alpha_finite && !sp.counters;
       const float op = opaque ? 1.0f : march_vertex<FDT>(g, m, sp, a_tab, x, v, seed, &taken);
       store_t<MDT>(seeded ? bk->map_final : m.map, v, op);
-      if (sp.counters && taken) atomicAdd(&sp.counters[1], (unsigned long long)taken);
+      if (sp.counters) count_add(&sp.counters[1], (uint32_t)taken);
     }
   }
 }

@@ -27,7 +27,7 @@ __global__ void __launch_bounds__(kBlock) k_tmap_ref(GridParams g, MapParams m, 
   int64_t taken = 0;
   const float op = march_vertex<FDT>(g, m, s, a_tab, x, id, seed ? seed[id] : 0.0f, &taken);
   store_t<MDT>(m.map, id, op);
-  if (s.counters && taken) atomicAdd(&s.counters[1], (unsigned long long)taken);
+  if (s.counters) count_add(&s.counters[1], (uint32_t)taken);
 }
 
 template <int FDT, int MDT, bool USE_MAP>

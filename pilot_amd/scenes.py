@@ -125,11 +125,13 @@ def make_scene(name, n, seed, nblobs, noise_octaves=0, noise_amp=0.0, light="poi
 def plain_small(seed=1, **kw):
     """64^3 FP32, 16 blobs, directional light, 64^3-cell map, 512x512, step 0.5 voxel."""
     kw.setdefault("light", "directional")
+    kw.setdefault("map_cells", (64, 64, 64))
     return make_scene("plain_small", 64, seed, 16, width=512, height=512, **kw)
 
 
 def precision(seed=2, field_dtype=B.F32, map_dtype=B.F32, **kw):
     """256^3 blobs (64) + 3-octave noise at 0.2, point light, 256^3-cell map, 1024x1024."""
+    kw.setdefault("map_cells", (256, 256, 256))
     return make_scene("precision", 256, seed, 64, noise_octaves=3, noise_amp=0.2,
                       light="point", width=kw.pop("width", 1024), height=kw.pop("height", 1024),
                       field_dtype=field_dtype, map_dtype=map_dtype, **kw)
