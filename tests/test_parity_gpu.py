@@ -34,10 +34,12 @@ def _render_both(O, sc, rays_o, rays_d, twin_field=None, min_d=None, max_d=None,
 
 
 def _dump(label, **arrays):
-    """Keep a failing case's inputs and outputs for analysis off the GPU."""
-    os.makedirs("gpurun_out", exist_ok=True)
+    """Keep a failing case's inputs and outputs for analysis off the GPU (in the folder
+    BEAMS_FAILURE_DIR names, default ./failures)."""
+    out = os.environ.get("BEAMS_FAILURE_DIR", "failures")
+    os.makedirs(out, exist_ok=True)
     safe = "".join(ch if ch.isalnum() else "_" for ch in label)
-    np.savez_compressed(os.path.join("gpurun_out", f"fail_{safe}.npz"),
+    np.savez_compressed(os.path.join(out, f"fail_{safe}.npz"),
                         **{k: v for k, v in arrays.items() if v is not None})
 
 
