@@ -1145,6 +1145,7 @@ int beams_set_colormap(beams_renderer* h, const float* rgba, int64_t n)
 {
   CHECK_HANDLE(h);
   if (!rgba || n < 1 || n > 8192) return h->fail(BEAMS_ERR_INVALID, "colormap needs 1..8192 RGBA entries");
+  h->map_valid = false;  // the map's opacities come from the table's alpha
   h->htf.assign(rgba, rgba + 4 * n);
   HIP_TRY(h, h->tf.ensure(sizeof(float) * 4 * (size_t)n));
   HIP_TRY(h, hipMemcpyAsync(h->tf.p, h->htf.data(), sizeof(float) * 4 * (size_t)n, hipMemcpyHostToDevice, h->stream));
@@ -1162,18 +1163,17 @@ int beams_set_light(beams_renderer* h, int32_t kind, const float xyz[3], const f
   for (int d = 0; d < 3; ++d)
     if (!std::isfinite(xyz[d]) || !std::isfinite(color[d] * intensity))
       return h->fail(BEAMS_ERR_INVALID, "light position and colour must be finite");
+  const float n2 = xyz[0] * xyz[0] + xyz[1] * xyz[1] + xyz[2] * xyz[2];
+  if (kind == BEAMS_LIGHT_DIRECTIONAL && !(n2 > 0.0f))
+    return h->fail(BEAMS_ERR_INVALID, "directional light needs a nonzero direction");
+  const float rs = kind == BEAMS_LIGHT_DIRECTIONAL ? 1.0f / std::sqrt(n2) : 1.0f;
   for (int d = 0; d < 3; ++d) {
-    h->light[d] = xyz[d];
+    h->light[d] = kind == BEAMS_LIGHT_DIRECTIONAL ? xyz[d] * rs : xyz[d];
     h->light_color[d] = color[d] * intensity;  // Lights::AddLight(radius, pos, Color * Intensity)
-  }
-  if (kind == BEAMS_LIGHT_DIRECTIONAL) {
-    const float n2 = xyz[0] * xyz[0] + xyz[1] * xyz[1] + xyz[2] * xyz[2];
-    if (!(n2 > 0.0f)) return h->fail(BEAMS_ERR_INVALID, "directional light needs a nonzero direction");
-    const float rs = 1.0f / std::sqrt(n2);
-    for (int d = 0; d < 3; ++d) h->light[d] = xyz[d] * rs;
   }
   h->light_kind = kind;
   h->has_light = true;
+  h->map_valid = false;  // the map belongs to the previous light
   return BEAMS_OK;
 }
 
@@ -1185,11 +1185,11 @@ int beams_set_shadow(beams_renderer* h, int32_t enabled, const int64_t map_cells
     return h->fail(BEAMS_ERR_UNSUPPORTED, "unknown shadow mode");
   if (map_dtype != BEAMS_F32 && map_dtype != BEAMS_F16)
     return h->fail(BEAMS_ERR_UNSUPPORTED, "map storage must be F32 or F16");
-  if (map_cells)
-    for (int d = 0; d < 3; ++d) {
+  if (map_cells) {
+    for (int d = 0; d < 3; ++d)
       if (map_cells[d] < 1 || map_cells[d] > 4096) return h->fail(BEAMS_ERR_INVALID, "map cells must be in [1, 4096]");
-      h->map_cells[d] = map_cells[d];
-    }
+    for (int d = 0; d < 3; ++d) h->map_cells[d] = map_cells[d];
+  }
   h->use_shadow = enabled != 0;
   h->shadow_mode = mode;
   h->map_dtype = map_dtype;

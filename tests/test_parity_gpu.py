@@ -222,6 +222,36 @@ def test_set_field_rejects_grids_outside_the_kernels_domain():
                   spacing=(1, 1, 1))
 
 
+def test_rejected_setters_leave_the_renderer_unchanged():
+    """A refused SetShadow (one bad axis) or AddLight (zero direction) changes nothing: the
+    next frame and map equal the previous ones bit for bit.  A new light or colour table
+    marks the old map stale until the next frame builds one."""
+    sc = scenes.make_scene("setters", 20, seed=5, nblobs=6, width=48, height=40, light="directional")
+    with B.LightedVolumeRenderer() as r:
+        scenes.configure(r, sc)
+        img0, _ = r.RenderImage(sc.camera, sc.width, sc.height)
+        map0 = r.GetMap(sc.map_cells)
+        bad_cells = (sc.map_cells[0] + 3, 0, sc.map_cells[2])
+        with pytest.raises(B.BeamsError) as ei:
+            r.SetShadow(True, bad_cells, sc.map_dtype)
+        assert ei.value.status == 1
+        with pytest.raises(B.BeamsError) as ei:
+            r.AddLight((0.0, 0.0, 0.0), sc.light_color, 1.0, kind=B.LIGHT_DIRECTIONAL)
+        assert ei.value.status == 1
+        img1, _ = r.RenderImage(sc.camera, sc.width, sc.height)
+        assert np.array_equal(img0, img1)
+        assert np.array_equal(map0, r.GetMap(sc.map_cells))
+        r.AddLight(-sc.light_pos, sc.light_color, 1.0, kind=B.LIGHT_DIRECTIONAL)
+        with pytest.raises(B.BeamsError) as ei:
+            r.GetMap(sc.map_cells)
+        assert ei.value.status == 3
+        img2, _ = r.RenderImage(sc.camera, sc.width, sc.height)
+        assert not np.array_equal(map0, r.GetMap(sc.map_cells)) and not np.array_equal(img0, img2)
+        r.SetColorMap(sc.tf32)
+        with pytest.raises(B.BeamsError):
+            r.GetMap(sc.map_cells)
+
+
 def test_field_over_4gib_takes_64bit_offsets(oracle_mod):
     """A 1024 x 1024 x 1100 FP32 field (4.6 GB, past the 32-bit buffer offsets the kernels use
     below 4 GiB): the map march and the camera march read its cell corners through 64-bit
