@@ -185,7 +185,9 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
                        int32_t mem);
 
 /* Copy of the last transmittance (opacity) map, (cells+1)^3 FP32 values; the debugging
- * dump SaveTransmittance (TransmittanceMap.h:527-537) writes the same field. */
+ * dump SaveTransmittance (TransmittanceMap.h:527-537) writes the same field.  After a change
+ * of field, bricks, light, colour table or shadow settings the map is stale and this returns
+ * BEAMS_ERR_STATE until the next frame with shadows builds one. */
 int beams_get_map(beams_renderer* h, float* out, int32_t mem);
 
 /* Per-phase device times of the last render in milliseconds (BEAMS_NUM_PHASES values),
