@@ -389,6 +389,83 @@ def test_nonlocal_hits_brute_force(oracle_mod):
     assert checked == 120
 
 
+def _steps_inside(L, step, box_hi):
+    """Light steps a march of length L takes inside a box it entered at distance 0 (moving
+    toward box_hi): floor(L/step) + 1 with the step past the vertex, minus that step when it
+    lands beyond box_hi.  Multiples of step are exact here (step = 3/256)."""
+    K = int(np.floor(L / step)) + 1
+    return K - 1 if K * step > box_hi + 1e-15 else K
+
+
+def test_two_brick_exchange_closed_form(oracle_mod):
+    """Two bricks split by the plane x = 1/2, a constant field with alpha/max = r at its
+    TF entry and a directional light travelling along +x.  The far brick's final map is
+    the near brick's opacity where the light ray leaves it, composited over the far brick's
+    own march: 1 - (1 - r)^K1 (1 - r)^K2, with K1 and K2 the steps taken inside each brick
+    (step = |brick extent| / 128 = 3/256).  A slip in the exchange (a wrong exit point, a
+    seed from the wrong brick, a seed composited in the wrong order) moves K1.
+
+    Rays that lie in a face of a brick keep the reference's slab-test outcome
+    (Intersections.cxx:8-49): a zero direction component divides 0 by 0, and fmin/fmax
+    (vtkm::Min/Max) pass over the NaN.  On the low faces (y or z = 0) the entry distance
+    becomes +inf and the march takes no step; on the high faces the exit distance becomes
+    -inf, the length bound is lost and the march crosses the whole brick (K = 42).  A
+    min/max written with comparisons would give other values there."""
+    import ctypes as C
+
+    O = oracle_mod
+    n, r, v = 17, 0.25, 0.5
+    c = np.arange(n, dtype=np.float64) / (n - 1)
+    tf = np.tile([1.0, 1.0, 1.0, 0.8], (256, 1))
+    tf[int(v * 255)] = [1, 1, 1, 0.8 * r]
+    field = np.full((n, n, 9), v)  # z, y, x of one brick: 9 x-points each
+    light = (1.0, 0.0, 0.0)
+    far = 4.0 * np.sqrt(3.0)
+    vols = [O.Volume((9, n, n), [c[lo:lo + 9], c, c], field, tf, light_kind=1, light_pos=light,
+                     map_cells=(8, n - 1, n - 1), sample_distance=0.5 / (n - 1), twin="f64",
+                     light_far=far)
+            for lo in (0, 8)]
+    local = [O.tmap_build(vb) for vb in vols]
+    arr = (O.OrcScene * 2)(*[vb.s for vb in vols])
+    mptr = (C.c_void_p * 2)(*[m.ctypes.data for m in local])
+    final = []
+    for b, vb in enumerate(vols):
+        seed = np.zeros(vb.nverts)
+        O.lib().orc_nonlocal_seed_f64(2, arr, mptr, b, seed.ctypes.data, 1)
+        final.append(O.tmap_build(vb, seed))
+    step = 1.5 / 128  # |(1/2, 1, 1)| / 128
+    K1 = _steps_inside(0.5, step, 0.5)
+    assert K1 == 42
+    near = 1 - (1 - r) ** K1
+    g = c[8:]
+    for m in local:
+        m3 = m.reshape(n, n, 9)
+        assert np.all(m3[0, :, :] == 0.0) and np.all(m3[:, 0, :] == 0.0)
+        assert np.all(np.abs(m3[n - 1, 1:, :] - near) < 1e-12)
+        assert np.all(np.abs(m3[1:, n - 1, :] - near) < 1e-12)
+    checked = 0
+    for k in range(1, n - 1):
+        for j in range(1, n - 1):
+            # the near brick's exit-face vertex: its local opacity is the seed the far brick
+            # receives; its own final value also takes the far brick's opacity at the touching
+            # face (one step: r), a zero-length hit as the slab test reports it
+            assert abs(local[0][(k * n + j) * 9 + 8] - near) < 1e-12
+            assert abs(local[1][(k * n + j) * 9] - r) < 1e-15
+            assert abs(final[0][(k * n + j) * 9 + 8] - (1 - (1 - near) * (1 - r))) < 1e-12
+            # the near brick's other vertices see no other brick before them
+            row = slice((k * n + j) * 9, (k * n + j) * 9 + 8)
+            assert np.array_equal(final[0][row], local[0][row])
+            for i in range(1, 9):
+                K2 = _steps_inside(g[i] - 0.5, step, 0.5)
+                expect = 1 - (1 - near) * (1 - r) ** K2
+                got = final[1][(k * n + j) * 9 + i]
+                assert abs(got - expect) < 1e-12, (i, j, k, got, expect)
+                # without the exchange the far brick sees only its own march
+                assert abs(local[1][(k * n + j) * 9 + i] - (1 - (1 - r) ** K2)) < 1e-12
+                checked += 1
+    assert checked == 8 * (n - 2) * (n - 2)
+
+
 # ---- GlobalComposite (MapperLightedVolume.cxx:215-303) --------------------------------------
 
 def test_composite_depth_order_and_over(oracle_mod):
