@@ -296,6 +296,31 @@ def test_sampler_opaque_entry_stops_after_one_sample(oracle_mod):
     assert np.allclose(rb.rgba, [[0.25, 0.5, 0.75, 1.0]])
 
 
+def test_sampler_linear_field_selects_the_tf_entry(oracle_mod):
+    """Field f = x (scalar range [0, 1]) and an opaque TF whose 256 entries all differ: a ray
+    parallel to y at x0 = (e + 1/2) / 255 samples f = x0 once and returns entry e's colour,
+    e = trunc(f (n - 1)).  A rounded index, a scale by n instead of n - 1, a wrong scalar
+    normalisation or a swapped channel picks another entry.  A range [lo, hi] maps
+    lo + (hi - lo) x0 to the same entry."""
+    O = oracle_mod
+    n = 9
+    c = np.arange(n, dtype=np.float64) / (n - 1)
+    i = np.arange(256) / 255.0
+    tf = np.stack([i, 1 - i, (np.arange(256) % 7) / 7.0, np.ones(256)], axis=1)
+    entries = np.array([0, 1, 17, 63, 64, 127, 128, 200, 253, 254])
+    x0 = (entries + 0.5) / 255.0
+    o = np.stack([x0, np.full_like(x0, -1.0), np.full_like(x0, 0.5)])
+    d = np.stack([np.zeros_like(x0), np.ones_like(x0), np.zeros_like(x0)])
+    for lo, hi in ((0.0, 1.0), (-2.0, 6.0)):
+        field = np.tile(lo + (hi - lo) * c, n * n)  # x fastest
+        vol = _vol(O, n, field, tf, "f64", use_shadow=False, sample_distance=0.01,
+                   scalar_range=(lo, hi))
+        rb = O.RayBuf(o, d, "f64")
+        O.render_rays(vol, rb)
+        assert np.all(rb.nsamples == 1)
+        np.testing.assert_allclose(rb.rgba, tf[entries], atol=1e-12)
+
+
 def test_calc_ray_start_slab(oracle_mod):
     """Entry/exit against the block bounds (FP64 twin vs the slab formula), misses flagged
     with MinDistance = -1, axis-parallel rays through rcp_safe."""
