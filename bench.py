@@ -401,10 +401,11 @@ def valu_issue(kernel, workload, kernel_ms):
             "kernel_ms": kernel_ms, "frac": issue_ms / kernel_ms}
 
 
-def cpu_baseline(sc, field32, gpu_map, threads, samples_per_frame):
+def cpu_baseline(sc, field32, gpu_map, threads, samples_per_frame, gpu_rgba=None):
     """Oracle (FP32 restatement of the reference) over the whole frame on the host cores:
     every map vertex (checked against the GPU map) and every camera ray, shaded against the
-    bit-identical GPU map.  The frame is a few seconds of work on 16 threads."""
+    bit-identical GPU map and compared with the GPU frame (gpu_rgba, W*H x 4).  The frame is
+    a few seconds of work on 16 threads."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -438,6 +439,8 @@ def cpu_baseline(sc, field32, gpu_map, threads, samples_per_frame):
                    f"({t_march:.2f} s, {nsamples} samples) = {frame_s:.2f} s"),
         "frame_s": frame_s,
         "samples_equal_gpu": nsamples == samples_per_frame,
+        "image_equal_gpu": (None if gpu_rgba is None
+                            else bool(np.array_equal(rb.rgba, gpu_rgba.reshape(-1, 4)))),
     }
 
 
@@ -574,8 +577,9 @@ def main():
         net.close()
     if not args.no_cpu:
         field32 = sc.field_store.numpy().astype(np.float32)
+        gimg, _ = r.RenderImage(sc.camera, sc.width, sc.height)  # this frame's RGBA on the host
         gmap = r.GetMap(sc.map_cells)
-        line["cpu_baseline"] = cpu_baseline(sc, field32, gmap, threads, cam_s)
+        line["cpu_baseline"] = cpu_baseline(sc, field32, gmap, threads, cam_s, gimg)
     r.close()
     print(json.dumps(line))
 
