@@ -499,6 +499,24 @@ def main():
             r.close()
 
     sc, r, sec_eager, sec, stats, cam_s, light_s = main_state
+    if not args.no_variants:
+        # SamplerCellAssoc: the same field as (n-1)^3 cell values, no shadow term
+        cells = np.ascontiguousarray(sc.field_store.numpy().reshape(n, n, n)[:-1, :-1, :-1]).ravel()
+        rgba = DeviceArray((img, img, 4), np.float32)
+        depth = DeviceArray((img, img), np.float32)
+        with B.LightedVolumeRenderer() as rc:
+            rc.SetData(cells, sc.dims, coords=sc.coords, dtype=B.F32, assoc=B.ASSOC_CELLS)
+            rc.SetColorMap(sc.tf32)
+            rc.AddLight(sc.light_pos, sc.light_color, 1.0, kind=sc.light_kind)
+            rc.SetShadow(False, sc.map_cells, B.F32)
+            rc.SetSampleDistance(sc.sample_distance)
+            csec, cstats, ccam, _ = frame_loop(rc, sc, rgba, depth, args.steps, args.warmup)
+        results["f32 cells, no shadow"] = dict(frame_ms=csec * 1e3, march_ms=cstats["march"],
+                                               samples_per_frame=ccam)
+        print(f"[bench] f32 cell field, no shadow: {csec * 1e3:.3f} ms/frame, {ccam} samples",
+              file=sys.stderr, flush=True)
+        rgba.free()
+        depth.free()
     npts = n ** 3
     nmap = int(np.prod([c + 1 for c in sc.map_cells]))
     npix = img * img
