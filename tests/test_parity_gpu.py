@@ -199,6 +199,27 @@ def test_rectilinear_coordinates(oracle_mod, light):
     assert rays.rgba[:, 3].max() > 0.1
 
 
+@pytest.mark.parametrize("dims,cells,light", [((37, 23, 51), (20, 13, 29), "point"),
+                                              ((37, 23, 51), (20, 13, 29), "directional"),
+                                              ((2, 2, 2), (1, 1, 1), "point")])
+def test_ragged_and_smallest_grids(oracle_mod, dims, cells, light):
+    """A field with a different point count per axis and a map with a different cell count
+    per axis (neither matching the field's), and the smallest grid: one cell, a one-cell
+    map.  Map and RGBA bit-exact."""
+    import dataclasses
+
+    sc = scenes.make_scene("ragged", 8, seed=29, nblobs=6, light=light, width=56, height=40)
+    f32 = B.gen_field(dims, seed=29, nblobs=6, noise_octaves=2, noise_amp=0.2, dtype=B.F32)
+    coords = [scenes.unit_coords(n)[0] for n in dims]
+    sc = dataclasses.replace(sc, dims=dims, coords=coords, field_store=f32,
+                             field32=np.asarray(f32, np.float32), map_cells=cells,
+                             sample_distance=float(np.float32(0.5 / (max(dims) - 1))))
+    o, d = B.camera_rays(B.orbit_camera(sc.bounds, 40.0, 20.0, 55.0), sc.width, sc.height)
+    rays, gmap, rb, omap = _render_both(oracle_mod, sc, o, d)
+    _assert_same(rays, gmap, rb, omap, f"ragged {dims} {cells} {light}")
+    assert (rays.min_d != -1).any()
+
+
 def test_set_field_rejects_grids_outside_the_kernels_domain():
     """Coordinates beyond 2^60, cells narrower than 2^-60 (the reciprocal-width division's
     range) and coordinate tables too large for LDS are refused with BEAMS_ERR_INVALID /
