@@ -1,6 +1,7 @@
 #include "Config.h"
 
 #include <cctype>
+#include <cmath>
 #include <fstream>
 #include <sstream>
 
@@ -165,6 +166,9 @@ Result Apply(Preset& p, const std::string& key, const std::string& raw, int line
   {
     if (!isArray || nums.size() % 2 != 0)
       return fail("expected [x0, alpha0, x1, alpha1, ...]");
+    for (double v : nums)
+      if (!std::isfinite(v))
+        return fail("point_alphas must be finite");
     p.ColorTable.PointAlphas.clear();
     for (size_t i = 0; i < nums.size(); i += 2)
       p.ColorTable.PointAlphas.push_back({ static_cast<float>(nums[i]), static_cast<float>(nums[i + 1]) });

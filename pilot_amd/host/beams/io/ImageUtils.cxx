@@ -1,6 +1,7 @@
 #include "ImageUtils.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -15,20 +16,19 @@ namespace
 {
 uint32_t Crc32(const uint8_t* data, size_t n, uint32_t crc = 0)
 {
-  // CRC-32 (ISO-HDLC, reflected polynomial 0xEDB88320), as PNG chunks carry it
-  static uint32_t table[256];
-  static bool init = false;
-  if (!init)
-  {
+  // CRC-32 (ISO-HDLC, reflected polynomial 0xEDB88320), as PNG chunks carry it; the table
+  // is a function-local static, built once even when frames are saved from several threads
+  static const std::array<uint32_t, 256> table = [] {
+    std::array<uint32_t, 256> t{};
     for (uint32_t i = 0; i < 256; ++i)
     {
       uint32_t c = i;
       for (int k = 0; k < 8; ++k)
         c = (c & 1u) ? (0xEDB88320u ^ (c >> 1)) : (c >> 1);
-      table[i] = c;
+      t[i] = c;
     }
-    init = true;
-  }
+    return t;
+  }();
   crc = ~crc;
   for (size_t i = 0; i < n; ++i)
     crc = table[(crc ^ data[i]) & 0xFFu] ^ (crc >> 8);

@@ -311,6 +311,7 @@ static void test_color_tables()
   CHECK(scene->ColorMap.size() == 1024 && scene->ColorMap[1023].W == 128.0f / 255.0f);
   CHECK(scene->ColorMap[0].X == 59.0f / 255.0f && scene->ColorMap[0].W == 0.0f);
   CHECK(!cfg.Deserialize("[presets.t]\npoint_alphas = [0.0, 0.0, 1.0]\n").Success);
+  CHECK(!cfg.Deserialize("[presets.t]\npoint_alphas = [0.0, nan, 1.0, 0.5]\n").Success);
   beams::Config named;
   CHECK(named.Deserialize("[presets.u]\ndataset = \"blobs\"\nsize = 8\ncolor_table = \"nope\"\n").Success);
   bool threw = false;
