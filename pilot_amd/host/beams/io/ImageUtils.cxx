@@ -1,6 +1,7 @@
 #include "ImageUtils.h"
 
 #include <algorithm>
+#include <cctype>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -68,7 +69,8 @@ bool Ends(const std::string& s, const std::string& suffix)
   if (s.size() < suffix.size())
     return false;
   std::string tail = s.substr(s.size() - suffix.size());
-  std::transform(tail.begin(), tail.end(), tail.begin(), ::tolower);
+  std::transform(tail.begin(), tail.end(), tail.begin(),
+                 [](unsigned char ch) { return static_cast<char>(std::tolower(ch)); });
   return tail == suffix;
 }
 } // namespace

@@ -1,6 +1,7 @@
 #include "ColorTable.h"
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 
 namespace beams
@@ -117,7 +118,8 @@ std::array<double, 3> DivergingInterpolate(const std::array<double, 3>& rgb1,
 ColorTable::ColorTable(const std::string& presetName)
 {
   std::string name = presetName;
-  std::transform(name.begin(), name.end(), name.begin(), ::tolower);
+  std::transform(name.begin(), name.end(), name.begin(),
+                 [](unsigned char ch) { return static_cast<char>(std::tolower(ch)); });
   if (name.empty() || name == "cool to warm")
   {
     this->Diverging = true;
