@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <limits>
 #include <sstream>
 
 namespace beams
@@ -495,6 +496,123 @@ std::vector<DataSet> ChunkDataSet(const DataSet& ds, int chunksX, int chunksY, i
     }
     out.push_back(std::move(c));
   }
+  return out;
+}
+
+namespace
+{
+// The source cell holding x on one axis (the last cell for x on the upper bound) and the
+// parametric coordinate in it; false when x lies outside beyond the tolerance.
+bool LocateAxis(const std::vector<float>& ax, double x, double tol, int64_t& c, double& t)
+{
+  const double lo = ax.front(), hi = ax.back();
+  if (x < lo - tol || x > hi + tol || !(x == x))
+    return false;
+  x = std::min(std::max(x, lo), hi);
+  const int64_t n = static_cast<int64_t>(ax.size());
+  if (n < 2)
+  {
+    c = 0;
+    t = 0.0;
+    return true;
+  }
+  const auto it = std::upper_bound(ax.begin(), ax.end(), static_cast<float>(x));
+  c = std::min<int64_t>(std::max<int64_t>((it - ax.begin()) - 1, 0), n - 2);
+  const double w = static_cast<double>(ax[c + 1]) - ax[c];
+  t = w > 0.0 ? (x - ax[c]) / w : 0.0;
+  return true;
+}
+} // namespace
+
+std::vector<DataSet> ResampleChunks(const DataSet& ds, int chunksX, int chunksY, int chunksZ)
+{
+  const std::array<int, 3> chunks = { chunksX, chunksY, chunksZ };
+  const auto& dims = ds.CellSet.PointDims;
+  std::array<float, 3> origin, chunkSize, spacing;
+  std::array<int64_t, 3> cdims, pdims;
+  std::array<double, 3> tol;
+  for (int d = 0; d < 3; ++d)
+  {
+    if (chunks[d] < 1 || dims[d] / chunks[d] < 1)
+      throw ErrorBadValue("ResampleChunks: each axis needs 1 <= chunks <= points");
+    const auto& ax = ds.Coords.Axes[d];
+    origin[d] = ax.front();
+    chunkSize[d] = (ax.back() - ax.front()) / static_cast<float>(chunks[d]);
+    cdims[d] = dims[d] / chunks[d];
+    pdims[d] = cdims[d] + 1;
+    spacing[d] = chunkSize[d] / static_cast<float>(cdims[d]);
+    tol[d] = 1e-6 * std::max(1e-30, static_cast<double>(ax.back()) - ax.front());
+  }
+  for (const auto& f : ds.Fields)
+    if (f.Device && f.Values.empty() && f.Half.empty())
+      throw ErrorBadValue("field " + f.Name + " lives in device memory only");
+  std::vector<DataSet> out;
+  for (int z = 0; z < chunksZ; ++z)
+    for (int y = 0; y < chunksY; ++y)
+      for (int x = 0; x < chunksX; ++x)
+      {
+        const std::array<int, 3> idx = { x, y, z };
+        DataSet c;
+        c.CellSet.PointDims = pdims;
+        for (int d = 0; d < 3; ++d)
+        {
+          const float o = origin[d] + static_cast<float>(idx[d]) * chunkSize[d];
+          c.Coords.Axes[d].resize(static_cast<size_t>(pdims[d]));
+          for (int64_t i = 0; i < pdims[d]; ++i)
+            c.Coords.Axes[d][static_cast<size_t>(i)] = o + spacing[d] * static_cast<float>(i);
+        }
+        for (const auto& f : ds.Fields)
+        {
+          Field g;
+          g.Name = f.Name;
+          g.Assoc = Association::Points;
+          g.Type = DType::F32;
+          g.Values.resize(static_cast<size_t>(pdims[0] * pdims[1] * pdims[2]));
+          const bool pts = f.IsPointField();
+          const int64_t sx = pts ? dims[0] : dims[0] - 1, sy = pts ? dims[1] : dims[1] - 1;
+          size_t o = 0;
+          for (int64_t k = 0; k < pdims[2]; ++k)
+            for (int64_t j = 0; j < pdims[1]; ++j)
+              for (int64_t i = 0; i < pdims[0]; ++i, ++o)
+              {
+                const std::array<int64_t, 3> q = { i, j, k };
+                std::array<int64_t, 3> cell;
+                std::array<double, 3> t;
+                bool found = true;
+                for (int d = 0; d < 3 && found; ++d)
+                  found = LocateAxis(ds.Coords.Axes[d], c.Coords.Axes[d][static_cast<size_t>(q[d])], tol[d],
+                                     cell[d], t[d]);
+                if (!found)
+                {
+                  g.Values[o] = std::numeric_limits<float>::quiet_NaN();
+                  continue;
+                }
+                auto at = [&](int64_t a, int64_t b, int64_t e) {
+                  return static_cast<double>(DecodeValue(f, static_cast<size_t>((e * sy + b) * sx + a)));
+                };
+                if (!pts)
+                {
+                  g.Values[o] = static_cast<float>(at(cell[0], cell[1], cell[2]));
+                  continue;
+                }
+                double v = 0.0;
+                for (int dz = 0; dz < 2; ++dz)
+                  for (int dy = 0; dy < 2; ++dy)
+                    for (int dx = 0; dx < 2; ++dx)
+                    {
+                      const int64_t a = std::min(cell[0] + dx, dims[0] - 1);
+                      const int64_t b = std::min(cell[1] + dy, dims[1] - 1);
+                      const int64_t e = std::min(cell[2] + dz, dims[2] - 1);
+                      const double w = (dx ? t[0] : 1.0 - t[0]) * (dy ? t[1] : 1.0 - t[1]) *
+                                       (dz ? t[2] : 1.0 - t[2]);
+                      v += w * at(a, b, e);
+                    }
+                g.Values[o] = static_cast<float>(v);
+              }
+          c.Fields.push_back(std::move(g));
+        }
+        out.push_back(std::move(c));
+      }
   return out;
 }
 

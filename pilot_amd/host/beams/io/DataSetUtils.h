@@ -51,9 +51,9 @@ struct DataSetUtils
 
 // convert's chunking (convert/Chunk.cxx:48-152) for the brick path: chunks x chunks y chunks z
 // blocks of a point grid as inclusive point ranges (i0,i1,j0,j1,k0,k1), neighbours sharing
-// their boundary plane, sizes differing by at most one cell per axis.  The reference resamples
-// each chunk onto its own grid (vtkm Probe); these ranges cut the grid itself, so no value is
-// interpolated.
+// their boundary plane, sizes differing by at most one cell per axis.  These ranges cut the
+// grid itself, so no value is interpolated; ResampleChunks below is the reference's own
+// resampled chunking.
 std::vector<std::array<int64_t, 6>> ChunkPointRanges(const std::array<int64_t, 3>& pointDims,
                                                      int chunksX, int chunksY, int chunksZ);
 
@@ -62,6 +62,17 @@ std::vector<std::array<int64_t, 6>> ChunkPointRanges(const std::array<int64_t, 3
 std::vector<DataSet> ChunkDataSet(const DataSet& dataSet, int chunksX, int chunksY, int chunksZ);
 WriteResult SaveChunksToDisk(const std::vector<DataSet>& chunks, const std::string& fileNamePrefix,
                              FileType fileType = FileType::BINARY);
+
+// convert's own chunking (convert/Chunk.cxx:48-152): chunk (x, y, z), x fastest, gets a
+// uniform grid of dims/chunks + 1 points per axis (dims = the source's point counts, integer
+// division), spacing = (bounds length / chunks) / (dims / chunks), origin = bounds minimum +
+// chunk index x bounds length / chunks, in FP32 as vtkm::Vec3f holds them.  Fields are probed
+// onto it (vtkm::filter::resampling::Probe): a point field trilinearly from the source cell
+// that holds the point, a cell field from that cell's value; FP32 output.  A point outside
+// the source bounds by more than 1e-6 of the extent gets NaN (Probe's invalid value); closer
+// ones are clamped onto the bounds.  VTK-m's locator tolerance and interpolation rounding are
+// not reproduced.
+std::vector<DataSet> ResampleChunks(const DataSet& dataSet, int chunksX, int chunksY, int chunksZ);
 } // namespace io
 } // namespace beams
 

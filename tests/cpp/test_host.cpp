@@ -10,6 +10,8 @@
 
 #include <unistd.h>
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -191,6 +193,74 @@ static float FrameValue(int x, int y, int c)
   return static_cast<float>((x * 37 + y * 101 + c * 53) % 256) / 255.0f;
 }
 
+// ResampleChunks: convert's chunk grids (dims / chunks cells per chunk, not the source's
+// spacing) probed from a linear field, which trilinear interpolation reproduces exactly.
+static void test_resample_chunks()
+{
+  namespace io = beams::io;
+  const std::array<int64_t, 3> dims = { 20, 14, 11 };
+  beams::DataSet ds;
+  ds.CellSet.PointDims = dims;
+  ds.Coords = beams::CoordinateSystem::Uniform(dims, { 0.0, 0.0, 0.0 },
+                                               { 1.0 / 19, 1.0 / 13, 1.0 / 10 });
+  auto lin = [](double x, double y, double z) { return 2.0 * x + 3.0 * y - z + 0.5; };
+  beams::Field pf, cf;
+  pf.Name = "lin";
+  cf.Name = "cellid";
+  cf.Assoc = beams::Association::Cells;
+  for (int64_t k = 0; k < dims[2]; ++k)
+    for (int64_t j = 0; j < dims[1]; ++j)
+      for (int64_t i = 0; i < dims[0]; ++i) {
+        pf.Values.push_back(static_cast<float>(lin(ds.Coords.Axes[0][i], ds.Coords.Axes[1][j], ds.Coords.Axes[2][k])));
+        if (i < dims[0] - 1 && j < dims[1] - 1 && k < dims[2] - 1)
+          cf.Values.push_back(static_cast<float>((k * (dims[1] - 1) + j) * (dims[0] - 1) + i));
+      }
+  ds.Fields = { pf, cf };
+  const auto chunks = io::ResampleChunks(ds, 2, 3, 2);
+  CHECK(chunks.size() == 12);
+  bool shape = true, linear = true, cells = true, finite = true;
+  double maxerr = 0.0;
+  for (size_t c = 0; c < chunks.size(); ++c) {
+    const auto& ch = chunks[c];
+    const int cx = static_cast<int>(c % 2), cy = static_cast<int>((c / 2) % 3), cz = static_cast<int>(c / 6);
+    shape &= ch.CellSet.PointDims == std::array<int64_t, 3>{ 11, 5, 6 };
+    // origin = chunk index x bounds / chunks; spacing = (1 / chunks) / (dims / chunks)
+    shape &= std::fabs(ch.Coords.Axes[0][0] - cx * 0.5f) < 1e-6f && std::fabs(ch.Coords.Axes[1][0] - cy / 3.0f) < 1e-6f &&
+             std::fabs(ch.Coords.Axes[2][0] - cz * 0.5f) < 1e-6f;
+    shape &= std::fabs(ch.Coords.Axes[0][1] - ch.Coords.Axes[0][0] - 0.05f) < 1e-6f &&
+             std::fabs(ch.Coords.Axes[1][1] - ch.Coords.Axes[1][0] - 1.0f / 12) < 1e-6f;
+    const auto& v = ch.GetField("lin").Values;
+    const auto& id = ch.GetField("cellid").Values;
+    size_t o = 0;
+    for (int64_t k = 0; k < 6; ++k)
+      for (int64_t j = 0; j < 5; ++j)
+        for (int64_t i = 0; i < 11; ++i, ++o) {
+          const double x = ch.Coords.Axes[0][i], y = ch.Coords.Axes[1][j], z = ch.Coords.Axes[2][k];
+          finite &= std::isfinite(v[o]);
+          maxerr = std::fmax(maxerr, std::fabs(v[o] - lin(std::fmin(x, 1.0), std::fmin(y, 1.0), std::fmin(z, 1.0))));
+          // the cell field takes the value of the source cell holding the point
+          const int64_t ci = std::min<int64_t>(static_cast<int64_t>(std::floor(x * 19 + 1e-9)), 18);
+          const int64_t cj = std::min<int64_t>(static_cast<int64_t>(std::floor(y * 13 + 1e-9)), 12);
+          const int64_t ck = std::min<int64_t>(static_cast<int64_t>(std::floor(z * 10 + 1e-9)), 9);
+          const float want = static_cast<float>((ck * 13 + cj) * 19 + ci);
+          cells &= id[o] == want || std::fabs(x * 19 - std::round(x * 19)) < 1e-4 ||
+                   std::fabs(y * 13 - std::round(y * 13)) < 1e-4 || std::fabs(z * 10 - std::round(z * 10)) < 1e-4;
+        }
+  }
+  linear = maxerr < 2e-5;
+  CHECK(shape);
+  CHECK(finite);
+  CHECK(linear);
+  CHECK(cells);
+  bool threw = false;
+  try {
+    io::ResampleChunks(ds, 21, 1, 1);
+  } catch (const beams::ErrorBadValue&) {
+    threw = true;
+  }
+  CHECK(threw);
+}
+
 static void test_images()
 {
   namespace io = beams::io;
@@ -325,6 +395,7 @@ static void test_color_tables()
 
 static void test_cpu()
 {
+  test_resample_chunks();
   test_images();
   test_color_tables();
 
