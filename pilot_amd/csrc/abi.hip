@@ -63,6 +63,8 @@ struct beams_renderer {
   int32_t dtype = -1, assoc = 0;
   int64_t dims[3] = { 0, 0, 0 };
   std::vector<float> hx[3];
+  bool use_bins = false;  // uneven coordinates: bin tables follow the reciprocals
+  float binv[3] = { 0.0f, 0.0f, 0.0f };
   DevBuf coords;
   double bounds[6] = { 0, 0, 0, 0, 0, 0 };
   bool has_field = false;
@@ -187,6 +189,8 @@ GridParams grid_params(const beams_renderer* h)
     g.ginv[d] = (float)((double)(h->dims[d] - 1) / ((double)x.back() - (double)x.front()));
   }
   std::memcpy(g.bounds, h->bounds, sizeof(g.bounds));
+  g.use_bins = h->use_bins ? 1 : 0;
+  for (int d = 0; d < 3; ++d) g.binv[d] = h->binv[d];
   // Math.h:12-20 BoundsMagnitude<Float32>
   const float e0 = (float)(h->bounds[1] - h->bounds[0]);
   const float e1 = (float)(h->bounds[3] - h->bounds[2]);
@@ -963,9 +967,41 @@ int beams_set_field(beams_renderer* h, const void* values, int32_t dtype, int32_
     }
     flat.push_back(0.0f);
   }
-  HIP_TRY(h, h->coords.ensure(sizeof(float) * 2 * nc));
-  HIP_TRY(h, hipMemcpyAsync(h->coords.p, flat.data(), sizeof(float) * 2 * nc, hipMemcpyHostToDevice, h->stream));
+  // Uneven coordinates (some cell not where the uniform-spacing guess puts it): per axis,
+  // n-1 uniform bins, each holding the cell its lower edge lies in, so a guess lands within a
+  // cell or two of the answer.  Staged in LDS beside the coordinates, so only up to 8192
+  // points in all; beyond that the kernels walk from the uniform guess.
+  bool uneven = false;
+  float binv[3] = { 0.0f, 0.0f, 0.0f };
+  for (int d = 0; d < 3; ++d) {
+    const double x0 = x[d].front(), span = (double)x[d].back() - x0;
+    const float ginv = (float)((double)(dims[d] - 1) / span);
+    binv[d] = ginv;
+    for (size_t c = 0; c + 1 < x[d].size() && !uneven; ++c) {
+      const float mid = (float)(0.5 * ((double)x[d][c] + (double)x[d][c + 1]));
+      uneven = (int64_t)((mid - (float)x0) * ginv) != (int64_t)c;
+    }
+  }
+  const bool use_bins = uneven && nc <= 8192;
+  if (use_bins)
+    for (int d = 0; d < 3; ++d) {
+      const int64_t ncell = dims[d] - 1;
+      for (int64_t b = 0; b < dims[d]; ++b) {
+        const double edge = (double)x[d].front() + (double)std::min(b, ncell - 1) / (double)binv[d];
+        int64_t c = (int64_t)(std::upper_bound(x[d].begin(), x[d].end(), (float)edge) - x[d].begin()) - 1;
+        c = std::min(std::max<int64_t>(c, 0), ncell - 1);
+        const int32_t ci = (int32_t)c;
+        float bits;
+        std::memcpy(&bits, &ci, 4);
+        flat.push_back(bits);
+      }
+    }
+  const size_t words = (use_bins ? 3 : 2) * nc;
+  HIP_TRY(h, h->coords.ensure(sizeof(float) * words));
+  HIP_TRY(h, hipMemcpyAsync(h->coords.p, flat.data(), sizeof(float) * words, hipMemcpyHostToDevice, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->use_bins = use_bins;
+  for (int d = 0; d < 3; ++d) h->binv[d] = binv[d];
   for (int d = 0; d < 3; ++d) {
     h->hx[d] = std::move(x[d]);
     h->dims[d] = dims[d];

@@ -22,6 +22,11 @@ struct GridParams {
   double bounds[6];         // FP64 bounds of the coordinates
   float ext_mag;            // |extent| in FP32 (BoundsMagnitude)
   uint32_t bytes32;         // bytes from field to the end of its array when below 2^32, else 0
+  // Uneven coordinates: a bin table per axis (n-1 uniform bins over the axis, each holding the
+  // cell its lower edge lies in) follows the reciprocals in the coordinate array, at the
+  // coordinates' own offsets plus 2 (n0+n1+n2); binv = bins per unit length.
+  int32_t use_bins;
+  float binv[3];
 };
 
 // Transmittance (opacity) map over the block bounds.

@@ -11,7 +11,7 @@ namespace {
 constexpr int kBlock = 256;
 
 // ---- TransmittanceMap.h:239-340 (TransmittanceMapGenerator over all map vertices) -------
-template <int FDT, int MDT>
+template <int FDT, int MDT, bool BINS>
 __global__ void __launch_bounds__(kBlock) k_tmap_ref(GridParams g, MapParams m, ShadeParams s,
                                                      const float* seed, int64_t nv)
 {
@@ -20,17 +20,18 @@ __global__ void __launch_bounds__(kBlock) k_tmap_ref(GridParams g, MapParams m, 
   float* xs = lds + s.ntf;     // coordinates
   for (int i = threadIdx.x; i < s.ntf; i += blockDim.x) a_tab[i] = s.tf[4 * i + 3] / s.max_density;
   stage_coords(g, xs);
+  if (BINS) stage_bins(g, xs);
   __syncthreads();
   const float* x[3] = { xs, xs + g.n[0], xs + g.n[0] + g.n[1] };
   const int64_t id = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (id >= nv) return;
   int64_t taken = 0;
-  const float op = march_vertex<FDT>(g, m, s, a_tab, x, id, seed ? seed[id] : 0.0f, &taken);
+  const float op = march_vertex<FDT, BINS>(g, m, s, a_tab, x, id, seed ? seed[id] : 0.0f, &taken);
   store_t<MDT>(m.map, id, op);
   if (s.counters) count_add(&s.counters[1], (uint32_t)taken);
 }
 
-template <int FDT, int MDT, bool USE_MAP>
+template <int FDT, int MDT, bool USE_MAP, bool BINS>
 __global__ void __launch_bounds__(kBlock) k_march_rays(GridParams g, MapParams m, ShadeParams s,
                                                        RaySoA r)
 {
@@ -39,6 +40,7 @@ __global__ void __launch_bounds__(kBlock) k_march_rays(GridParams g, MapParams m
   float* xs = (float*)(lds4 + s.ntf);
   stage_tf(s, tf);
   stage_coords(g, xs);
+  if (BINS) stage_bins(g, xs);
   __syncthreads();
   const float* x[3] = { xs, xs + g.n[0], xs + g.n[0] + g.n[1] };
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -52,14 +54,14 @@ __global__ void __launch_bounds__(kBlock) k_march_rays(GridParams g, MapParams m
   r.dist[i] = dist;
   float4 c4 = ((float4*)r.rgba)[i];
   float color[4] = { c4.x, c4.y, c4.z, c4.w };
-  shade<FDT, MDT, USE_MAP>(g, m, s, tf, x, org, dir, minD, maxD, color);
+  shade<FDT, MDT, USE_MAP, BINS>(g, m, s, tf, x, org, dir, minD, maxD, color);
   ((float4*)r.rgba)[i] = make_float4(color[0], color[1], color[2], color[3]);
 }
 
 // 16x16-pixel workgroups made of four 8x8 wave64 tiles; workgroup ids are remapped
 // so that each of the 8 XCDs (which receive workgroups round-robin) marches one
 // contiguous band of tiles and keeps that band's bricks in its own L2.
-template <int FDT, int MDT, bool USE_MAP>
+template <int FDT, int MDT, bool USE_MAP, bool BINS>
 __global__ void __launch_bounds__(kBlock) k_march_image(GridParams g, MapParams m, ShadeParams s,
                                                         CameraParams cam, ImageOut out,
                                                         int32_t tiles_x, int32_t ntiles,
@@ -70,6 +72,7 @@ __global__ void __launch_bounds__(kBlock) k_march_image(GridParams g, MapParams 
   float* xs = (float*)(lds4 + s.ntf);
   stage_tf(s, tf);
   stage_coords(g, xs);
+  if (BINS) stage_bins(g, xs);
   __syncthreads();
   const int32_t logical = (int32_t)(blockIdx.x % 8u) * per_xcd + (int32_t)(blockIdx.x / 8u);
   if (logical >= ntiles) return;
@@ -83,7 +86,7 @@ __global__ void __launch_bounds__(kBlock) k_march_image(GridParams g, MapParams 
   float minD = cam.near_d, maxD = cam.far_d, dist = 0.0f;
   calc_ray_start(g, org, dir, minD, dist, maxD);
   float color[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
-  shade<FDT, MDT, USE_MAP>(g, m, s, tf, x, org, dir, minD, maxD, color);
+  shade<FDT, MDT, USE_MAP, BINS>(g, m, s, tf, x, org, dir, minD, maxD, color);
   if (out.use_bg) {
     const float ia = 1.0f - color[3];
     color[0] = color[0] + out.bg[0] * ia;
@@ -102,7 +105,11 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
 }
 
 // coordinates and reciprocal cell widths (stage_coords)
-size_t lds_bytes_coords(const GridParams& g) { return 2 * sizeof(float) * (g.n[0] + g.n[1] + g.n[2]); }
+// coordinates and reciprocal cell widths (stage_coords), then the bin tables (stage_bins)
+size_t lds_bytes_coords(const GridParams& g)
+{
+  return (g.use_bins ? 3 : 2) * sizeof(float) * (g.n[0] + g.n[1] + g.n[2]);
+}
 }  // namespace
 
 // ---- launchers -----------------------------------------------------------------------------
@@ -119,7 +126,10 @@ hipError_t launch_tmap_ref(const GridParams& g, const MapParams& m, const ShadeP
   const size_t lds = sizeof(float) * s.ntf + lds_bytes_coords(g);
   const dim3 grid((unsigned)((nv + kBlock - 1) / kBlock));
   BEAMS_DISPATCH_FIELD(g.dtype, BEAMS_DISPATCH_MAP(m.dtype, {
-    hipLaunchKernelGGL((k_tmap_ref<FDT, MDT>), grid, dim3(kBlock), lds, st, g, m, s, seed, nv);
+    if (g.use_bins)
+      hipLaunchKernelGGL((k_tmap_ref<FDT, MDT, true>), grid, dim3(kBlock), lds, st, g, m, s, seed, nv);
+    else
+      hipLaunchKernelGGL((k_tmap_ref<FDT, MDT, false>), grid, dim3(kBlock), lds, st, g, m, s, seed, nv);
   }));
   return hipGetLastError();
 }
@@ -131,10 +141,14 @@ hipError_t launch_march_rays(const GridParams& g, const MapParams& m, const Shad
   const size_t lds = sizeof(float4) * s.ntf + lds_bytes_coords(g);
   const dim3 grid((unsigned)((r.n + kBlock - 1) / kBlock));
   BEAMS_DISPATCH_FIELD(g.dtype, BEAMS_DISPATCH_MAP(m.dtype, {
-    if (s.use_map)
-      hipLaunchKernelGGL((k_march_rays<FDT, MDT, true>), grid, dim3(kBlock), lds, st, g, m, s, r);
+    if (s.use_map && g.use_bins)
+      hipLaunchKernelGGL((k_march_rays<FDT, MDT, true, true>), grid, dim3(kBlock), lds, st, g, m, s, r);
+    else if (s.use_map)
+      hipLaunchKernelGGL((k_march_rays<FDT, MDT, true, false>), grid, dim3(kBlock), lds, st, g, m, s, r);
+    else if (g.use_bins)
+      hipLaunchKernelGGL((k_march_rays<FDT, MDT, false, true>), grid, dim3(kBlock), lds, st, g, m, s, r);
     else
-      hipLaunchKernelGGL((k_march_rays<FDT, MDT, false>), grid, dim3(kBlock), lds, st, g, m, s, r);
+      hipLaunchKernelGGL((k_march_rays<FDT, MDT, false, false>), grid, dim3(kBlock), lds, st, g, m, s, r);
   }));
   return hipGetLastError();
 }
@@ -148,12 +162,18 @@ hipError_t launch_march_image(const GridParams& g, const MapParams& m, const Sha
   const dim3 grid((unsigned)(per_xcd * 8));
   const size_t lds = sizeof(float4) * s.ntf + lds_bytes_coords(g);
   BEAMS_DISPATCH_FIELD(g.dtype, BEAMS_DISPATCH_MAP(m.dtype, {
-    if (s.use_map)
-      hipLaunchKernelGGL((k_march_image<FDT, MDT, true>), grid, dim3(kBlock), lds, st, g, m, s, c,
-                         o, tiles_x, ntiles, per_xcd);
-    else
-      hipLaunchKernelGGL((k_march_image<FDT, MDT, false>), grid, dim3(kBlock), lds, st, g, m, s,
+    if (s.use_map && g.use_bins)
+      hipLaunchKernelGGL((k_march_image<FDT, MDT, true, true>), grid, dim3(kBlock), lds, st, g, m, s,
                          c, o, tiles_x, ntiles, per_xcd);
+    else if (s.use_map)
+      hipLaunchKernelGGL((k_march_image<FDT, MDT, true, false>), grid, dim3(kBlock), lds, st, g, m, s,
+                         c, o, tiles_x, ntiles, per_xcd);
+    else if (g.use_bins)
+      hipLaunchKernelGGL((k_march_image<FDT, MDT, false, true>), grid, dim3(kBlock), lds, st, g, m, s,
+                         c, o, tiles_x, ntiles, per_xcd);
+    else
+      hipLaunchKernelGGL((k_march_image<FDT, MDT, false, false>), grid, dim3(kBlock), lds, st, g, m,
+                         s, c, o, tiles_x, ntiles, per_xcd);
   }));
   return hipGetLastError();
 }

@@ -258,12 +258,22 @@ struct AxisCell {
   int c;
   float lo, hi, rw;
 };
+// BINS: the guess comes from the axis's bin table (int entries at x + 2 rofs, staged with the
+// coordinates) and ginv is the bins per unit length; the walk that follows corrects any guess.
+template <bool BINS = false>
 __device__ __forceinline__ AxisCell axis_guess(const float* x, int rofs, int ncell, float v,
                                                float x0, float ginv)
 {
   AxisCell a;
   // clamp (med3; v is finite inside the bounds, so the guess is never NaN)
   a.c = (int)__builtin_amdgcn_fmed3f((v - x0) * ginv, 0.0f, (float)(ncell - 1));
+  if (BINS) {
+    // the bin's first cell, then up to two cells on without a branch: a bin spans at most
+    // three cells where widths stay within twice the mean; the callers' checks settle the rest
+    a.c = min(max(((const int*)(x + 2 * rofs))[a.c], 0), ncell - 1);
+    a.c += (a.c < ncell - 1 && v > x[a.c + 1]) ? 1 : 0;
+    a.c += (a.c < ncell - 1 && v > x[a.c + 1]) ? 1 : 0;
+  }
   a.lo = x[a.c];
   a.hi = x[a.c + 1];
   a.rw = x[rofs + a.c];
@@ -284,10 +294,11 @@ __device__ __forceinline__ void oracle_settle(const float* x, int rofs, int ncel
 }
 
 // walk_cell32's cell (the camera march): v in [x[c], x[c+1]).
+template <bool BINS = false>
 __device__ __forceinline__ AxisCell walk_axis(const float* x, int rofs, int ncell, float v,
                                               float x0, float ginv)
 {
-  AxisCell a = axis_guess(x, rofs, ncell, v, x0, ginv);
+  AxisCell a = axis_guess<BINS>(x, rofs, ncell, v, x0, ginv);
   if (__builtin_expect(!(v >= a.lo && v < a.hi), 0)) {
     int c = a.c;
     while (c > 0 && v < x[c]) --c;

@@ -30,6 +30,16 @@ __device__ __forceinline__ void stage_coords(const GridParams& g, float* xs)
 
 __device__ __forceinline__ int coords_rofs(const GridParams& g) { return (int)(g.n[0] + g.n[1] + g.n[2]); }
 
+// The bin tables of uneven coordinates (GridParams::use_bins) into LDS after the reciprocals:
+// xs[2 nc .. 3 nc) as ints, at the coordinates' offsets.
+__device__ __forceinline__ void stage_bins(const GridParams& g, float* xs)
+{
+  const int64_t nc = g.n[0] + g.n[1] + g.n[2];
+  const int* src = (const int*)(g.x[0] + 2 * nc);
+  int* dst = (int*)(xs + 2 * nc);
+  for (int64_t i = threadIdx.x; i < nc; i += blockDim.x) dst[i] = src[i];
+}
+
 // Adds the counts of the lanes active here to *ctr with one atomic per wave: the active lanes'
 // values are summed through readlane (a wave-uniform loop over the ballot), then the first
 // active lane adds the sum.  Per-lane atomics on one address serialise in L2 (about 10 ns
@@ -49,7 +59,7 @@ __device__ __forceinline__ void count_add(unsigned long long* ctr, uint32_t v)
 // ---- TransmittanceMap.h:279-331: one map vertex's light march ----------------------------
 // a_tab: alpha / max alpha per TF entry; x: the block's coordinates (LDS).  Returns the
 // opacity; *taken receives the number of steps inside the volume.
-template <int FDT>
+template <int FDT, bool BINS = false>
 __device__ __forceinline__ float march_vertex(const GridParams& g, const MapParams& m,
                                               const ShadeParams& s, const float* a_tab,
                                               const float* const* x, int64_t id, float seed_value,
@@ -89,9 +99,9 @@ __device__ __forceinline__ float march_vertex(const GridParams& g, const MapPara
     if (!inside(g, p)) break;
     ++taken;
     // FindCell and the parametric coordinates (p - x[c]) / (x[c+1] - x[c])
-    AxisCell ax = axis_guess(x[0], rofs, c0, p[0], g.gx0[0], g.ginv[0]);
-    AxisCell ay = axis_guess(x[1], rofs, c1, p[1], g.gx0[1], g.ginv[1]);
-    AxisCell az = axis_guess(x[2], rofs, c2, p[2], g.gx0[2], g.ginv[2]);
+    AxisCell ax = axis_guess<BINS>(x[0], rofs, c0, p[0], g.gx0[0], BINS ? g.binv[0] : g.ginv[0]);
+    AxisCell ay = axis_guess<BINS>(x[1], rofs, c1, p[1], g.gx0[1], BINS ? g.binv[1] : g.ginv[1]);
+    AxisCell az = axis_guess<BINS>(x[2], rofs, c2, p[2], g.gx0[2], BINS ? g.binv[2] : g.ginv[2]);
     // all three tests evaluated (no short circuit: the reads above are issued together)
     const bool settled = ((int)oracle_in(ax, p[0]) & (int)oracle_in(ay, p[1]) & (int)oracle_in(az, p[2])) != 0;
     if (__builtin_expect(!settled, 0)) {
@@ -115,7 +125,7 @@ __device__ __forceinline__ float march_vertex(const GridParams& g, const MapPara
 }
 
 // ---- LightedVolumeRenderer.cxx:345-498 ----------------------------------------------------
-template <int FDT, int MDT, bool USE_MAP>
+template <int FDT, int MDT, bool USE_MAP, bool BINS = false>
 __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapParams& m,
                                              const ShadeParams& s, const float4* tf,
                                              const float* const* x, const float* org,
@@ -164,7 +174,8 @@ __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapPa
           bl[d] = x[d][cell[d]];
         } else {
           // the cell and 1 / (x[c+1] - x[c]) (the staged IEEE quotient)
-          const AxisCell a = walk_axis(x[d], rofs, (int)g.n[d] - 1, p[d], g.gx0[d], g.ginv[d]);
+          const AxisCell a = walk_axis<BINS>(x[d], rofs, (int)g.n[d] - 1, p[d], g.gx0[d],
+                                             BINS ? g.binv[d] : g.ginv[d]);
           invsp[d] = a.rw;
           cell[d] = a.c;
           bl[d] = a.lo;
@@ -318,7 +329,7 @@ __device__ __forceinline__ void stage_tf(const ShadeParams& s, float4* tf)
     tf[i] = make_float4(s.tf[4 * i], s.tf[4 * i + 1], s.tf[4 * i + 2], s.tf[4 * i + 3]);
 }
 
-template <int FDT, int MDT, bool USE_MAP>
+template <int FDT, int MDT, bool USE_MAP, bool BINS = false>
 __device__ __forceinline__ void shade(const GridParams& g, const MapParams& m,
                                       const ShadeParams& s, const float4* tf,
                                       const float* const* x, const float* org, const float* dir,
@@ -326,7 +337,7 @@ __device__ __forceinline__ void shade(const GridParams& g, const MapParams& m,
 {
   const int64_t n = (g.assoc == 1)
                       ? sample_cell<FDT>(g, s, tf, x, org, dir, minD, maxD, color)
-                      : sample_point<FDT, MDT, USE_MAP>(g, m, s, tf, x, org, dir, minD, maxD, color);
+                      : sample_point<FDT, MDT, USE_MAP, BINS>(g, m, s, tf, x, org, dir, minD, maxD, color);
   if (s.counters) count_add(&s.counters[0], (uint32_t)n);
 }
 
