@@ -220,6 +220,29 @@ def test_ragged_and_smallest_grids(oracle_mod, dims, cells, light):
     assert (rays.min_d != -1).any()
 
 
+@pytest.mark.parametrize("spread", [3.0, 100.0])
+def test_uneven_coordinates_bin_guess(oracle_mod, spread):
+    """Cell widths spread over a factor of 3 or 100 (log-uniform) on x and z, y uniform: the
+    bin-table guess plus its two branch-free steps lands on the cell, or short of it when a
+    bin spans many narrow cells, and the exact walk finishes; map and RGBA bit-exact."""
+    import dataclasses
+
+    sc = scenes.make_scene("uneven", 28, seed=31, nblobs=8, light="point", width=64, height=48)
+    rng = np.random.default_rng(int(spread))
+    coords = []
+    for d in range(3):
+        if d == 1:
+            coords.append(sc.coords[1])
+            continue
+        w = np.exp(rng.uniform(0.0, np.log(spread), size=sc.dims[d] - 1))
+        coords.append((np.concatenate([[0.0], np.cumsum(w)]) / w.sum()).astype(np.float32))
+    sc = dataclasses.replace(sc, coords=coords)
+    o, d = B.camera_rays(B.orbit_camera(sc.bounds, 30.0, 15.0, 50.0), sc.width, sc.height)
+    rays, gmap, rb, omap = _render_both(oracle_mod, sc, o, d)
+    _assert_same(rays, gmap, rb, omap, f"uneven coordinates x{spread}")
+    assert rays.rgba[:, 3].max() > 0.1
+
+
 def test_set_field_rejects_grids_outside_the_kernels_domain():
     """Coordinates beyond 2^60, cells narrower than 2^-60 (the reciprocal-width division's
     range) and coordinate tables too large for LDS are refused with BEAMS_ERR_INVALID /
