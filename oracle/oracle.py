@@ -60,32 +60,38 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
-        for tw in ("f32", "f64"):
-            getattr(L, f"orc_segment_aabb_{tw}").restype = C.c_int
-            getattr(L, f"orc_find_cell_{tw}").restype = C.c_int64
-            getattr(L, f"orc_interp_hex_{tw}").restype = C.c_double
-            getattr(L, f"orc_nonlocal_hits_{tw}").restype = C.c_int64
-        VP, I32, I64, D = C.c_void_p, C.c_int, C.c_int64, C.c_double
-        for tw in ("f32", "f64"):
-            sig = {
-                "orc_segment_aabb": [VP, VP, VP, D, VP, VP],
-                "orc_find_cell": [VP, VP, VP],
-                "orc_interp_hex": [VP, VP],
-                "orc_tmap_build": [VP, VP, VP, I32],
-                "orc_tmap_build_range": [VP, VP, I64, I64, I32],
-                "orc_map_estimate": [VP, VP, VP, VP, VP],
-                "orc_calc_ray_start": [VP, VP, I32],
-                "orc_sample_rays": [VP, VP, VP, I32],
-                "orc_render_rays": [VP, VP, VP, I32],
-                "orc_nonlocal_hits": [I32, VP, I32, VP, VP, VP, I64],
-                "orc_nonlocal_seed": [I32, VP, VP, I32, VP, I32],
-                "orc_composite": [I32, VP, VP, VP, I64, VP],
-            }
-            for name, args in sig.items():
-                getattr(L, f"{name}_{tw}").argtypes = args
-        _lib = L
+        _lib = load(LIB)
     return _lib
+
+
+def load(path):
+    """A library built from oracle.c at `path` (the oracle itself, or a copy the tests build
+    with a planted mistake to show that a pin catches it), with its signatures declared."""
+    L = C.CDLL(path)
+    for tw in ("f32", "f64"):
+        getattr(L, f"orc_segment_aabb_{tw}").restype = C.c_int
+        getattr(L, f"orc_find_cell_{tw}").restype = C.c_int64
+        getattr(L, f"orc_interp_hex_{tw}").restype = C.c_double
+        getattr(L, f"orc_nonlocal_hits_{tw}").restype = C.c_int64
+    VP, I32, I64, D = C.c_void_p, C.c_int, C.c_int64, C.c_double
+    for tw in ("f32", "f64"):
+        sig = {
+            "orc_segment_aabb": [VP, VP, VP, D, VP, VP],
+            "orc_find_cell": [VP, VP, VP],
+            "orc_interp_hex": [VP, VP],
+            "orc_tmap_build": [VP, VP, VP, I32],
+            "orc_tmap_build_range": [VP, VP, I64, I64, I32],
+            "orc_map_estimate": [VP, VP, VP, VP, VP],
+            "orc_calc_ray_start": [VP, VP, I32],
+            "orc_sample_rays": [VP, VP, VP, I32],
+            "orc_render_rays": [VP, VP, VP, I32],
+            "orc_nonlocal_hits": [I32, VP, I32, VP, VP, VP, I64],
+            "orc_nonlocal_seed": [I32, VP, VP, I32, VP, I32],
+            "orc_composite": [I32, VP, VP, VP, I64, VP],
+        }
+        for name, args in sig.items():
+            getattr(L, f"{name}_{tw}").argtypes = args
+    return L
 
 
 def _dt(twin):

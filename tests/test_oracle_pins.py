@@ -518,3 +518,320 @@ def test_composite_depth_order_and_over(oracle_mod):
     perm = rng.permutation(nb)
     out2 = O.composite(np.array(bounds)[perm].ravel(), cam, [imgs[p] for p in perm], "f64")
     assert np.allclose(out2, out, atol=1e-14)
+
+
+# ---- helpers for the pins below ----------------------------------------------------------------
+# Each pin is a function of the oracle module so that tests/test_oracle_mutants.py can run it
+# against copies of the oracle with a planted mistake and show that it fails there.
+
+def _padded(a, pad, fill):
+    """a inside a larger buffer, pad elements of `fill` on each side: (buffer, address of a's
+    first element).  A planted out-of-range index then reads the fill, not foreign memory."""
+    a = np.ascontiguousarray(a).ravel()
+    buf = np.full(a.size + 2 * pad, fill, a.dtype)
+    buf[pad:pad + a.size] = a
+    return buf, buf.ctypes.data + pad * buf.itemsize
+
+
+def _uneven(n, rng, lo=0.3, hi=1.0):
+    """n increasing coordinates from 0 to 1 with random cell widths (ratio up to hi/lo)."""
+    c = np.concatenate([[0.0], np.cumsum(rng.uniform(lo, hi, n - 1))])
+    return c / c[-1]
+
+
+def _axis_ray(axis, sign, other):
+    """A ray along +/- axis through the point `other` (the two other coordinates, inside the
+    unit box), starting one unit outside the box."""
+    o = np.zeros(3)
+    d = np.zeros(3)
+    rest = [a for a in range(3) if a != axis]
+    o[rest[0]], o[rest[1]] = other
+    o[axis] = -1.0 if sign > 0 else 2.0
+    d[axis] = float(sign)
+    return o, d
+
+
+def _distinct_tf(alpha):
+    """256 entries with distinct colours (red i/255, green 1 - i/255, blue (i mod 7)/7)."""
+    i = np.arange(256)
+    return np.stack([i / 255.0, 1 - i / 255.0, (i % 7) / 7.0, np.full(256, alpha)], axis=1)
+
+
+# ---- TransmittanceLocator + GetEstimateUsingVertices (TransmittanceMap.h:90-112,166-200) --------
+
+def pin_map_lookup(O):
+    """The map lookup at points in and around a ragged map (5 x 3 x 7 cells) over non-unit
+    bounds.  Vertex opacities are affine + a quadratic per axis + cross terms xy, yz, xz, xyz
+    with distinct coefficients.  Trilinear interpolation reproduces the affine and cross terms
+    exactly in any cell, and a quadratic term u^2 as the chord (a + b) u - a b of the cell
+    [a, b] the lookup interpolates in: a swapped corner, a wrong stride or spacing, or a wrong
+    cell (the clamps included: points up to 1.5 cells outside the map use its end cells, a
+    point on the max face its last cell) changes the value.  At a vertex the lookup returns
+    the vertex's own value exactly.  The estimate is (1 - opacity) times the light colour."""
+    import ctypes as C
+
+    coords = [np.linspace(-0.3, 1.7, 6), np.linspace(0.2, 0.9, 5), np.linspace(-1.0, 2.0, 4)]
+    cells = (5, 3, 7)
+    lc = np.array([1.5, 2.0, 0.5])
+    vol = _vol(O, coords=coords, twin="f64", map_cells=cells, light_color=tuple(lc))
+    lo = np.array([c[0] for c in coords])
+    hi = np.array([c[-1] for c in coords])
+    sp = (hi - lo) / np.array(cells)
+    V = [lo[d] + sp[d] * np.arange(cells[d] + 1) for d in range(3)]
+
+    def value(x, y, z, q=None):
+        qx, qy, qz = (x * x, y * y, z * z) if q is None else q
+        return (0.7 * x - 1.3 * y + 0.45 * z + 2.0 * qx - 0.8 * qy + 1.1 * qz + 0.6 * x * y
+                - 0.9 * y * z + 0.35 * x * z + 0.25 * x * y * z + 0.1)
+
+    X, Y, Z = np.meshgrid(*V, indexing="ij")
+    m = value(X, Y, Z).transpose(2, 1, 0).ravel()  # x fastest
+    _buf, mptr = _padded(m, 64, np.nan)
+    rng = np.random.default_rng(21)
+    pts = [rng.uniform(lo - 1.5 * sp, hi + 1.5 * sp) for _ in range(400)]
+    verts = [(0, 0, 0), (5, 3, 7), (2, 1, 4), (5, 0, 7), (0, 3, 0), (3, 2, 6)]
+    pts += [np.array([V[0][i], V[1][j], V[2][k]]) for i, j, k in verts]
+    L = O.lib()
+    for n, p in enumerate(pts):
+        p = np.ascontiguousarray(p, np.float64)
+        rgb, op = np.zeros(3), np.zeros(1)
+        L.orc_map_estimate_f64(C.byref(vol.s), mptr, p.ctypes.data, rgb.ctypes.data, op.ctypes.data)
+        q = []
+        for d in range(3):
+            c = int(np.clip(np.floor((p[d] - lo[d]) / sp[d]), 0, cells[d] - 1))
+            a, b = V[d][c], V[d][c + 1]
+            q.append((a + b) * p[d] - a * b)
+        expect = value(*p, q=q)
+        assert abs(op[0] - expect) <= 1e-12 * (1 + abs(expect)), (p, op[0], expect)
+        assert np.all(np.abs(rgb - (1 - expect) * lc) <= 1e-11 * (1 + abs(expect))), (p, rgb)
+        if n >= 400:  # a vertex: its own value
+            i, j, k = verts[n - 400]
+            assert op[0] == m[(k * 4 + j) * 6 + i]
+
+
+def test_map_lookup_non_constant_map(oracle_mod):
+    pin_map_lookup(oracle_mod)
+
+
+# ---- SamplerCellAssoc (LightedVolumeRenderer.cxx:501-642) ----------------------------------------
+
+def pin_sample_cell_constant(O):
+    """Constant cell field and TF (r, g, b, a): A_n = 1 - (1 - a)^n with n the samples between
+    the slab entry (+ mesh epsilon) and exit, RGB = min(c A_n, 1) (the final clamp: blue is 1.15
+    times A_n before it), and no shadow term although a light is set."""
+    rng = np.random.default_rng(14)
+    a = 0.02
+    c = np.array([0.3, 0.6, 1.15])
+    tf = np.tile(np.r_[c, a], (256, 1))
+    sd = 0.011
+    vol = _vol(O, 9, np.full(512, 0.5), tf, "f64", use_shadow=True, sample_distance=sd, assoc=1,
+               light_color=(0.1, 0.1, 0.1))
+    o, d = _rays_into_box(400, rng, "f64")
+    rb = O.RayBuf(o, d, "f64")
+    O.sample_rays(vol, None, rb)
+    eps = np.sqrt(3.0) * 1e-4
+    with np.errstate(divide="ignore"):
+        t0 = np.where(d != 0, (0 - o) / d, -np.inf)
+        t1 = np.where(d != 0, (1 - o) / d, np.inf)
+    tin = np.max(np.minimum(t0, t1), axis=0)
+    tout = np.min(np.maximum(t0, t1), axis=0)
+    nexp = np.floor((tout - tin - eps) / sd).astype(int) + 1
+    A = 1 - (1 - a) ** nexp
+    ok = np.abs(rb.rgba[:, 3] - A) < 1e-12
+    assert ok.mean() > 0.97
+    expect = np.minimum(c[None, :] * A[ok, None], 1.0)
+    assert np.all(np.abs(rb.rgba[ok, :3] - expect) < 1e-12)
+    assert (expect[:, 2] == 1.0).any() and (expect[:, 2] < 1.0).any()  # both sides of the clamp
+    assert np.array_equal(rb.nsamples[ok], nexp[ok])
+
+
+def _cell_scene(O, rng, alpha, sd=0.0123):
+    """Cell field on truly rectilinear coordinates (5 x 6 x 4 cells): each cell's scalar sits at
+    the centre of its own TF entry, three cells lie outside the scalar range (entries 0 and
+    255 after the clamp); the 256 TF entries have distinct colours."""
+    coords = [_uneven(n, rng) for n in (6, 7, 5)]
+    cd = [len(c) - 1 for c in coords]
+    ncell = int(np.prod(cd))
+    entries = rng.permutation(256)[:ncell]
+    vals = (entries + 0.5) / 255.0
+    vals[[0, 17, 63]] = [-0.4, 1.6, 2.5]
+    entries[[0, 17, 63]] = [0, 255, 255]
+    tf = _distinct_tf(alpha)
+    vol = _vol(O, field=vals, tf=tf, coords=coords, twin="f64", use_shadow=False, sample_distance=sd,
+               assoc=1)
+    # the field behind a pad of valid scalars: a planted wrong cell index reads those
+    buf, ptr = _padded(vals, 1024, 0.5)
+    vol._padded_field = buf
+    vol.s.field = ptr
+    return vol, coords, cd, entries.reshape(cd[2], cd[1], cd[0]), tf
+
+
+def pin_sample_cell_entries(O):
+    """Opaque TF: a ray parallel to an axis, entering through a face at the centre of a face
+    cell, composites one sample, in that face cell, and returns the cell's own entry.  Every
+    face cell of all six faces is hit.  A wrong cell index (point dims for cell dims, a
+    swapped stride), a wrong walk on the uneven spacing, a rounded or unclamped TF index picks
+    another entry."""
+    rng = np.random.default_rng(15)
+    vol, coords, cd, ent, tf = _cell_scene(O, rng, 1.0)
+    origins, dirs, expect = [], [], []
+    for axis in range(3):
+        r0, r1 = [a for a in range(3) if a != axis]
+        for sign in (1, -1):
+            for i in range(cd[r0]):
+                for j in range(cd[r1]):
+                    mid = (0.5 * (coords[r0][i] + coords[r0][i + 1]), 0.5 * (coords[r1][j] + coords[r1][j + 1]))
+                    o, d = _axis_ray(axis, sign, mid)
+                    cell = [0, 0, 0]
+                    cell[axis] = 0 if sign > 0 else cd[axis] - 1
+                    cell[r0], cell[r1] = i, j
+                    origins.append(o)
+                    dirs.append(d)
+                    expect.append(ent[cell[2], cell[1], cell[0]])
+    rb = O.RayBuf(np.array(origins).T, np.array(dirs).T, "f64")
+    O.sample_rays(vol, None, rb)
+    assert np.all(rb.nsamples == 1)
+    np.testing.assert_array_equal(rb.rgba, tf[np.array(expect)])
+
+
+def _walk_expect(xs, u0, step, sign, entry_of_cell, alpha, tf, value_fn=None, margin=1e-9):
+    """Front-to-back composite of the samples u_k = u0 + sign k step (u0 just inside the
+    entry face) along one axis of coordinates xs: each sample takes the entry of the cell
+    [x_c, x_{c+1}) that holds it (value_fn(c, u) gives a TF index from the sample's value
+    instead when set).  Returns (rgba, samples) or None when a sample lies within `margin`
+    of a coordinate, where the cell choice would hinge on rounding."""
+    A, rgb, k = 0.0, np.zeros(3), 0
+    while True:
+        u = u0 + sign * k * step
+        if not (0.0 <= u < 1.0 - margin if sign > 0 else margin < u <= 1.0):
+            break
+        if np.min(np.abs(xs - u)) < margin:
+            return None
+        c = min(int(np.searchsorted(xs, u, side="right")) - 1, len(xs) - 2)
+        e = entry_of_cell(c) if value_fn is None else value_fn(c, u)
+        if e is None:
+            return None
+        a = tf[e, 3] * (1 - A)
+        rgb = rgb + tf[e, :3] * a
+        A = A + a
+        k += 1
+    return np.r_[rgb, A], k
+
+
+def pin_sample_cell_walk(O):
+    """Translucent TF (alpha 0.05): rays along each axis in both directions through cell
+    centres of the other two axes.  Samples at the mesh epsilon past the entry face, then
+    every 0.0123, each composited with the TF entry of the uneven cell that holds it: the
+    RectilinearLocator walk up and down the coordinates.  Expected values come from a direct
+    search of the coordinates."""
+    rng = np.random.default_rng(16)
+    alpha, sd = 0.05, 0.0123
+    vol, coords, cd, ent, tf = _cell_scene(O, rng, alpha, sd)
+    eps = np.sqrt(3.0) * 1e-4
+    origins, dirs, expect, nexp = [], [], [], []
+    for axis in range(3):
+        r0, r1 = [a for a in range(3) if a != axis]
+        for sign in (1, -1):
+            for _ in range(6):
+                i, j = int(rng.integers(cd[r0])), int(rng.integers(cd[r1]))
+                mid = (0.5 * (coords[r0][i] + coords[r0][i + 1]), 0.5 * (coords[r1][j] + coords[r1][j + 1]))
+
+                def entry_of(c, axis=axis, r0=r0, r1=r1, i=i, j=j):
+                    cell = [0, 0, 0]
+                    cell[axis], cell[r0], cell[r1] = c, i, j
+                    return ent[cell[2], cell[1], cell[0]]
+
+                u0 = eps if sign > 0 else 1.0 - eps
+                got = _walk_expect(coords[axis], u0, sd, sign, entry_of, alpha, tf)
+                if got is None:
+                    continue
+                o, d = _axis_ray(axis, sign, mid)
+                origins.append(o)
+                dirs.append(d)
+                expect.append(got[0])
+                nexp.append(got[1])
+    assert len(expect) >= 30
+    rb = O.RayBuf(np.array(origins).T, np.array(dirs).T, "f64")
+    O.sample_rays(vol, None, rb)
+    np.testing.assert_array_equal(rb.nsamples, nexp)
+    assert np.all(np.abs(rb.rgba - np.array(expect)) < 1e-12)
+
+
+def test_sample_cell_constant_closed_form(oracle_mod):
+    pin_sample_cell_constant(oracle_mod)
+
+
+def test_sample_cell_face_entries(oracle_mod):
+    pin_sample_cell_entries(oracle_mod)
+
+
+def test_sample_cell_walks_uneven_cells(oracle_mod):
+    pin_sample_cell_walk(oracle_mod)
+
+
+# ---- RectilinearLocator::LocateCell (LightedVolumeRenderer.cxx:119-165) in the point Sampler ------
+
+def pin_locate_cell_uneven(O):
+    """Point field on truly rectilinear coordinates (widths varying up to 5x) that depends on
+    one axis only, f = u^2 of that axis's coordinate at each grid point.  Inside a cell the
+    trilinear sample is the chord (a + b) u - a b of the cell [a, b] the locator found, so a
+    wrong cell, a wrong corner or a wrong reciprocal width (a uniform-spacing assumption)
+    shifts the value and with it the TF entry.
+    - translucent TF (alpha 0.05), rays along that axis in both directions: each sample's
+      entry trunc(255 chord) composited front to back (the walk up and down the axis);
+    - opaque TF, rays across that axis: one sample, whose entry is trunc(255 chord(u0)).
+    Repeated with f on x, on y and on z.  Expected values come from a direct search."""
+    rng = np.random.default_rng(17)
+    sd = 0.0117
+    eps = np.sqrt(3.0) * 1e-4
+    for fa in range(3):
+        coords = [_uneven(n, rng, 0.2, 1.0) for n in (8, 7, 9)]
+        xs = coords[fa]
+        G = np.meshgrid(*coords, indexing="ij")
+        field = (G[fa] ** 2).transpose(2, 1, 0).ravel()
+
+        def entry(c, u, xs=xs):
+            a, b = xs[c], xs[c + 1]
+            v = 255.0 * ((a + b) * u - a * b)
+            if abs(v - np.round(v)) < 1e-7:
+                return None  # on an entry boundary: rounding decides
+            return int(v)
+
+        for alpha in (0.05, 1.0):
+            tf = _distinct_tf(alpha)
+            vol = _vol(O, field=field, tf=tf, coords=coords, twin="f64", use_shadow=False,
+                       sample_distance=sd)
+            origins, dirs, expect, nexp = [], [], [], []
+            for _ in range(40):
+                if alpha < 1.0:
+                    axis, sign = fa, (1 if rng.random() < 0.5 else -1)
+                else:
+                    axis, sign = int(rng.choice([a for a in range(3) if a != fa])), 1
+                r0, r1 = [a for a in range(3) if a != axis]
+                other = rng.uniform(0.02, 0.98, 2)
+                if alpha < 1.0:
+                    u0 = eps if sign > 0 else 1.0 - eps
+                    got = _walk_expect(xs, u0, sd, sign, None, alpha, tf, value_fn=entry)
+                else:
+                    u = other[0] if r0 == fa else other[1]
+                    if np.min(np.abs(xs - u)) < 1e-9:
+                        continue
+                    c = int(np.searchsorted(xs, u, side="right")) - 1
+                    e = entry(c, u)
+                    got = None if e is None else (tf[e], 1)
+                if got is None:
+                    continue
+                o, d = _axis_ray(axis, sign, other)
+                origins.append(o)
+                dirs.append(d)
+                expect.append(got[0])
+                nexp.append(got[1])
+            assert len(expect) >= 25
+            rb = O.RayBuf(np.array(origins).T, np.array(dirs).T, "f64")
+            O.sample_rays(vol, None, rb)
+            np.testing.assert_array_equal(rb.nsamples, nexp)
+            assert np.all(np.abs(rb.rgba - np.array(expect)) < 1e-12), (fa, alpha)
+
+
+def test_locate_cell_uneven_spacing(oracle_mod):
+    pin_locate_cell_uneven(oracle_mod)
