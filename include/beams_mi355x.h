@@ -103,7 +103,11 @@ int beams_synchronize(beams_renderer* h);
  * Limits (BEAMS_ERR_INVALID / BEAMS_ERR_UNSUPPORTED otherwise): each dims[i] in [2, 2^20],
  *   dims[0]*dims[1] below 2^31, dims[0]+dims[1]+dims[2] at most 16384 (the coordinate tables
  *   are staged in LDS), coordinates finite with magnitude at most 2^60 and cell widths at
- *   least 2^-60 (the range of the kernels' corrected-reciprocal quotients). */
+ *   least 2^-60 (the range of the kernels' corrected-reciprocal quotients).
+ * Joint limit, checked by the render calls (BEAMS_ERR_UNSUPPORTED): 16 B per colormap entry
+ *   plus 8 B per coordinate point (12 B for unevenly spaced coordinates, which carry bin
+ *   tables) within 159 KiB, the LDS a workgroup stages; the learned closure takes at most
+ *   1024 colormap entries. */
 int beams_set_field(beams_renderer* h, const void* values, int32_t dtype, int32_t assoc,
                     const int64_t dims[3], const float* const coords[3], const double origin[3],
                     const double spacing[3], int32_t mem);

@@ -329,12 +329,29 @@ bool camera_params(const beams_camera* cam, int32_t width, int32_t height, Camer
   return true;
 }
 
+// LDS a workgroup may allocate on gfx950, and the most the kernels' own fixed arrays (work
+// queues, hit lists, reductions) take beside what a frame stages.
+constexpr size_t kLdsPerGroup = 160 * 1024;
+constexpr size_t kLdsFixed = 1024;
+
 int check_ready(beams_renderer* h)
 {
   if (!h->has_field) return h->fail(BEAMS_ERR_STATE, "no field: call beams_set_field first");
   if (h->htf.empty()) return h->fail(BEAMS_ERR_STATE, "no colormap: call beams_set_colormap first");
   if (!h->has_range) return h->fail(BEAMS_ERR_STATE, "no scalar range: call beams_set_scalar_range");
   if (h->dtype == BEAMS_F64) return h->fail(BEAMS_ERR_UNSUPPORTED, "FP64 field storage is not rendered");
+  // The camera marches stage the colormap as float4 beside the coordinates, their reciprocal
+  // widths and (uneven coordinates) the bin tables; the map and sweep kernels stage less.
+  const size_t ntf = h->htf.size() / 4;
+  const size_t ncoord = (size_t)(h->dims[0] + h->dims[1] + h->dims[2]);
+  const size_t staged = 16 * ntf + (h->use_bins ? 12 : 8) * ncoord;
+  if (staged + kLdsFixed > kLdsPerGroup)
+    return h->fail(BEAMS_ERR_UNSUPPORTED,
+                   "colormap and coordinates exceed the LDS a workgroup stages: 16 B per colormap "
+                   "entry + 8 B per coordinate point (12 B for unevenly spaced coordinates) must "
+                   "stay within 159 KiB (got " + std::to_string(staged) + " B)");
+  if (h->use_shadow && h->assoc == 0 && h->shadow_mode == BEAMS_SHADOW_LEARNED && ntf > 1024)
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "the learned closure stages at most 1024 colormap entries");
   if (h->use_shadow && h->assoc == 0) {
     if (!h->has_light) return h->fail(BEAMS_ERR_STATE, "shadow map needs a light: call beams_set_light");
     float ma = 0.0f;

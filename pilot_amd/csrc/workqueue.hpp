@@ -6,12 +6,14 @@
 
 namespace beams {
 
-// Work queue of a persistent workgroup.  Items are handed out one at a time from eight counters
-// (one per XCD: workgroups are dealt to the XCDs round-robin), counter x handing out items
-// x, x + 8, x + 16, ...: the workgroups always work on the next few thousand consecutive
-// items, so the bricks they read stay few and cached (larger chunks, or a fixed stride whose
-// workgroups drift apart, spread the reads over many bricks and slowed the 1024^3 maps by
-// half), while the device-scope atomics are spread over eight addresses instead of one.
+// Work queue of a persistent workgroup.  Items are handed out one at a time from up to eight
+// counters (one per XCD: workgroups are dealt to the XCDs round-robin), counter x handing out
+// items x, x + S, x + 2S, ... with S = min(8, workgroups) counters in use, so that every
+// counter has a workgroup behind it whatever the grid: the workgroups always work on the next
+// few thousand consecutive items, so the bricks they read stay few and cached (larger chunks,
+// or a fixed stride whose workgroups drift apart, spread the reads over many bricks and
+// slowed the 1024^3 maps by half), while the device-scope atomics are spread over eight
+// addresses instead of one.
 struct ItemQueue {
   int next, end, item;
 };
@@ -27,9 +29,10 @@ __device__ __forceinline__ int next_item(unsigned int* counters, ItemQueue* q, i
   __syncthreads();  // every thread is done with the previous item and its LDS
   if (threadIdx.x == 0) {
     if (q->next >= q->end) {
-      const unsigned x = blockIdx.x & 7u;
+      const unsigned slots = gridDim.x < 8u ? gridDim.x : 8u;
+      const unsigned x = blockIdx.x % slots;
       const int64_t k = (int64_t)atomicAdd(&counters[x], 1u);
-      const int64_t first = (k * 8 + x) * chunk;
+      const int64_t first = (k * slots + x) * chunk;
       q->next = first < nitems ? (int)first : nitems;
       q->end = first + chunk < nitems ? (int)(first + chunk) : nitems;
     }

@@ -537,7 +537,9 @@ std::vector<DataSet> ResampleChunks(const DataSet& ds, int chunksX, int chunksY,
       throw ErrorBadValue("ResampleChunks: each axis needs 1 <= chunks <= points");
     const auto& ax = ds.Coords.Axes[d];
     origin[d] = ax.front();
-    chunkSize[d] = (ax.back() - ax.front()) / static_cast<float>(chunks[d]);
+    // convert/Chunk.cxx: bounds.X.Length() / chunksX in FP64 (vtkm::Bounds), rounded once
+    chunkSize[d] = static_cast<float>((static_cast<double>(ax.back()) - static_cast<double>(ax.front())) /
+                                      static_cast<double>(chunks[d]));
     cdims[d] = dims[d] / chunks[d];
     pdims[d] = cdims[d] + 1;
     spacing[d] = chunkSize[d] / static_cast<float>(cdims[d]);

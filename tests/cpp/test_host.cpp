@@ -259,6 +259,32 @@ static void test_resample_chunks()
     threw = true;
   }
   CHECK(threw);
+
+  // The chunk size is the FP64 bounds length over the chunk count, rounded once to FP32
+  // (Chunk.cxx: bounds.X.Length() / chunksX).  On [0.1431428, 2.2749565] in 5 chunks an FP32
+  // subtraction and an FP32 division round twice, to 0.42636275 instead of 0.42636272.
+  {
+    beams::DataSet r;
+    r.CellSet.PointDims = { 11, 2, 2 };
+    const float a = 0.1431428f, b = 2.2749565f;
+    for (int i = 0; i < 11; ++i) r.Coords.Axes[0].push_back(a + (b - a) * static_cast<float>(i) / 10.0f);
+    r.Coords.Axes[0].back() = b;
+    r.Coords.Axes[1] = { 0.0f, 1.0f };
+    r.Coords.Axes[2] = { 0.0f, 1.0f };
+    beams::Field f;
+    f.Name = "zero";
+    f.Values.assign(11 * 2 * 2, 0.0f);
+    r.Fields = { f };
+    const auto rc = io::ResampleChunks(r, 5, 1, 1);
+    const float once = static_cast<float>((static_cast<double>(b) - static_cast<double>(a)) / 5.0);
+    const float twice = (b - a) / 5.0f;
+    CHECK(once != twice);
+    bool origins = rc.size() == 5;
+    for (size_t c = 0; origins && c < rc.size(); ++c)
+      origins = rc[c].Coords.Axes[0][0] == a + static_cast<float>(c) * once &&
+                rc[c].Coords.Axes[0][1] == a + static_cast<float>(c) * once + once / 2.0f;
+    CHECK(origins);
+  }
 }
 
 static void test_images()

@@ -112,3 +112,24 @@ def test_bricks_axis_aligned_light_segments(oracle_mod, kind, pos):
         assert np.array_equal(maps[b][0], local[b]), f"brick {b} local map"
         assert np.array_equal(maps[b][1], final[b]), f"brick {b} final map"
     assert np.array_equal(img, oimg)
+
+
+@pytest.mark.parametrize("groups", [1, 3, 7])
+def test_bricks_small_persistent_grids(oracle_mod, groups):
+    """Persistent launches of fewer than eight workgroups (the work queue's counter count)
+    still hand out every item: maps and image bit-equal to the default schedule's."""
+    sc = scenes.make_scene("fewgroups", 41, seed=13, nblobs=10, width=64, height=48)
+    ranges = B.kd_bricks(sc.dims, 9, 21, 8, 24)
+    print(f"[bricks persistent groups {groups}]", flush=True)
+    ref_img, ref_depth, ref_maps, _ = _gpu(sc, ranges)
+    with B.LightedVolumeRenderer() as r:
+        scenes.configure(r, sc)
+        r.SetBricks(ranges, ghost=1)
+        r.SetPersistentGroups(groups)
+        img, depth = r.RenderImage(sc.camera, sc.width, sc.height)
+        for b, rg in enumerate(ranges):
+            nv = int(np.prod(rg[1::2] - rg[0::2] + 1))
+            assert np.array_equal(r.GetBrickMap(b, nv, final=False), ref_maps[b][0]), f"brick {b} local"
+            assert np.array_equal(r.GetBrickMap(b, nv, final=True), ref_maps[b][1]), f"brick {b} final"
+    assert np.array_equal(img.reshape(-1, 4), ref_img)
+    assert np.array_equal(depth.reshape(-1), ref_depth)

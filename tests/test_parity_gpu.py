@@ -349,3 +349,117 @@ def test_device_generator_matches_host():
     d16 = DeviceArray(n, np.uint16)
     B.gen_field(dims, 21, nblobs=9, noise_octaves=3, noise_amp=0.2, dtype=B.BF16, out=d16)
     assert np.array_equal(d16.numpy(), h16)
+
+
+def _uses_bins(coords):
+    """The library's rule for bin tables (beams_set_field): some cell's midpoint is not
+    where the uniform-spacing guess puts it, and at most 8192 points over the three axes."""
+    uneven = False
+    for x in coords:
+        x = np.asarray(x, np.float32)
+        ginv = np.float32((len(x) - 1) / (float(x[-1]) - float(x[0])))
+        mid = (0.5 * (x[:-1].astype(np.float64) + x[1:])).astype(np.float32)
+        guess = ((mid - x[0]) * ginv).astype(np.int64)
+        uneven |= bool(np.any(guess != np.arange(len(x) - 1)))
+    return uneven and sum(len(x) for x in coords) <= 8192
+
+
+def test_uneven_coordinates_render_image(oracle_mod):
+    """render_image (the frame entry point, k_march_image) on coordinates whose widths
+    spread 3x: the bin-table variant runs (the coordinates satisfy the library's rule for
+    it) and RGBA and depth equal the oracle's rays of the same camera bit for bit."""
+    import dataclasses
+
+    sc = scenes.make_scene("uneven_img", 30, seed=37, nblobs=9, light="point", width=72, height=56)
+    rng = np.random.default_rng(3)
+    coords = []
+    for d in range(3):
+        w = np.exp(rng.uniform(0.0, np.log(3.0), size=sc.dims[d] - 1))
+        coords.append((np.concatenate([[0.0], np.cumsum(w)]) / w.sum()).astype(np.float32))
+    assert _uses_bins(coords)
+    sc = dataclasses.replace(sc, coords=coords, camera=B.orbit_camera([0, 1] * 3, 210.0, 25.0, 50.0))
+    print("[uneven render_image]", flush=True)
+    with B.LightedVolumeRenderer() as r:
+        scenes.configure(r, sc)
+        img, depth = r.RenderImage(sc.camera, sc.width, sc.height)
+        gmap = r.GetMap(sc.map_cells)
+    o, d = B.camera_rays(sc.camera, sc.width, sc.height)
+    O = oracle_mod
+    rb = O.RayBuf(o, d, "f32")
+    omap = O.render_rays(O.volume_from_scene(sc, "f32"), rb)
+    assert np.array_equal(gmap, omap)
+    assert np.array_equal(img.reshape(-1, 4), rb.rgba)
+    assert np.array_equal(depth.reshape(-1), np.where(rb.min_d == -1, np.inf, rb.min_d).astype(np.float32))
+    assert rb.rgba[:, 3].max() > 0.1
+
+
+def test_colormap_and_coordinates_beyond_lds_are_refused():
+    """8192 colormap entries (128 KiB as float4) with 3,982 coordinate points (31 KiB) exceed
+    what a workgroup stages: the render call refuses them with BEAMS_ERR_UNSUPPORTED and a
+    message naming the limit; 2048 entries with the same grid render."""
+    dims = (1990, 1990, 2)
+    vals = np.zeros(int(np.prod(dims)), np.float16)
+    tf = np.tile(np.float32([0.5, 0.5, 0.5, 0.2]), (8192, 1))
+    rays_o = np.float32([[0.5], [0.5], [-1.0]])
+    rays_d = np.float32([[0.0], [0.0], [1.0]])
+    with B.LightedVolumeRenderer() as r:
+        r.SetData(vals, dims, origin=(0, 0, 0), spacing=(1.0 / 1989, 1.0 / 1989, 1.0), dtype=B.F16)
+        r.SetColorMap(tf)
+        r.SetShadow(False)
+        with pytest.raises(B.BeamsError) as ei:
+            r.Render(B.Rays(rays_o, rays_d))
+        assert ei.value.status == 2 and "LDS" in str(ei.value)
+        r.SetColorMap(tf[:2048])
+        out = r.Render(B.Rays(rays_o, rays_d))
+        assert out.rgba[0, 3] > 0.0
+
+
+@pytest.mark.parametrize("case", ["cells", "points"])
+def test_oracle_pin_scenes_on_gpu(oracle_mod, case):
+    """The scenes that pin the oracle's SamplerCellAssoc and RectilinearLocator walk (cell and
+    point fields on truly rectilinear coordinates, rays along each axis in both directions)
+    rendered by the GPU: RGBA bit-equal to the FP32 oracle."""
+    O = oracle_mod
+    rng = np.random.default_rng(41 if case == "cells" else 43)
+    coords = []
+    for n in (9, 11, 8):
+        w = rng.uniform(0.2, 1.0, n - 1)
+        coords.append((np.concatenate([[0.0], np.cumsum(w)]) / w.sum()).astype(np.float32))
+    dims = tuple(len(c) for c in coords)
+    i = np.arange(256)
+    tf = np.stack([i / 255.0, 1 - i / 255.0, (i % 7) / 7.0, np.full(256, 0.05)], axis=1).astype(np.float32)
+    if case == "cells":
+        ncell = int(np.prod([n - 1 for n in dims]))
+        field = ((rng.integers(0, 256, ncell) + 0.5) / 255.0).astype(np.float32)
+        field[:3] = [-0.4, 1.6, 2.5]
+        assoc = B.ASSOC_CELLS
+    else:
+        G = np.meshgrid(*coords, indexing="ij")
+        field = (G[0] ** 2 * 0.5 + G[1] * 0.3 + G[2] ** 2 * 0.2).transpose(2, 1, 0).ravel().astype(np.float32)
+        assoc = B.ASSOC_POINTS
+    org, drs = [], []
+    for axis in range(3):
+        for sign in (1.0, -1.0):
+            for _ in range(40):
+                o = rng.uniform(0.02, 0.98, 3)
+                o[axis] = -1.0 if sign > 0 else 2.0
+                d = np.zeros(3)
+                d[axis] = sign
+                org.append(o)
+                drs.append(d)
+    o = np.array(org, np.float32).T.copy()
+    d = np.array(drs, np.float32).T.copy()
+    sd = np.float32(0.0117)
+    print(f"[pin scene {case}] rays={o.shape[1]}", flush=True)
+    with B.LightedVolumeRenderer() as r:
+        r.SetData(field, dims, coords=coords, scalar_range=(0.0, 1.0), dtype=B.F32, assoc=assoc)
+        r.SetColorMap(tf)
+        r.SetShadow(False)
+        r.SetSampleDistance(float(sd))
+        rays = r.Render(B.Rays(o, d))
+    vol = O.Volume(dims, coords, field, tf, use_shadow=False, sample_distance=float(sd),
+                   assoc=1 if case == "cells" else 0, twin="f32")
+    rb = O.RayBuf(o, d, "f32")
+    O.sample_rays(vol, None, rb)
+    assert np.array_equal(rays.rgba, rb.rgba)
+    assert rb.rgba[:, 3].min() > 0.0
