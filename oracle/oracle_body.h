@@ -645,8 +645,53 @@ typedef struct {
   R opacity;
 } FN(Hit);
 
-int64_t FN(orc_nonlocal_hits)(int nblocks, const OrcScene* scenes, int self, int64_t* counts,
-                              int64_t* offsets, void* hits_out, int64_t hits_cap)
+/* One pass over the map vertices of block `self`: the number of other blocks each light
+ * segment crosses (CountNonLocalBlockHits, into counts), or, with hits given, the hit records
+ * at offsets (CalculateNonLocalBlockHits).  Vertices are independent: nthr threads share them. */
+static void FN(hits_pass)(const FN(Geo) * g, const double* bb, int nblocks, int self, int kind,
+                          int64_t* counts, const int64_t* offsets, FN(Hit) * hits, int nthr)
+{
+  const int64_t nv = g->mp[0] * g->mp[1] * g->mp[2];
+  (void)nthr;
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(nthr > 0 ? nthr : 1)
+  for (int64_t id = 0; id < nv; ++id) {
+    R v[3], o[3];
+    FN(map_vertex)(g, id, v);
+    FN(light_origin)(g, kind, v, o);
+    if (!hits) {
+      int64_t n = 0;
+      for (int b = 0; b < nblocks; ++b) {
+        if (b == self) continue;
+        R t0, t1;
+        if (FN(segment_aabb)(o, v, bb + 6 * b, (R)1e-4f, &t0, &t1)) ++n; /* glancing hits kept */
+      }
+      counts[id] = n;
+      continue;
+    }
+    R dd[3] = { v[0] - o[0], v[1] - o[1], v[2] - o[2] };
+    R rs = (R)1 / SQRT_(FN(dot3)(dd, dd)); /* vtkm::Normalize */
+    R dir[3] = { dd[0] * rs, dd[1] * rs, dd[2] * rs };
+    int64_t off = offsets ? offsets[id] : 0;
+    for (int b = 0; b < nblocks; ++b) {
+      if (b == self) continue;
+      R t0, t1;
+      /* the reference passes SelfBlockId as tEps here; tMin is never < 0, so it is inert */
+      if (!FN(segment_aabb)(o, v, bb + 6 * b, (R)self, &t0, &t1)) continue;
+      FN(Hit)* h = &hits[off++];
+      h->ray_id = (int32_t)id;
+      h->block_id = b;
+      h->from_block_id = self;
+      h->ray_t = t1;
+      for (int d = 0; d < 3; ++d) h->point[d] = o[d] + dir[d] * t1;
+      h->opacity = 0;
+    }
+  }
+}
+
+/* CountNonLocalBlockHits + ScanExclusive + CalculateNonLocalBlockHits for block `self`
+ * (TransmittanceMap.h:342-453, 623-649); returns the number of hits. */
+static int64_t FN(nonlocal_hits)(int nblocks, const OrcScene* scenes, int self, int64_t* counts,
+                                 int64_t* offsets, void* hits_out, int64_t hits_cap, int nthr)
 {
   FN(Geo) g, gb;
   FN(geo_init)(&g, &scenes[self]);
@@ -657,50 +702,24 @@ int64_t FN(orc_nonlocal_hits)(int nblocks, const OrcScene* scenes, int self, int
     memcpy(bb + 6 * b, gb.bounds, sizeof(double) * 6);
   }
   const int kind = scenes[self].light_kind;
-  /* CountNonLocalBlockHits + ScanExclusive */
+  int64_t* cnt = counts ? counts : (int64_t*)malloc(sizeof(int64_t) * (size_t)nv);
+  FN(hits_pass)(&g, bb, nblocks, self, kind, cnt, 0, 0, nthr);
   int64_t total = 0;
   for (int64_t id = 0; id < nv; ++id) {
-    R v[3], o[3];
-    FN(map_vertex)(&g, id, v);
-    FN(light_origin)(&g, kind, v, o);
-    int64_t n = 0;
-    for (int b = 0; b < nblocks; ++b) {
-      if (b == self) continue;
-      R t0, t1;
-      if (FN(segment_aabb)(o, v, bb + 6 * b, (R)1e-4f, &t0, &t1)) ++n; /* glancing hits kept */
-    }
-    if (counts) counts[id] = n;
     if (offsets) offsets[id] = total;
-    total += n;
+    total += cnt[id];
   }
-  if (hits_out && hits_cap >= total) {
-    FN(Hit)* hits = (FN(Hit)*)hits_out;
-    /* CalculateNonLocalBlockHits */
-    for (int64_t id = 0; id < nv; ++id) {
-      R v[3], o[3];
-      FN(map_vertex)(&g, id, v);
-      FN(light_origin)(&g, kind, v, o);
-      R dd[3] = { v[0] - o[0], v[1] - o[1], v[2] - o[2] };
-      R rs = (R)1 / SQRT_(FN(dot3)(dd, dd)); /* vtkm::Normalize */
-      R dir[3] = { dd[0] * rs, dd[1] * rs, dd[2] * rs };
-      int64_t off = offsets ? offsets[id] : 0;
-      for (int b = 0; b < nblocks; ++b) {
-        if (b == self) continue;
-        R t0, t1;
-        /* the reference passes SelfBlockId as tEps here; tMin is never < 0, so it is inert */
-        if (!FN(segment_aabb)(o, v, bb + 6 * b, (R)self, &t0, &t1)) continue;
-        FN(Hit)* h = &hits[off++];
-        h->ray_id = (int32_t)id;
-        h->block_id = b;
-        h->from_block_id = self;
-        h->ray_t = t1;
-        for (int d = 0; d < 3; ++d) h->point[d] = o[d] + dir[d] * t1;
-        h->opacity = 0;
-      }
-    }
-  }
+  if (hits_out && hits_cap >= total)
+    FN(hits_pass)(&g, bb, nblocks, self, kind, cnt, offsets, (FN(Hit)*)hits_out, nthr);
+  if (!counts) free(cnt);
   free(bb);
   return total;
+}
+
+int64_t FN(orc_nonlocal_hits)(int nblocks, const OrcScene* scenes, int self, int64_t* counts,
+                              int64_t* offsets, void* hits_out, int64_t hits_cap)
+{
+  return FN(nonlocal_hits)(nblocks, scenes, self, counts, offsets, hits_out, hits_cap, 1);
 }
 
 /* Phase 2 fetch (TransmittanceFetcher2, TransmittanceMap.h:505-508) and phase 3
@@ -716,22 +735,34 @@ static int FN(hit_less)(const FN(Hit) * a, const FN(Hit) * b)
 int FN(orc_nonlocal_seed)(int nblocks, const OrcScene* scenes, const void* const* maps, int self,
                           void* seed_out, int nthr)
 {
-  (void)nthr;
   FN(Geo) g;
   FN(geo_init)(&g, &scenes[self]);
   const int64_t nv = g.mp[0] * g.mp[1] * g.mp[2];
   int64_t* counts = (int64_t*)malloc(sizeof(int64_t) * (size_t)nv);
   int64_t* offs = (int64_t*)malloc(sizeof(int64_t) * (size_t)nv);
-  int64_t total = FN(orc_nonlocal_hits)(nblocks, scenes, self, counts, offs, 0, 0);
-  FN(Hit)* hits = (FN(Hit)*)malloc(sizeof(FN(Hit)) * (size_t)(total > 0 ? total : 1));
-  FN(orc_nonlocal_hits)(nblocks, scenes, self, counts, offs, hits, total);
   FN(Geo)* gs = (FN(Geo)*)malloc(sizeof(FN(Geo)) * (size_t)nblocks);
-  for (int b = 0; b < nblocks; ++b) FN(geo_init)(&gs[b], &scenes[b]);
+  double* bb = (double*)malloc(sizeof(double) * 6 * (size_t)nblocks);
+  for (int b = 0; b < nblocks; ++b) {
+    FN(geo_init)(&gs[b], &scenes[b]);
+    memcpy(bb + 6 * b, gs[b].bounds, sizeof(double) * 6);
+  }
+  const int kind = scenes[self].light_kind;
+  FN(hits_pass)(&g, bb, nblocks, self, kind, counts, 0, 0, nthr);
+  int64_t total = 0;
+  for (int64_t id = 0; id < nv; ++id) {
+    offs[id] = total;
+    total += counts[id];
+  }
+  FN(Hit)* hits = (FN(Hit)*)malloc(sizeof(FN(Hit)) * (size_t)(total > 0 ? total : 1));
+  FN(hits_pass)(&g, bb, nblocks, self, kind, counts, offs, hits, nthr);
+  free(bb);
+#pragma omp parallel for schedule(dynamic, 4096) num_threads(nthr > 0 ? nthr : 1)
   for (int64_t h = 0; h < total; ++h) {
     int b = hits[h].block_id;
     hits[h].opacity = FN(map_opacity)(&gs[b], (const R*)maps[b], hits[h].point);
   }
   R* seed = (R*)seed_out;
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(nthr > 0 ? nthr : 1)
   for (int64_t id = 0; id < nv; ++id) {
     /* insertion sort of this vertex's hits by RayT (a vertex crosses few blocks) */
     FN(Hit)* hs = hits + offs[id];

@@ -121,6 +121,7 @@ struct beams_renderer {
     int64_t nverts = 0, seed_off = 0;
   };
   std::vector<Brick> bricks;
+  std::vector<Brick> brick_pool;  // storage of the decomposition a new field replaced
   int32_t ghost = 0;
   DevBuf bmap_local, bmap_final, bseeds, btables, bpartial, bqueue;
   DevBuf bredo;  // learned bricks: per-brick fallback flags
@@ -903,6 +904,7 @@ int beams_destroy(beams_renderer* h)
   h->counters.release();
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
   for (auto& br : h->bricks) br.field.release();
+  for (auto& br : h->brick_pool) br.field.release();
   h->bmap_local.release();
   h->bmap_final.release();
   h->bseeds.release();
@@ -1030,8 +1032,11 @@ int beams_set_field(beams_renderer* h, const void* values, int32_t dtype, int32_
   h->assoc = assoc;
   h->has_field = true;
   h->map_valid = false;
-  h->bricks.clear();  // a new field drops the previous brick decomposition
-  h->btab_host.clear();
+  // A new field drops the brick decomposition; its storage waits in the pool for the next
+  // beams_set_bricks, which reuses it where the stored regions match.
+  for (auto& br : h->brick_pool) br.field.release();
+  h->brick_pool.swap(h->bricks);
+  h->bricks.clear();
   return BEAMS_OK;
 }
 
@@ -1040,6 +1045,7 @@ int beams_set_bricks(beams_renderer* h, int32_t nbricks, const int64_t* point_ra
 {
   CHECK_HANDLE(h);
   if (nbricks == 0) {
+    for (auto& br : h->bricks) br.field.release();
     h->bricks.clear();
     h->btab_host.clear();
     return BEAMS_OK;
@@ -1051,7 +1057,6 @@ int beams_set_bricks(beams_renderer* h, int32_t nbricks, const int64_t* point_ra
   if (h->assoc != BEAMS_ASSOC_POINTS || h->dtype == BEAMS_F64)
     return h->fail(BEAMS_ERR_UNSUPPORTED, "bricks need a point field stored as F32, F16 or BF16");
   std::vector<beams_renderer::Brick> bricks((size_t)nbricks);
-  const size_t esz = dtype_size(h->dtype);
   int64_t off = 0;
   for (int b = 0; b < nbricks; ++b) {
     auto& br = bricks[(size_t)b];
@@ -1069,15 +1074,38 @@ int beams_set_bricks(beams_renderer* h, int32_t nbricks, const int64_t* point_ra
     br.nverts = (br.cells[0] + 1) * (br.cells[1] + 1) * (br.cells[2] + 1);
     br.seed_off = off;
     off += br.nverts;
-    HIP_TRY(h, br.field.ensure(esz * (size_t)(br.sdims[0] * br.sdims[1] * br.sdims[2])));
-    HIP_TRY(h, launch_extract_brick(h->field.p, (int32_t)esz, h->dims, br.slo, br.sdims, br.field.p,
-                                    h->stream));
   }
-  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  // Storage of the previous decomposition (the current one, or the one a new field replaced)
+  // is taken over brick by brick, so that a series of fields on one decomposition keeps every
+  // device address, the brick tables and a captured frame graph unchanged.
+  std::vector<beams_renderer::Brick>& prev = h->bricks.empty() ? h->brick_pool : h->bricks;
+  const size_t esz = dtype_size(h->dtype);
+  int rc = BEAMS_OK;
+  for (int b = 0; b < nbricks && rc == BEAMS_OK; ++b) {
+    auto& br = bricks[(size_t)b];
+    if ((size_t)b < prev.size()) std::swap(br.field, prev[(size_t)b].field);
+    const hipError_t e = br.field.ensure(esz * (size_t)(br.sdims[0] * br.sdims[1] * br.sdims[2]));
+    const hipError_t x = e != hipSuccess ? e
+                                         : launch_extract_brick(h->field.p, (int32_t)esz, h->dims, br.slo,
+                                                                br.sdims, br.field.p, h->stream);
+    if (x != hipSuccess) rc = h->fail(BEAMS_ERR_HIP, std::string("bricks: ") + hipGetErrorString(x));
+  }
+  if (rc == BEAMS_OK && hipStreamSynchronize(h->stream) != hipSuccess)
+    rc = h->fail(BEAMS_ERR_HIP, "bricks: extraction failed");
+  for (auto& br : prev) br.field.release();
+  prev.clear();
+  if (rc != BEAMS_OK) {
+    for (auto& br : bricks) br.field.release();
+    h->bricks.clear();
+    h->btab_host.clear();
+    h->map_valid = false;
+    return rc;
+  }
+  for (auto& br : h->brick_pool) br.field.release();
+  h->brick_pool.clear();
+  // the brick tables are rebuilt every frame and uploaded only when their bytes change
   h->bricks.swap(bricks);
-  for (auto& br : bricks) br.field.release();
   h->ghost = ghost;
-  h->btab_host.clear();
   h->map_valid = false;
   return BEAMS_OK;
 }
