@@ -179,6 +179,20 @@ int beams_render_rays(beams_renderer* h, int64_t n, const float* const origin[3]
                       const float* const dir[3], float* min_d, float* max_d, float* dist,
                       float* rgba, int32_t mem);
 
+/* beams_render_rays without allocating: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  The same map build and march as beams_render_rays with BEAMS_MEM_DEVICE
+ * rays (device memory only, used in place, stream-ordered on the handle's stream, nothing
+ * waited for), but the transmittance map goes to a scratch array of 64 MiB that the library
+ * holds per device from load until the process ends (at most 2^24 FP32 or 2^25 FP16 map
+ * vertices; larger maps are refused with BEAMS_ERR_INVALID).  Every handle on a device shares
+ * that device's array and it keeps the last call's map: calls that overlap on one device race.
+ * Single block, reference-march or no shadow map only (BEAMS_ERR_UNSUPPORTED otherwise); no
+ * phase timings or sample counts are recorded, and beams_get_map does not see this map. */
+int beams_render_rays_noalloc(beams_renderer* h, int64_t n, const float* const origin[3],
+                              const float* const dir[3], float* min_d, float* max_d, float* dist,
+                              float* rgba);
+
 /* ---- MapperLightedVolume::RenderCells (MapperLightedVolume.cxx:114-193) ------------------
  * Full frame for a width x height image: camera rays, map build, march, optional
  * background blend (background may be NULL: none; else RGBA blended under the image as

@@ -114,6 +114,7 @@ SIGNATURES = {
     "beams_set_learned": (C.c_int, [_P, _P, _I32, _F, _I32]),
     "beams_learned_samples": (C.c_int, [_P, _I32, _I64, C.c_uint64, _P, _P, _I32]),
     "beams_get_learned_stats": (C.c_int, [_P, _PF, C.POINTER(C.c_int32)]),
+    "beams_render_rays_noalloc": (C.c_int, [_P, _I64, C.POINTER(_P), C.POINTER(_P), _P, _P, _P, _P]),
 }
 
 _LIB = None

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -52,10 +53,21 @@ size_t dtype_size(int32_t dt)
   }
 }
 
+// Scratch of beams_render_rays_noalloc: the transmittance map of callers that cannot
+// allocate.  One array per device, held from module load until the process ends; host code
+// uses only the address hipGetSymbolAddress reports for the current device (kept in
+// g_scratch), never the array's name.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
 }  // namespace
 
 struct beams_renderer {
   int device = 0;
+  void* scratch = nullptr;  // this device's scratch array (g_scratch[device])
   hipStream_t stream = nullptr;
   std::string err;
   // SetData
@@ -878,8 +890,20 @@ int beams_create(beams_renderer** out)
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return BEAMS_ERR_NO_DEVICE;
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return BEAMS_ERR_NO_DEVICE;
+  void* scr = nullptr;
+  if (dev < kMaxDevices) {
+    // the scratch array's address on this device, looked up once with the device current
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+      void* p = nullptr;
+      if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return BEAMS_ERR_HIP;
+      g_scratch[dev] = p;
+    }
+    scr = g_scratch[dev];
+  }
   auto* h = new beams_renderer();
   h->device = dev;
+  h->scratch = scr;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return BEAMS_ERR_HIP;
@@ -1360,6 +1384,51 @@ int beams_render_rays(beams_renderer* h, int64_t n, const float* const origin[3]
     HIP_TRY(h, hipMemcpyAsync(rgba, r.rgba, 4 * fb, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
   }
+  return BEAMS_OK;
+}
+
+int beams_render_rays_noalloc(beams_renderer* h, int64_t n, const float* const origin[3],
+                              const float* const dir[3], float* min_d, float* max_d, float* dist,
+                              float* rgba)
+{
+  CHECK_HANDLE(h);
+  if (n < 0) return h->fail(BEAMS_ERR_INVALID, "negative ray count");
+  if (!origin || !dir || !min_d || !max_d || !dist || !rgba)
+    return h->fail(BEAMS_ERR_INVALID, "null ray array");
+  for (int d = 0; d < 3; ++d)
+    if (!origin[d] || !dir[d]) return h->fail(BEAMS_ERR_INVALID, "null ray array");
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (!h->bricks.empty())
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "the no-allocation render takes a single block, not bricks");
+  const bool with_map = h->use_shadow && h->assoc == 0;
+  if (with_map && h->shadow_mode != BEAMS_SHADOW_REF_MARCH)
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "the no-allocation render builds its map with the reference march");
+  if (!h->scratch) return h->fail(BEAMS_ERR_STATE, "no scratch array on this device");
+  GridParams g = grid_params(h);
+  ShadeParams s = shade_params(h, g);
+  s.counters = nullptr;  // the sample counters live in an allocated buffer
+  MapParams m = map_params(h, g, BEAMS_SHADOW_REF_MARCH);
+  if (with_map) {
+    const uint64_t bytes = (uint64_t)m.mp[0] * (uint64_t)m.mp[1] * (uint64_t)m.mp[2] * dtype_size(h->map_dtype);
+    if (bytes > kScratchBytes)
+      return h->fail(BEAMS_ERR_INVALID, "map of " + std::to_string(bytes) + " B exceeds the " +
+                                            std::to_string(kScratchBytes >> 20) +
+                                            " MiB scratch of the no-allocation render");
+    m.map = h->scratch;
+    HIP_TRY(h, launch_tmap_ref(g, m, s, nullptr, h->stream));
+  }
+  RaySoA r{};
+  r.n = n;
+  for (int d = 0; d < 3; ++d) {
+    r.org[d] = origin[d];
+    r.dir[d] = dir[d];
+  }
+  r.min_d = min_d;
+  r.max_d = max_d;
+  r.dist = dist;
+  r.rgba = rgba;
+  HIP_TRY(h, launch_march_rays(g, m, s, r, h->stream));
   return BEAMS_OK;
 }
 
