@@ -1,17 +1,17 @@
 #!/usr/bin/env python3
 """Lit-frame benchmark of the MI355X Beams renderer on one GPU.
 
-A step is one lit frame of the exact-shadow path: transmittance-map build (the
-reference's per-vertex light march) followed by the camera-ray march with shadow
-lookups, rendered into HBM-resident RGBA + depth buffers.  The workload is the
-precision-sweep configuration: a 256^3 field (64 Gaussian blobs + 3-octave value
-noise at 0.2) generated directly in HBM, a 256^3-cell (257^3-vertex) map, a point
-light outside the box, 1024x1024 pixels, step 0.5 voxel.
+A step is one lit frame of the time-series workload with exact shadows: a 1024^3 FP16
+blob+noise field (one of 8 drifting timesteps, resident in HBM) cut into 128 uneven kd bricks
+that stand in for the reference's 128 MPI ranks; each frame rebuilds every brick's
+transmittance map (local light march, non-local exchange, seeded re-march), marches every
+brick's camera rays at 2048x2048 and composites them in depth order, as one hipGraph launch.
 
-Prints one JSON line: ray samples/s (composited camera-ray samples, counted on the
-GPU) with frame ms, per-phase kernel times from HIP events on the renderer's
-stream, the dominant kernel's roofline entry, and the CPU oracle timed on a bounded
-sample of the same frame.
+Prints one JSON line: ray samples/s (composited camera-ray samples, counted on the GPU) and
+frame ms, per-phase kernel times from HIP events on the renderer's stream, the dominant
+kernel's roofline entry, and the CPU oracle timed (Serial and OpenMP) on a bounded sample of
+the same frame, plus the other workloads (precision-sweep frame, batched bricks, learned
+closure, learned time series, RGBA error against the FP64 oracle).
 """
 
 import argparse
@@ -39,7 +39,6 @@ def parse():
     ap.add_argument("--no-variants", action="store_true", help="skip the FP16/BF16 storage variants")
     ap.add_argument("--no-bricks", action="store_true", help="skip the batched-bricks workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-closure workloads")
-    ap.add_argument("--no-series", action="store_true", help="skip the 1024^3 time-series workload")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the RGBA error report against the FP64 oracle")
     ap.add_argument("--learned-size", type=int, default=512)
@@ -47,6 +46,9 @@ def parse():
     ap.add_argument("--learned-epochs", type=int, default=12)
     ap.add_argument("--learned-eval-seeds", type=int, default=4, help="held-out fields evaluated (1..4)")
     ap.add_argument("--series-timesteps", type=int, default=8, help="timesteps of the time series (1..8)")
+    ap.add_argument("--series-size", type=int, default=1024, help="points per axis of the time series")
+    ap.add_argument("--series-image", type=int, default=2048, help="time-series image width = height")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed runs of each CPU mode (median)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device 5 times (the byte-counter calibration run)")
     return ap.parse_args()
@@ -264,60 +266,267 @@ def learned_workload(steps, warmup, net, n=512, img=2048, seeds=(2001, 2002, 200
     }
 
 
-def time_series_workload(steps, warmup, net, n=1024, img=2048, timesteps=8, nbricks=128):
-    """Drifting 1024^3 FP16 field over 8 timesteps in 128 kd bricks (one ghost layer, maps at
-    half the voxel resolution), the shadow maps rebuilt every timestep, one hipGraph per frame
-    for the exact path; exact and learned side by side."""
+# ---- the headline: the time series (1024^3 FP16, 8 timesteps, 128 kd bricks, 2048^2) ------------
+
+SERIES = dict(n=1024, img=2048, nbricks=128, timesteps=8, seed=9, brick_seed=5, min_cells=96,
+              max_cells=320)
+
+
+def series_scene(ts, n, img):
+    """Timestep ts of the drifting field: 64 blobs moving 0.01 box/step + 3-octave value noise
+    (amplitude 0.2) whose phase advances 0.25/step, generated in HBM as FP16; FP16 maps."""
+    from pilot_amd import beams as B
+    from pilot_amd import scenes
+
+    return scenes.make_scene("series", n, seed=SERIES["seed"], nblobs=64, noise_octaves=3,
+                             noise_amp=0.2, light="point", width=img, height=img,
+                             field_dtype=B.F16, map_dtype=B.F16, device=True, time=float(ts),
+                             drift_speed=0.01, noise_phase_rate=0.25)
+
+
+def series_bricks(n):
+    from pilot_amd import beams as B
+
+    ranges = B.kd_bricks((n, n, n), SERIES["nbricks"], SERIES["brick_seed"],
+                         SERIES["min_cells"] * n // 1024, SERIES["max_cells"] * n // 1024)
+    return ranges, ranges[:, 1::2] - ranges[:, 0::2]  # a map cell per field cell
+
+
+def series_exact(steps, warmup, n, img, timesteps, keep_first=True):
+    """The exact-shadow frames of the time series.  One renderer per timestep holds that
+    timestep's field in its bricks (inputs resident in HBM before timing); every frame rebuilds
+    the maps (phase-1 local maps, the non-local exchange, the seeded re-march), marches every
+    brick and composites, as one hipGraph launch.  Untimed: per-timestep sample counts (one
+    counting frame) and per-phase HIP-event times (eager frames).  Timed: `steps` graph
+    launches cycling through the timesteps, bracketed by stream synchronisation."""
+    import numpy as np
+
+    from pilot_amd import beams as B
+    from pilot_amd import scenes
+    from pilot_amd.devmem import DeviceArray, synchronize
+
+    ranges, cells = series_bricks(n)
+    rgba = DeviceArray((img, img, 4), np.float32)
+    depth = DeviceArray((img, img), np.float32)
+    rs, samples, light_steps, phase_rows, first = [], [], [], [], None
+    for ts in range(timesteps):
+        sc = series_scene(ts, n, img)
+        r = B.LightedVolumeRenderer()
+        scenes.configure(r, sc)
+        r.SetBricks(ranges, ghost=1, map_cells=cells)
+        # eager frames with per-phase events (the first one also warms up)
+        r.RenderImage(sc.camera, img, img, rgba=rgba, depth=depth)
+        r.ResetTimings()
+        for _ in range(2):
+            r.RenderImage(sc.camera, img, img, rgba=rgba, depth=depth)
+        stats, _ = r.GetTimingStats()
+        phase_rows.append(stats)
+        r.SetSampleCounting(True)
+        r.RenderImage(sc.camera, img, img, rgba=rgba, depth=depth)
+        cs, ls = r.GetSampleCounts()
+        r.SetSampleCounting(False)
+        samples.append(cs)
+        light_steps.append(ls)
+        r.SetGraph(True)
+        r.RenderImage(sc.camera, img, img, rgba=rgba, depth=depth)  # capture
+        if ts == 0 and keep_first:
+            first = sc  # its device field stays for the CPU sample
+        else:
+            sc.field_store.free()
+        rs.append((r, sc.camera))
+        print(f"[bench] series t={ts}: eager {stats['frame']:.1f} ms (map {stats['map']:.1f}, exchange "
+              f"{stats['exchange']:.1f}, remap {stats['remap']:.1f}, march {stats['march']:.1f}), "
+              f"{cs} samples", file=sys.stderr, flush=True)
+    for k in range(warmup):
+        r, cam = rs[k % timesteps]
+        r.RenderImage(cam, img, img, rgba=rgba, depth=depth)
+    synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        r, cam = rs[k % timesteps]
+        r.RenderImage(cam, img, img, rgba=rgba, depth=depth)
+    synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    timed_samples = sum(samples[k % timesteps] for k in range(steps))
+    images = []
+    for ts, (r, cam) in enumerate(rs):
+        r.SetGraph(False)
+        im, _ = r.RenderImage(cam, img, img)
+        images.append(im.reshape(-1, 4))
+    phase = {k: float(np.mean([p[k] for p in phase_rows])) for k in phase_rows[0]}
+    out = dict(sec=sec, samples=samples, light_steps=light_steps, timed_samples=timed_samples,
+               phase=phase, phase_per_timestep=phase_rows, ranges=ranges, cells=cells,
+               images=images, first_scene=first, renderer0=rs[0][0])
+    for r, _ in rs[1:]:
+        r.close()
+    rgba.free()
+    depth.free()
+    return out
+
+
+def series_map_bytes(ranges, cells, ghost=1, n=1024):
+    """Algorithmic bytes of one k_bricks_map launch: every brick's ghosted FP16 field read
+    once, its FP16 map written once, the 256-entry TF read."""
+    import numpy as np
+
+    lo = np.maximum(ranges[:, 0::2] - ghost, 0)
+    hi = np.minimum(ranges[:, 1::2] + ghost, n - 1)
+    field = int(np.prod(hi - lo + 1, axis=1).sum()) * 2
+    maps = int(np.prod(cells + 1, axis=1).sum()) * 2
+    return field + maps + 256 * 16, field, maps
+
+
+def series_cpu_baseline(ex, n, img, threads, runs, per_brick=4, tile=256):
+    """The oracle (FP32 restatement of the reference) on a bounded sample of one timestep of
+    the time series, Serial (1 thread) and OpenMP (`threads`), median of `runs` after one
+    warm-up run each:
+      map:      `per_brick` ranges of 256 map vertices in every brick (phase-1 local march);
+      exchange: the same vertices' non-local seeds (hits, fetch from the other bricks' phase-1
+                maps, composite), from the GPU's phase-1 maps;
+      remap:    the same vertices' seeded re-march;
+      march:    every brick's CalcRayStart + Sampler over a tile x tile block of camera rays at
+                the image centre against the GPU's final maps, then the depth-ordered composite.
+    Every sampled value is compared with the GPU's (local and final map vertices, the tile's
+    RGBA): a full-size parity check.  The frame time is extrapolated: each map phase by the
+    frame's map vertices over the sampled ones, the march by the frame's camera samples over
+    the tile's."""
+    import dataclasses
+
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from pilot_amd import beams as B
+
+    sc = ex["first_scene"]
+    r0 = ex["renderer0"]
+    ranges, cells = ex["ranges"], ex["cells"]
+    f16 = sc.field_store.numpy()
+    sc.field_store.free()
+    field32 = f16.astype(np.float32)
+    del f16
+    scf = dataclasses.replace(sc, field32=field32)
+    vols = O.brick_volumes(scf, ranges, twin="f32", map_cells=cells)
+    del field32, scf
+    nb = len(vols)
+    nverts = [v.nverts for v in vols]
+    local = [r0.GetBrickMap(b, nverts[b], final=False) for b in range(nb)]
+    final = [r0.GetBrickMap(b, nverts[b], final=True) for b in range(nb)]
+    sel = []
+    for b in range(nb):
+        for q in range(per_brick):
+            lo = (nverts[b] - 256) * (2 * q + 1) // (2 * per_brick)
+            sel.append((b, lo, lo + 256))
+    sampled = 256 * len(sel)
+    o, d = B.camera_rays(sc.camera, img, img)
+    y0 = x0 = (img - tile) // 2
+    pix = ((np.arange(tile)[:, None] + y0) * img + (np.arange(tile)[None, :] + x0)).ravel()
+    to, td = np.ascontiguousarray(o[:, pix]), np.ascontiguousarray(d[:, pix])
+    bounds = []
+    for v in vols:
+        bounds += [float(v.coords[0][0]), float(v.coords[0][-1]), float(v.coords[1][0]),
+                   float(v.coords[1][-1]), float(v.coords[2][0]), float(v.coords[2][-1])]
+    bounds = np.array(bounds)
+    gpu_tile = ex["images"][0][pix]
+    r16 = lambda m: m.astype(np.float16).astype(np.float32)  # noqa: E731
+    scratch = [np.zeros(nv, np.float32) for nv in nverts]
+    seeds = [np.zeros(nv, np.float32) for nv in nverts]
+
+    def one_run(nthr):
+        t = {}
+        ok = {"local": True, "final": True, "tile": True}
+        t0 = time.perf_counter()
+        for b, lo, hi in sel:
+            O.tmap_build_range(vols[b], scratch[b], lo, hi, nthr)
+        t["map"] = time.perf_counter() - t0
+        for b, lo, hi in sel:
+            ok["local"] &= bool(np.array_equal(r16(scratch[b][lo:hi]), local[b][lo:hi]))
+        t0 = time.perf_counter()
+        for b, lo, hi in sel:
+            O.nonlocal_seed_range(vols, local, b, lo, hi, seeds[b], nthr)
+        t["exchange"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for b, lo, hi in sel:
+            O.tmap_build_range(vols[b], scratch[b], lo, hi, nthr, seed=seeds[b])
+        t["remap"] = time.perf_counter() - t0
+        for b, lo, hi in sel:
+            ok["final"] &= bool(np.array_equal(r16(scratch[b][lo:hi]), final[b][lo:hi]))
+        t0 = time.perf_counter()
+        partial, nsamp = [], 0
+        for b in range(nb):
+            rb = O.RayBuf(to, td, "f32")
+            O.sample_rays(vols[b], final[b], rb, nthr)
+            partial.append(rb.rgba)
+            nsamp += int(rb.nsamples.sum())
+        timg = O.composite(bounds, np.asarray(sc.camera.eye, np.float64), partial, "f32")
+        t["march"] = time.perf_counter() - t0
+        ok["tile"] = bool(np.array_equal(timg, gpu_tile))
+        return t, ok, nsamp
+
+    frame_samples = ex["samples"][0]
+    total_verts = int(sum(nverts))
+    modes = {}
+    for label, nthr in (("serial", 1), ("openmp", threads)):
+        one_run(nthr)  # warm-up
+        rows = [one_run(nthr) for _ in range(runs)]
+        med = {k: float(np.median([r[0][k] for r in rows])) for k in rows[0][0]}
+        tile_samples = rows[0][2]
+        frame_s = (med["map"] + med["exchange"] + med["remap"]) * total_verts / sampled + \
+            med["march"] * frame_samples / max(tile_samples, 1)
+        modes[label] = {"threads": nthr, "median_s": med, "tile_samples": tile_samples,
+                        "frame_s_extrapolated": frame_s, "ray_samples_per_s": frame_samples / frame_s,
+                        "equal_gpu": {k: all(r[1][k] for r in rows) for k in rows[0][1]}}
+        print(f"[bench] cpu {label}: {med} -> frame {frame_s:.1f} s (extrapolated)", file=sys.stderr,
+              flush=True)
+    omp = modes["openmp"]
+    return {
+        "value": omp["ray_samples_per_s"],
+        "unit": "ray samples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"FP32 oracle on timestep 0, OpenMP {threads} threads, median of {runs}: "
+                   f"{per_brick} ranges of 256 map vertices in each of the {nb} bricks ({sampled} of "
+                   f"{total_verts} vertices: local march, exchange, seeded re-march) and the "
+                   f"{tile}x{tile} centre tile ({omp['tile_samples']} of {frame_samples} camera samples); "
+                   f"frame time extrapolated per vertex and per sample to "
+                   f"{omp['frame_s_extrapolated']:.1f} s"),
+        "modes": modes,
+        "equal_gpu": {k: modes["serial"]["equal_gpu"][k] and omp["equal_gpu"][k] for k in omp["equal_gpu"]},
+    }
+
+
+def series_learned(steps, net, ex, n, img, timesteps):
+    """The learned-shadow frames of the same timesteps, one renderer at a time: frame ms,
+    samples/s and RGBA error against the exact frame of the same timestep."""
     import numpy as np
 
     from pilot_amd import beams as B
     from pilot_amd import scenes
     from pilot_amd.devmem import DeviceArray
 
-    ranges = B.kd_bricks((n, n, n), nbricks, 5, 96, 320)
-    cells = ranges[:, 1::2] - ranges[:, 0::2]  # one map cell per field cell
+    ranges, cells = ex["ranges"], ex["cells"]
     rgba = DeviceArray((img, img, 4), np.float32)
     depth = DeviceArray((img, img), np.float32)
     rows = []
     for ts in range(timesteps):
-        sc = scenes.make_scene("series", n, seed=9, nblobs=64, noise_octaves=3, noise_amp=0.2,
-                               light="point", width=img, height=img, field_dtype=B.F16,
-                               map_dtype=B.F16, device=True, time=float(ts), drift_speed=0.01,
-                               noise_phase_rate=0.25)
-        row = {"timestep": ts}
-        imgs = {}
-        for label, mode in (("exact", B.SHADOW_REF_MARCH), ("learned", B.SHADOW_LEARNED)):
-            with B.LightedVolumeRenderer() as r:
-                scenes.configure(r, sc)
-                r.SetShadow(True, sc.map_cells, B.F16, mode=mode)
-                if mode == B.SHADOW_LEARNED:
-                    r.SetLearned(net, fallback=B.SHADOW_REF_MARCH, error_budget=2e-3, check_stride=8)
-                r.SetBricks(ranges, ghost=1, map_cells=cells)
-                r.SetGraph(True)
-                sec, stats, cam_s, _ = frame_loop(r, sc, rgba, depth, steps, warmup)
-                imgs[label] = rgba.numpy()
-                row[label] = {"frame_ms": sec * 1e3, "phase_ms": stats, "samples_per_frame": cam_s,
-                              "ray_samples_per_s": cam_s / sec}
-                if mode == B.SHADOW_LEARNED:
-                    e_l, _, fb = r.GetLearnedStats()
-                    row[label].update({"check_max_opacity_err": e_l, "bricks_fell_back": fb})
-            print(f"[bench] series t={ts} {label}: {sec * 1e3:.2f} ms", file=sys.stderr, flush=True)
-        row["learned"]["rgba_err_vs_exact"] = image_error(imgs["learned"], imgs["exact"])
-        rows.append(row)
+        sc = series_scene(ts, n, img)
+        with B.LightedVolumeRenderer() as r:
+            scenes.configure(r, sc)
+            r.SetShadow(True, sc.map_cells, B.F16, mode=B.SHADOW_LEARNED)
+            r.SetLearned(net, fallback=B.SHADOW_REF_MARCH, error_budget=2e-3, check_stride=8)
+            r.SetBricks(ranges, ghost=1, map_cells=cells)
+            r.SetGraph(True)
+            sec, stats, cam_s, _ = frame_loop(r, sc, rgba, depth, steps, 1)
+            im = rgba.numpy().reshape(-1, 4)
+            e_l, _, fb = r.GetLearnedStats()
         sc.field_store.free()
+        rows.append({"timestep": ts, "frame_ms": sec * 1e3, "phase_ms": stats, "samples_per_frame": cam_s,
+                     "ray_samples_per_s": cam_s / sec, "check_max_opacity_err": e_l, "bricks_fell_back": fb,
+                     "rgba_err_vs_exact": image_error(im, ex["images"][ts])})
+        print(f"[bench] series learned t={ts}: {sec * 1e3:.1f} ms, fell back {fb}", file=sys.stderr, flush=True)
     rgba.free()
     depth.free()
-    ext = ranges[:, 1::2] - ranges[:, 0::2]
-    return {
-        "workload": (f"time series: {n}^3 FP16 blob+noise field, {timesteps} timesteps (blob drift 0.01 "
-                     f"box/step, noise phase +0.25/step), {nbricks} kd bricks (cell extents "
-                     f"{int(ext.min())}..{int(ext.max())}, 1 ghost layer, a map cell per field cell), "
-                     f"{img}x{img}, maps rebuilt every timestep; exact = multi-rank exchange in one "
-                     "hipGraph per frame, learned = per-brick closure with per-brick fallback"),
-        "frame_ms_exact": float(np.mean([r["exact"]["frame_ms"] for r in rows])),
-        "frame_ms_learned": float(np.mean([r["learned"]["frame_ms"] for r in rows])),
-        "per_timestep": rows,
-    }
+    return {"frame_ms": float(np.mean([r["frame_ms"] for r in rows])), "per_timestep": rows}
 
 
 def precision_report(n, img, threads, net=None):
@@ -401,7 +610,7 @@ def valu_issue(kernel, workload, kernel_ms):
             "kernel_ms": kernel_ms, "frac": issue_ms / kernel_ms}
 
 
-def cpu_baseline(sc, field32, gpu_map, threads, samples_per_frame, gpu_rgba=None):
+def c2_cpu_check(sc, field32, gpu_map, threads, samples_per_frame, gpu_rgba=None):
     """Oracle (FP32 restatement of the reference) over the whole frame on the host cores:
     every map vertex (checked against the GPU map) and every camera ray, shaded against the
     bit-identical GPU map and compared with the GPU frame (gpu_rgba, W*H x 4).  The frame is
@@ -444,31 +653,20 @@ def cpu_baseline(sc, field32, gpu_map, threads, samples_per_frame, gpu_rgba=None
     }
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this renderer runs on one GPU; bench.py measures --gpus 1 only",
-                          "n_gpus_requested": args.gpus}))
-        sys.exit(2)
+def c2_workload(args):
+    """The precision-sweep frame (256^3 FP32 field, 256^3-cell FP32 map, 1024^2, reference
+    march): eager frames with phase events and hipGraph replays; FP16/BF16 storage, cell-field
+    and rectilinear variants; the FP32 oracle over the whole frame on the host cores (map and
+    image compared with the GPU's)."""
     import numpy as np
 
     from pilot_amd import beams as B
     from pilot_amd import scenes
-    from pilot_amd.devmem import D2D, DeviceArray, hip
-
-    if args.copy_calibration:
-        a = DeviceArray((1 << 28,), np.float32).zero()
-        b = DeviceArray((1 << 28,), np.float32)
-        for _ in range(5):
-            hip().hipMemcpy(b.data_ptr(), a.data_ptr(), a.nbytes, D2D)
-        hip().hipDeviceSynchronize()
-        print(json.dumps({"copy_calibration_bytes": a.nbytes, "copies": 5}))
-        return
+    from pilot_amd.devmem import DeviceArray
 
     n, img = args.size, args.image
 
     def scene(fdt, mdt):
-        # a map of n cells per axis: (n+1)^3 vertices (SetShadowMapSize(n, n, n))
         return scenes.make_scene("precision", n, seed=2, nblobs=64, noise_octaves=3, noise_amp=0.2,
                                  light="point", width=img, height=img, field_dtype=fdt,
                                  map_dtype=mdt, map_cells=(n, n, n), device=True)
@@ -487,20 +685,15 @@ def main():
         sec, stats, cam_s, light_s = frame_loop(r, sc, rgba, depth, args.steps, args.warmup)
         results[label] = dict(frame_ms=sec * 1e3, map_ms=stats["map"], march_ms=stats["march"],
                               samples_per_frame=cam_s, light_steps_per_frame=light_s)
-        print(f"[bench] {label}: {sec * 1e3:.3f} ms/frame, map {stats['map']:.3f} ms, "
-              f"march {stats['march']:.3f} ms, {cam_s} samples", file=sys.stderr, flush=True)
         if label == "f32/f32":
-            # the headline frame as the renderer replays it: one hipGraph of map + march
             gsec = graph_frame_s(r, sc, rgba, depth, args.steps)
-            print(f"[bench] {label}: {gsec * 1e3:.3f} ms/frame as hipGraph replays", file=sys.stderr,
-                  flush=True)
-            main_state = (sc, r, sec, gsec, stats, cam_s, light_s)
+            results[label].update(frame_ms_graph=gsec * 1e3, ray_samples_per_s=cam_s / gsec)
+            main_state = (sc, r, stats, cam_s)
         else:
             r.close()
-
-    sc, r, sec_eager, sec, stats, cam_s, light_s = main_state
+        print(f"[bench] precision frame {label}: {sec * 1e3:.3f} ms/frame eager", file=sys.stderr, flush=True)
+    sc, r, stats, cam_s = main_state
     if not args.no_variants:
-        # SamplerCellAssoc: the same field as (n-1)^3 cell values, no shadow term
         cells = np.ascontiguousarray(sc.field_store.numpy().reshape(n, n, n)[:-1, :-1, :-1]).ravel()
         rgba = DeviceArray((img, img, 4), np.float32)
         depth = DeviceArray((img, img), np.float32)
@@ -513,10 +706,6 @@ def main():
             csec, cstats, ccam, _ = frame_loop(rc, sc, rgba, depth, args.steps, args.warmup)
         results["f32 cells, no shadow"] = dict(frame_ms=csec * 1e3, march_ms=cstats["march"],
                                                samples_per_frame=ccam)
-        print(f"[bench] f32 cell field, no shadow: {csec * 1e3:.3f} ms/frame, {ccam} samples",
-              file=sys.stderr, flush=True)
-        # RectilinearLocator: the same values on unevenly spaced coordinates (cell widths
-        # varying 3x along each axis), exact shadows
         rng = np.random.default_rng(7)
         rcoords = []
         for _ in range(3):
@@ -532,25 +721,66 @@ def main():
         results["f32 rectilinear coordinates"] = dict(frame_ms=rsec * 1e3, map_ms=rstats["map"],
                                                       march_ms=rstats["march"], samples_per_frame=rcam,
                                                       light_steps_per_frame=rlight)
-        print(f"[bench] f32 rectilinear coordinates: {rsec * 1e3:.3f} ms/frame, map "
-              f"{rstats['map']:.3f} ms, march {rstats['march']:.3f} ms, {rcam} samples",
-              file=sys.stderr, flush=True)
         rgba.free()
         depth.free()
     npts = n ** 3
     nmap = int(np.prod([c + 1 for c in sc.map_cells]))
-    npix = img * img
     bytes_map_kernel = npts * 4 + nmap * 4 + 256 * 16
-    bytes_march_kernel = npts * 4 + nmap * 4 + npix * 20 + 256 * 16
-    if stats["map"] >= stats["march"]:
-        dom, dom_ms, dom_bytes = "k_tmap_ref", stats["map"], bytes_map_kernel
-    else:
-        dom, dom_ms, dom_bytes = "k_march_image", stats["march"], bytes_march_kernel
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    counted = f"{n}^3 f32/f32, {sc.map_cells[0]}^3-cell map, {img}x{img}"  # counter-run key
+    out = {
+        "workload": (f"precision sweep, exact shadows: {n}^3 FP32 field, {n}^3-cell FP32 map (reference "
+                     f"per-vertex march), point light, {img}x{img}, step 0.5 voxel"),
+        "frame_ms_graph": results["f32/f32"]["frame_ms_graph"],
+        "ray_samples_per_s": results["f32/f32"]["ray_samples_per_s"],
+        "phase_ms": {"map": stats["map"], "march": stats["march"]},
+        "k_tmap_ref": {"algorithmic_bytes": bytes_map_kernel, "ms": stats["map"],
+                       "achieved_gbs": bytes_map_kernel / (stats["map"] * 1e-3) / 1e9,
+                       "traffic": measured_traffic("k_tmap_ref", f"{n}^3 f32/f32, {n}^3-cell map, {img}x{img}"),
+                       "valu_issue": valu_issue("k_tmap_ref", f"{n}^3 f32/f32, {n}^3-cell map, {img}x{img}",
+                                                stats["map"])},
+        "variants": results,
+    }
+    if not args.no_cpu:
+        field32 = sc.field_store.numpy().astype(np.float32)
+        gimg, _ = r.RenderImage(sc.camera, sc.width, sc.height)
+        gmap = r.GetMap(sc.map_cells)
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_full_frame"] = c2_cpu_check(sc, field32, gmap, threads, cam_s, gimg)
+    r.close()
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this renderer runs on one GPU; bench.py measures --gpus 1 only",
+                          "n_gpus_requested": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+
+    from pilot_amd.devmem import D2D, DeviceArray, hip
+
+    if args.copy_calibration:
+        a = DeviceArray((1 << 28,), np.float32).zero()
+        b = DeviceArray((1 << 28,), np.float32)
+        for _ in range(5):
+            hip().hipMemcpy(b.data_ptr(), a.data_ptr(), a.nbytes, D2D)
+        hip().hipDeviceSynchronize()
+        print(json.dumps({"copy_calibration_bytes": a.nbytes, "copies": 5}))
+        return
+
+    n, img, T = args.series_size, args.series_image, max(1, min(8, args.series_timesteps))
+    nb = SERIES["nbricks"]
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    ex = series_exact(args.steps, args.warmup, n, img, T)
+    sec = ex["sec"]
+    map_bytes, field_bytes, maps_bytes = series_map_bytes(ex["ranges"], ex["cells"], 1, n)
+    map_ms = ex["phase"]["map"]
+    achieved = map_bytes / (map_ms * 1e-3) / 1e9
+    counted = f"series {n}^3 f16/f16, {nb} bricks, {img}x{img}"  # counter-run key
+    ext = ex["ranges"][:, 1::2] - ex["ranges"][:, 0::2]
     line = {
         "metric": METRIC,
-        "value": cam_s / sec,
+        "value": ex["timed_samples"] / (sec * args.steps),
         "unit": "ray samples/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -560,43 +790,61 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: seeded 64-blob + 3-octave value-noise field generated in HBM, seeded TF/light/camera",
+        "data": ("synthetic: seeded 64-blob + 3-octave value-noise fields (blobs drifting 0.01 box/step, "
+                 "noise phase +0.25/step) generated in HBM, seeded TF/light/camera/kd bricks"),
         "config": {
-            "workload": (f"precision sweep, exact shadows: {n}^3 FP32 field, {n}^3-cell FP32 "
-                         f"transmittance map (reference per-vertex march), point light, "
-                         f"{img}x{img} image, step 0.5 voxel"),
-            "field": f"{n}^3", "map_vertices": nmap, "image": f"{img}x{img}",
-            "shadow": "exact (reference per-vertex march)", "storage": "f32 field / f32 map",
+            "workload": (f"time series, exact shadows: {n}^3 FP16 field (FP32 arithmetic), {T} timesteps "
+                         f"cycled, {nb} kd bricks (cell extents {int(ext.min())}..{int(ext.max())}, 1 ghost "
+                         f"layer, FP16 maps with a cell per field cell), maps rebuilt every frame (local "
+                         f"march, non-local exchange, seeded re-march), per-brick march, depth-ordered "
+                         f"composite, {img}x{img}, step 0.5 voxel, one hipGraph per frame"),
+            "field": f"{n}^3", "bricks": nb, "timesteps": T, "image": f"{img}x{img}",
+            "map_vertices": int(np.prod(ex["cells"] + 1, axis=1).sum()),
+            "shadow": "exact (reference per-vertex march per brick + exchange)",
+            "storage": "f16 field / f16 maps",
         },
         "frame_ms": sec * 1e3,
-        "frame_ms_eager": sec_eager * 1e3,
-        "timing": ("value and ms_per_step: K hipGraph replays of the whole frame (map + march) "
-                   "between stream synchronisations; phase_ms: HIP events around each phase "
-                   "of K eager frames"),
-        "phase_ms": {"map": stats["map"], "march": stats["march"]},
-        "samples_per_frame": cam_s,
-        "light_steps_per_frame": light_s,
-        "variants": results,
+        "timing": ("value and ms_per_step: K hipGraph launches of whole frames cycling through the "
+                   "timesteps between stream synchronisations; value = their camera-ray samples "
+                   "(counted on the GPU per timestep) / their time; phase_ms: HIP events around each "
+                   "phase of 2 eager frames per timestep on the renderer's stream"),
+        "phase_ms": ex["phase"],
+        "samples_per_frame": ex["samples"],
+        "light_steps_per_frame": ex["light_steps"],
         "roofline": {
-            "kernel": dom,
+            "kernel": "k_bricks_map (phase-1 local maps)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom, counted),
-            "algorithmic_bytes": dom_bytes,
+            "traffic": measured_traffic("k_bricks_map", counted),
+            "algorithmic_bytes": map_bytes,
+            "algorithmic_bytes_parts": {"brick_fields_f16": field_bytes, "maps_f16": maps_bytes,
+                                        "tf": 256 * 16},
+            "launch_ms": map_ms,
             # the bound that applies: instruction issue, not HBM
-            "valu_issue": valu_issue(dom, counted, dom_ms),
+            "valu_issue": valu_issue("k_bricks_map", counted, map_ms),
         },
         "cpu_baseline": None,
+        "workloads": {},
     }
+    print(f"[bench] headline: {sec * 1e3:.1f} ms/frame, {line['value']:.4g} samples/s", file=sys.stderr,
+          flush=True)
+    if not args.no_cpu:
+        line["cpu_baseline"] = series_cpu_baseline(ex, n, img, threads, args.cpu_runs)
+    else:
+        ex["first_scene"].field_store.free()
+    ex["renderer0"].close()
+    exact_images = ex["images"]
+    del ex
+    if not args.no_variants:
+        line["workloads"]["precision_frame"] = c2_workload(args)
     if not args.no_bricks:
-        line["workloads"] = {"bricks_512_96": bricks_workload(max(3, args.steps // 2), 1)}
+        line["workloads"]["bricks_512_96"] = bricks_workload(max(3, args.steps // 2), 1)
         print(f"[bench] bricks: {line['workloads']['bricks_512_96']['frame_ms_graph']:.3f} ms/frame",
               file=sys.stderr, flush=True)
     net = None
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     if not args.no_learned:
         seeds = tuple(1001 + i for i in range(8))
         net, train = train_closure(args.learned_size, seeds, args.learned_samples // len(seeds),
@@ -605,20 +853,15 @@ def main():
         wl = learned_workload(max(2, args.steps // 4), 1, net, n=args.learned_size,
                               seeds=(2001, 2002, 2003, 2004)[:max(1, min(4, args.learned_eval_seeds))])
         wl["training"] = train
-        line.setdefault("workloads", {})["learned"] = wl
-        if not args.no_series:
-            line["workloads"]["time_series"] = time_series_workload(
-                2, 1, net, timesteps=max(1, min(8, args.series_timesteps)))
+        line["workloads"]["learned"] = wl
+        sl = series_learned(2, net, {"ranges": series_bricks(n)[0], "cells": series_bricks(n)[1],
+                                     "images": exact_images}, n, img, T)
+        sl["speedup_vs_exact"] = line["frame_ms"] / sl["frame_ms"]
+        line["workloads"]["series_learned"] = sl
     if not args.no_precision:
-        line["precision"] = precision_report(n, img, threads, net)
+        line["precision"] = precision_report(args.size, args.image, threads, net)
     if net is not None:
         net.close()
-    if not args.no_cpu:
-        field32 = sc.field_store.numpy().astype(np.float32)
-        gimg, _ = r.RenderImage(sc.camera, sc.width, sc.height)  # this frame's RGBA on the host
-        gmap = r.GetMap(sc.map_cells)
-        line["cpu_baseline"] = cpu_baseline(sc, field32, gmap, threads, cam_s, gimg)
-    r.close()
     print(json.dumps(line))
 
 

@@ -62,8 +62,8 @@ typedef struct OrcRays {
   int64_t orc_find_cell_##SUF(const OrcScene* s, const void* point, void* pcoords);        \
   double orc_interp_hex_##SUF(const void* values8, const void* pcoords);                   \
   int orc_tmap_build_##SUF(const OrcScene* s, const void* seed, void* map_out, int nthr);  \
-  int orc_tmap_build_range_##SUF(const OrcScene* s, void* map_out, int64_t begin,          \
-                                 int64_t end, int nthr);                                   \
+  int orc_tmap_build_range_##SUF(const OrcScene* s, const void* seed, void* map_out,      \
+                                 int64_t begin, int64_t end, int nthr);                    \
   int orc_map_estimate_##SUF(const OrcScene* s, const void* map, const void* point,        \
                              void* rgb_out, void* opacity_out);                            \
   int orc_calc_ray_start_##SUF(const OrcScene* s, OrcRays* rays, int nthr);                \
@@ -74,6 +74,9 @@ typedef struct OrcRays {
                                   int64_t hits_cap);                                       \
   int orc_nonlocal_seed_##SUF(int nblocks, const OrcScene* scenes, const void* const* maps,\
                               int self, void* seed_out, int nthr);                         \
+  int orc_nonlocal_seed_range_##SUF(int nblocks, const OrcScene* scenes,                   \
+                                    const void* const* maps, int self, int64_t begin,      \
+                                    int64_t end, void* seed_out, int nthr);                \
   int orc_composite_##SUF(int nblocks, const double* block_bounds, const double* cam_pos,  \
                           const void* const* images, int64_t npix, void* out);
 

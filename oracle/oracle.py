@@ -80,13 +80,14 @@ def load(path):
             "orc_find_cell": [VP, VP, VP],
             "orc_interp_hex": [VP, VP],
             "orc_tmap_build": [VP, VP, VP, I32],
-            "orc_tmap_build_range": [VP, VP, I64, I64, I32],
+            "orc_tmap_build_range": [VP, VP, VP, I64, I64, I32],
             "orc_map_estimate": [VP, VP, VP, VP, VP],
             "orc_calc_ray_start": [VP, VP, I32],
             "orc_sample_rays": [VP, VP, VP, I32],
             "orc_render_rays": [VP, VP, VP, I32],
             "orc_nonlocal_hits": [I32, VP, I32, VP, VP, VP, I64],
             "orc_nonlocal_seed": [I32, VP, VP, I32, VP, I32],
+            "orc_nonlocal_seed_range": [I32, VP, VP, I32, I64, I64, VP, I32],
             "orc_composite": [I32, VP, VP, VP, I64, VP],
         }
         for name, args in sig.items():
@@ -183,10 +184,24 @@ def tmap_build(vol, seed=None, nthreads=8):
     return out
 
 
-def tmap_build_range(vol, out, begin, end, nthreads=8):
-    """Map vertices [begin, end) into out (a full-size map array)."""
-    getattr(lib(), f"orc_tmap_build_range_{vol.twin}")(C.byref(vol.s), out.ctypes.data, int(begin),
+def tmap_build_range(vol, out, begin, end, nthreads=8, seed=None):
+    """Map vertices [begin, end) into out (a full-size map array); seed (full-size, optional)
+    holds the phase-3 seed opacities."""
+    sd = None if seed is None else seed.ctypes.data
+    getattr(lib(), f"orc_tmap_build_range_{vol.twin}")(C.byref(vol.s), sd, out.ctypes.data, int(begin),
                                                        int(end), nthreads)
+    return out
+
+
+def nonlocal_seed_range(vols, maps, self_b, begin, end, out, nthreads=8):
+    """Seed opacities (non-local hits, fetch and composite) of block self_b's map vertices
+    [begin, end) into out (full-size), from every block's phase-1 map."""
+    twin = vols[0].twin
+    arr = (OrcScene * len(vols))(*[v.s for v in vols])
+    mp = [np.ascontiguousarray(m, _dt(twin)) for m in maps]
+    mptr = (C.c_void_p * len(vols))(*[m.ctypes.data for m in mp])
+    getattr(lib(), f"orc_nonlocal_seed_range_{twin}")(len(vols), arr, mptr, int(self_b), int(begin),
+                                                      int(end), out.ctypes.data, nthreads)
     return out
 
 

@@ -301,19 +301,21 @@ int FN(orc_tmap_build)(const OrcScene* s, const void* seed, void* map_out, int n
   return 0;
 }
 
-/* The map vertices [begin, end) only (a slab of the map, for timing samples). */
-int FN(orc_tmap_build_range)(const OrcScene* s, void* map_out, int64_t begin, int64_t end,
-                             int nthr)
+/* The map vertices [begin, end) only (a slab of the map, for timing samples); seed, when
+ * given, is a full-size array of phase-3 seed opacities. */
+int FN(orc_tmap_build_range)(const OrcScene* s, const void* seed, void* map_out, int64_t begin,
+                             int64_t end, int nthr)
 {
   FN(Geo) g;
   FN(geo_init)(&g, s);
   R* out = (R*)map_out;
+  const R* sd = (const R*)seed;
   const R* field = (const R*)s->field;
   const int64_t nf = FN(nfield)(s);
   (void)nthr;
-#pragma omp parallel for schedule(dynamic, 256) num_threads(nthr > 0 ? nthr : 1)
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nthr > 0 ? nthr : 1)
   for (int64_t id = begin; id < end; ++id)
-    out[id] = FN(march_vertex)(&g, field, nf, s->light_kind, id, (R)0);
+    out[id] = FN(march_vertex)(&g, field, nf, s->light_kind, id, sd ? sd[id] : (R)0);
   return 0;
 }
 
@@ -645,16 +647,17 @@ typedef struct {
   R opacity;
 } FN(Hit);
 
-/* One pass over the map vertices of block `self`: the number of other blocks each light
- * segment crosses (CountNonLocalBlockHits, into counts), or, with hits given, the hit records
- * at offsets (CalculateNonLocalBlockHits).  Vertices are independent: nthr threads share them. */
+/* One pass over the map vertices [begin, end) of block `self`: the number of other blocks each
+ * light segment crosses (CountNonLocalBlockHits, into counts), or, with hits given, the hit
+ * records at offsets (CalculateNonLocalBlockHits); counts and offsets are indexed from begin.
+ * Vertices are independent: nthr threads share them. */
 static void FN(hits_pass)(const FN(Geo) * g, const double* bb, int nblocks, int self, int kind,
-                          int64_t* counts, const int64_t* offsets, FN(Hit) * hits, int nthr)
+                          int64_t* counts, const int64_t* offsets, FN(Hit) * hits, int64_t begin,
+                          int64_t end, int nthr)
 {
-  const int64_t nv = g->mp[0] * g->mp[1] * g->mp[2];
   (void)nthr;
-#pragma omp parallel for schedule(dynamic, 1024) num_threads(nthr > 0 ? nthr : 1)
-  for (int64_t id = 0; id < nv; ++id) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthr > 0 ? nthr : 1)
+  for (int64_t id = begin; id < end; ++id) {
     R v[3], o[3];
     FN(map_vertex)(g, id, v);
     FN(light_origin)(g, kind, v, o);
@@ -665,13 +668,13 @@ static void FN(hits_pass)(const FN(Geo) * g, const double* bb, int nblocks, int 
         R t0, t1;
         if (FN(segment_aabb)(o, v, bb + 6 * b, (R)1e-4f, &t0, &t1)) ++n; /* glancing hits kept */
       }
-      counts[id] = n;
+      counts[id - begin] = n;
       continue;
     }
     R dd[3] = { v[0] - o[0], v[1] - o[1], v[2] - o[2] };
     R rs = (R)1 / SQRT_(FN(dot3)(dd, dd)); /* vtkm::Normalize */
     R dir[3] = { dd[0] * rs, dd[1] * rs, dd[2] * rs };
-    int64_t off = offsets ? offsets[id] : 0;
+    int64_t off = offsets ? offsets[id - begin] : 0;
     for (int b = 0; b < nblocks; ++b) {
       if (b == self) continue;
       R t0, t1;
@@ -703,14 +706,14 @@ static int64_t FN(nonlocal_hits)(int nblocks, const OrcScene* scenes, int self, 
   }
   const int kind = scenes[self].light_kind;
   int64_t* cnt = counts ? counts : (int64_t*)malloc(sizeof(int64_t) * (size_t)nv);
-  FN(hits_pass)(&g, bb, nblocks, self, kind, cnt, 0, 0, nthr);
+  FN(hits_pass)(&g, bb, nblocks, self, kind, cnt, 0, 0, 0, nv, nthr);
   int64_t total = 0;
   for (int64_t id = 0; id < nv; ++id) {
     if (offsets) offsets[id] = total;
     total += cnt[id];
   }
   if (hits_out && hits_cap >= total)
-    FN(hits_pass)(&g, bb, nblocks, self, kind, cnt, offsets, (FN(Hit)*)hits_out, nthr);
+    FN(hits_pass)(&g, bb, nblocks, self, kind, cnt, offsets, (FN(Hit)*)hits_out, 0, nv, nthr);
   if (!counts) free(cnt);
   free(bb);
   return total;
@@ -732,14 +735,21 @@ static int FN(hit_less)(const FN(Hit) * a, const FN(Hit) * b)
   return a->block_id < b->block_id; /* tie order fixed by block id */
 }
 
-int FN(orc_nonlocal_seed)(int nblocks, const OrcScene* scenes, const void* const* maps, int self,
-                          void* seed_out, int nthr)
+/* The seed opacities of the map vertices [begin, end) of block `self` (seed_out is
+ * full-size; other entries are left alone). */
+int FN(orc_nonlocal_seed_range)(int nblocks, const OrcScene* scenes, const void* const* maps,
+                                int self, int64_t begin, int64_t end, void* seed_out, int nthr)
 {
   FN(Geo) g;
   FN(geo_init)(&g, &scenes[self]);
   const int64_t nv = g.mp[0] * g.mp[1] * g.mp[2];
-  int64_t* counts = (int64_t*)malloc(sizeof(int64_t) * (size_t)nv);
-  int64_t* offs = (int64_t*)malloc(sizeof(int64_t) * (size_t)nv);
+  if (begin < 0) begin = 0;
+  if (end > nv) end = nv;
+  if (end <= begin) return 0;
+  const int64_t nr = end - begin;
+  /* counts and offsets of the range, indexed from `begin` */
+  int64_t* counts = (int64_t*)malloc(sizeof(int64_t) * (size_t)nr);
+  int64_t* offs = (int64_t*)malloc(sizeof(int64_t) * (size_t)nr);
   FN(Geo)* gs = (FN(Geo)*)malloc(sizeof(FN(Geo)) * (size_t)nblocks);
   double* bb = (double*)malloc(sizeof(double) * 6 * (size_t)nblocks);
   for (int b = 0; b < nblocks; ++b) {
@@ -747,26 +757,26 @@ int FN(orc_nonlocal_seed)(int nblocks, const OrcScene* scenes, const void* const
     memcpy(bb + 6 * b, gs[b].bounds, sizeof(double) * 6);
   }
   const int kind = scenes[self].light_kind;
-  FN(hits_pass)(&g, bb, nblocks, self, kind, counts, 0, 0, nthr);
+  FN(hits_pass)(&g, bb, nblocks, self, kind, counts, 0, 0, begin, end, nthr);
   int64_t total = 0;
-  for (int64_t id = 0; id < nv; ++id) {
-    offs[id] = total;
-    total += counts[id];
+  for (int64_t i = 0; i < nr; ++i) {
+    offs[i] = total;
+    total += counts[i];
   }
   FN(Hit)* hits = (FN(Hit)*)malloc(sizeof(FN(Hit)) * (size_t)(total > 0 ? total : 1));
-  FN(hits_pass)(&g, bb, nblocks, self, kind, counts, offs, hits, nthr);
+  FN(hits_pass)(&g, bb, nblocks, self, kind, counts, offs, hits, begin, end, nthr);
   free(bb);
-#pragma omp parallel for schedule(dynamic, 4096) num_threads(nthr > 0 ? nthr : 1)
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthr > 0 ? nthr : 1)
   for (int64_t h = 0; h < total; ++h) {
     int b = hits[h].block_id;
     hits[h].opacity = FN(map_opacity)(&gs[b], (const R*)maps[b], hits[h].point);
   }
   R* seed = (R*)seed_out;
-#pragma omp parallel for schedule(dynamic, 1024) num_threads(nthr > 0 ? nthr : 1)
-  for (int64_t id = 0; id < nv; ++id) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthr > 0 ? nthr : 1)
+  for (int64_t id = begin; id < end; ++id) {
     /* insertion sort of this vertex's hits by RayT (a vertex crosses few blocks) */
-    FN(Hit)* hs = hits + offs[id];
-    int64_t n = counts[id];
+    FN(Hit)* hs = hits + offs[id - begin];
+    int64_t n = counts[id - begin];
     for (int64_t i = 1; i < n; ++i) {
       FN(Hit) t = hs[i];
       int64_t j = i - 1;
@@ -785,6 +795,15 @@ int FN(orc_nonlocal_seed)(int nblocks, const OrcScene* scenes, const void* const
   free(offs);
   free(counts);
   return 0;
+}
+
+int FN(orc_nonlocal_seed)(int nblocks, const OrcScene* scenes, const void* const* maps, int self,
+                          void* seed_out, int nthr)
+{
+  FN(Geo) g;
+  FN(geo_init)(&g, &scenes[self]);
+  return FN(orc_nonlocal_seed_range)(nblocks, scenes, maps, self, 0, g.mp[0] * g.mp[1] * g.mp[2],
+                                     seed_out, nthr);
 }
 
 /* ---- MapperLightedVolume.cxx:215-303 --------------------------------------------------- */

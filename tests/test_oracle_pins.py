@@ -835,3 +835,33 @@ def pin_locate_cell_uneven(O):
 
 def test_locate_cell_uneven_spacing(oracle_mod):
     pin_locate_cell_uneven(oracle_mod)
+
+
+def test_range_variants_equal_whole_maps(oracle_mod):
+    """The ranged seed and seeded map builds (used for timing samples of large frames) give the
+    values of the whole-map builds on their vertices, and leave other vertices alone."""
+    from pilot_amd import beams as B
+    from pilot_amd import scenes
+
+    O = oracle_mod
+    sc = scenes.make_scene("rng", 29, seed=4, nblobs=6, light="point")
+    ranges = B.kd_bricks(sc.dims, 8, 7, 8, 20)
+    vols = O.brick_volumes(sc, ranges)
+    local = [O.tmap_build(v) for v in vols]
+    arr = (O.OrcScene * len(vols))(*[v.s for v in vols])
+    import ctypes as C
+
+    mptr = (C.c_void_p * len(vols))(*[m.ctypes.data for m in local])
+    for b in (0, 3):
+        v = vols[b]
+        seed = np.zeros(v.nverts, np.float32)
+        O.lib().orc_nonlocal_seed_f32(len(vols), arr, mptr, b, seed.ctypes.data, 4)
+        final = O.tmap_build(v, seed)
+        lo, hi = v.nverts // 5, v.nverts // 5 + 777
+        s2 = np.full(v.nverts, -7.0, np.float32)
+        O.nonlocal_seed_range(vols, local, b, lo, hi, s2)
+        assert np.array_equal(s2[lo:hi], seed[lo:hi]) and np.all(s2[:lo] == -7.0) and np.all(s2[hi:] == -7.0)
+        f2 = np.full(v.nverts, -7.0, np.float32)
+        O.tmap_build_range(v, f2, lo, hi, seed=s2)
+        assert np.array_equal(f2[lo:hi], final[lo:hi]) and np.all(f2[hi:] == -7.0)
+        assert (seed[lo:hi] > 0).any() or b == 0
