@@ -309,11 +309,31 @@ def series_exact(steps, warmup, n, img, timesteps, keep_first=True):
     rgba = DeviceArray((img, img, 4), np.float32)
     depth = DeviceArray((img, img), np.float32)
     rs, samples, light_steps, phase_rows, first = [], [], [], [], None
+    schedule, sched_ms = None, {}
     for ts in range(timesteps):
         sc = series_scene(ts, n, img)
         r = B.LightedVolumeRenderer()
         scenes.configure(r, sc)
         r.SetBricks(ranges, ghost=1, map_cells=cells)
+        if schedule is None:
+            # the schedule of the brick launches: the frame with every switch off and with every
+            # switch on, then the autotuner's pick (coordinate descent), used for every timestep
+            knobs = [k for k in B.LightedVolumeRenderer.SCHEDULE_KNOBS if k != "persistent_groups"]
+            trials = [("all switches off", {k: 0 for k in knobs}), ("all switches on", {k: 1 for k in knobs})]
+            trials += [(f"all on but {k}", {q: int(q != k) for q in knobs}) for k in knobs]
+            for label, sw in trials:
+                r.SetSchedule(persistent_groups=2048, **sw)
+                r.RenderImage(sc.camera, img, img, rgba=rgba, depth=depth)
+                synchronize()
+                t0 = time.perf_counter()
+                for _ in range(2):
+                    r.RenderImage(sc.camera, img, img, rgba=rgba, depth=depth)
+                synchronize()
+                sched_ms[label] = (time.perf_counter() - t0) / 2 * 1e3
+            _, sched_ms["autotuned"] = r.Autotune(sc.camera, img, img, frames=2)
+            schedule = r.GetSchedule()
+            print(f"[bench] series schedules: {sched_ms}, picked {schedule}", file=sys.stderr, flush=True)
+        r.SetSchedule(**schedule)
         # eager frames with per-phase events (the first one also warms up)
         r.RenderImage(sc.camera, img, img, rgba=rgba, depth=depth)
         r.ResetTimings()
@@ -356,7 +376,8 @@ def series_exact(steps, warmup, n, img, timesteps, keep_first=True):
     phase = {k: float(np.mean([p[k] for p in phase_rows])) for k in phase_rows[0]}
     out = dict(sec=sec, samples=samples, light_steps=light_steps, timed_samples=timed_samples,
                phase=phase, phase_per_timestep=phase_rows, ranges=ranges, cells=cells,
-               images=images, first_scene=first, renderer0=rs[0][0])
+               images=images, first_scene=first, renderer0=rs[0][0], schedule=schedule,
+               schedule_ms=sched_ms)
     for r, _ in rs[1:]:
         r.close()
     rgba.free()
@@ -809,6 +830,8 @@ def main():
                    "(counted on the GPU per timestep) / their time; phase_ms: HIP events around each "
                    "phase of 2 eager frames per timestep on the renderer's stream"),
         "phase_ms": ex["phase"],
+        "schedule": ex["schedule"],
+        "schedule_frame_ms": ex["schedule_ms"],
         "samples_per_frame": ex["samples"],
         "light_steps_per_frame": ex["light_steps"],
         "roofline": {

@@ -133,12 +133,29 @@ int beams_get_brick_map(beams_renderer* h, int32_t brick, int32_t which, float* 
 /* Workgroups of the persistent brick launches (a schedule knob; default 2048). */
 int beams_set_persistent_groups(beams_renderer* h, int32_t groups);
 
+/* Schedule of the brick launches: how work is dealt to the GPU, never what it computes (the
+ * maps and images are the same bit for bit under every schedule).  Defaults: 2048 persistent
+ * workgroups, every switch on.  Extension: the reference schedules through VTK-m's device
+ * adapter. */
+typedef struct beams_schedule {
+  int32_t persistent_groups; /* workgroups of the persistent map and exchange launches, >= 1    */
+  int32_t xcd_ranges;        /* 1: each XCD first drains its own contiguous range of map items */
+  int32_t light_rows;        /* 1: the map launches walk each brick's rows in light-sheet order */
+  int32_t march_xcd_tiles;   /* 1: the march deals screen tiles to the XCDs in contiguous blocks */
+  int32_t map_fast_fetch;    /* 1: the map launches take the branch-free corner fetch (24-bit
+                                offsets) when every brick's field and map allow it              */
+  int32_t march_fast_fetch;  /* 1: the same for the march launch                                */
+} beams_schedule;
+int beams_set_schedule(beams_renderer* h, const beams_schedule* s);
+int beams_get_schedule(const beams_renderer* h, beams_schedule* s);
+
 /* Frames of beams_render_image replay a hipGraph captured from their launches; the graph
  * is captured again whenever a kernel argument changes (camera, outputs, scene state). */
 int beams_set_graph(beams_renderer* h, int32_t enabled);
 
-/* Times `frames` frames of beams_render_image for each candidate schedule (the persistent
- * workgroup count of the brick launches), keeps the fastest and reports it. */
+/* Times `frames` frames of beams_render_image for candidate schedules of the brick launches
+ * (coordinate descent: the persistent workgroup count, then each switch of beams_schedule),
+ * keeps the fastest (beams_get_schedule reads it) and reports its workgroup count and time. */
 int beams_autotune(beams_renderer* h, const beams_camera* cam, int32_t width, int32_t height,
                    int32_t frames, int32_t* best_groups, float* best_ms);
 

@@ -115,6 +115,8 @@ SIGNATURES = {
     "beams_learned_samples": (C.c_int, [_P, _I32, _I64, C.c_uint64, _P, _P, _I32]),
     "beams_get_learned_stats": (C.c_int, [_P, _PF, C.POINTER(C.c_int32)]),
     "beams_render_rays_noalloc": (C.c_int, [_P, _I64, C.POINTER(_P), C.POINTER(_P), _P, _P, _P, _P]),
+    "beams_set_schedule": (C.c_int, [_P, C.c_void_p]),
+    "beams_get_schedule": (C.c_int, [_P, C.c_void_p]),
 }
 
 _LIB = None
@@ -337,6 +339,25 @@ class LightedVolumeRenderer:
 
     def SetPersistentGroups(self, groups):
         self._c(lib().beams_set_persistent_groups(self._h, int(groups)))
+
+    SCHEDULE_KNOBS = ("persistent_groups", "xcd_ranges", "light_rows", "march_xcd_tiles",
+                      "map_fast_fetch", "march_fast_fetch")
+
+    def SetSchedule(self, **knobs):
+        """Brick-launch schedule knobs (SCHEDULE_KNOBS; results are the same under every
+        schedule); a knob not given, or None, keeps its current value."""
+        unknown = set(knobs) - set(self.SCHEDULE_KNOBS)
+        if unknown:
+            raise ValueError(f"unknown schedule knobs {sorted(unknown)}")
+        cur = self.GetSchedule()
+        arr = (C.c_int32 * len(self.SCHEDULE_KNOBS))(
+            *[cur[k] if knobs.get(k) is None else int(knobs[k]) for k in self.SCHEDULE_KNOBS])
+        self._c(lib().beams_set_schedule(self._h, C.cast(arr, C.c_void_p)))
+
+    def GetSchedule(self):
+        arr = (C.c_int32 * len(self.SCHEDULE_KNOBS))()
+        self._c(lib().beams_get_schedule(self._h, C.cast(arr, C.c_void_p)))
+        return dict(zip(self.SCHEDULE_KNOBS, list(arr)))
 
     def SetGraph(self, enabled):
         self._c(lib().beams_set_graph(self._h, int(bool(enabled))))

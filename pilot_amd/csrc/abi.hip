@@ -140,6 +140,10 @@ struct beams_renderer {
   std::vector<unsigned char> btab_host;  // tables of the last frame (reused while unchanged)
   int32_t n_map_items = 0, n_px_items = 0, max_coords = 0;
   int32_t persistent_groups = 2048;
+  // schedule knobs of the brick launches (the same results whatever their values)
+  int32_t xcd_ranges = 1, light_rows = 1, march_xcd_tiles = 1, map_fast_fetch = 1, march_fast_fetch = 1;
+  DevBuf brows;                          // every brick's map rows in light-sheet order
+  std::vector<unsigned char> brows_key;  // the light and brick maps the rows were ordered for
   int64_t btab_version = 0;
   // hipGraph of the frame's launches, replayed while its key (every kernel argument) holds
   bool use_graph = false;
@@ -560,6 +564,55 @@ size_t append(std::vector<unsigned char>& buf, const T* data, size_t n)
   return off;
 }
 
+struct KeyBuilder {
+  std::vector<unsigned char> bytes;
+  template <typename T>
+  KeyBuilder& put(const T& v)
+  {
+    const unsigned char* p = (const unsigned char*)&v;
+    bytes.insert(bytes.end(), p, p + sizeof(T));
+    return *this;
+  }
+};
+
+// The rows (j, k) of a brick's map in light-sheet order, packed j | k << 16.  Seen along the
+// light, the rows (lines along x) that lie in one plane through the light's direction share
+// their light paths: with the depth axis a (y or z, whichever the light travels along more)
+// and the other axis e, row (j, k) at `depth` cells from the face the light enters through
+// meets that face at e-index ie - depth (msp_a / msp_e) (d_e / |d_a|).  Rows are grouped by
+// that index rounded (a sheet), sheets in order, each from the light's face inward.
+void light_sheet_rows(const MapParams& m, int32_t light_kind, const float* light, const double* bb,
+                      std::vector<uint32_t>& out)
+{
+  double d[3];
+  for (int a = 0; a < 3; ++a) {
+    const double c = 0.5 * (bb[2 * a] + bb[2 * a + 1]);
+    d[a] = light_kind == BEAMS_LIGHT_DIRECTIONAL ? (double)light[a] : c - (double)light[a];
+  }
+  const int a = std::fabs(d[2]) >= std::fabs(d[1]) ? 2 : 1, e = 3 - a;
+  const double da = std::fabs(d[a]) > 0.0 ? std::fabs(d[a]) : 1e-300;
+  const double shear = ((double)m.msp[a] / (double)m.msp[e]) * (d[e] / da);
+  const int64_t n1 = m.mp[1], n2 = m.mp[2], na = m.mp[a];
+  struct Row {
+    int64_t sheet, depth;
+    uint32_t jk;
+  };
+  std::vector<Row> rows((size_t)(n1 * n2));
+  size_t o = 0;
+  for (int64_t k = 0; k < n2; ++k)
+    for (int64_t j = 0; j < n1; ++j) {
+      const int64_t ia = a == 2 ? k : j, ie = a == 2 ? j : k;
+      const int64_t depth = d[a] >= 0.0 ? ia : na - 1 - ia;
+      const double sh = std::min(std::max((double)ie - (double)depth * shear, -1e15), 1e15);
+      rows[o++] = Row{ (int64_t)std::llround(sh), depth, (uint32_t)j | ((uint32_t)k << 16) };
+    }
+  std::stable_sort(rows.begin(), rows.end(), [](const Row& x, const Row& y) {
+    return x.sheet != y.sheet ? x.sheet < y.sheet : x.depth < y.depth;
+  });
+  out.resize(rows.size());
+  for (size_t i = 0; i < rows.size(); ++i) out[i] = rows[i].jk;
+}
+
 // Brick descriptors, work lists and depth order for camera c (host), uploaded when changed.
 int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int64_t& total_verts,
                    size_t& partial_pixels)
@@ -572,9 +625,9 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
   const size_t msz = dtype_size(h->map_dtype);
   const int nb = (int)h->bricks.size();
   std::vector<BrickDesc> desc((size_t)nb);
-  std::vector<int32_t> mi_brick, px_brick, px_tile, order((size_t)nb);
-  std::vector<int64_t> mi_first;
+  std::vector<int32_t> item_off((size_t)nb + 1, 0), px_brick, px_tile, order((size_t)nb);
   std::vector<float> depth((size_t)nb);
+  int64_t row_total = 0;
   total_verts = 0;
   partial_pixels = 0;
   int32_t max_coords = 0;
@@ -649,10 +702,9 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
     footprint(c, g.bounds, d.fp);
     d.img_off = (int64_t)partial_pixels;
     partial_pixels += (size_t)d.fp[2] * (size_t)d.fp[3];
-    for (int64_t v = 0; v < br.nverts; v += 256) {
-      mi_brick.push_back(b);
-      mi_first.push_back(v);
-    }
+    item_off[(size_t)b + 1] = item_off[(size_t)b] + (int32_t)((br.nverts + kMapItem - 1) / kMapItem);
+    d.row_off = row_total;
+    row_total += m.mp[1] * m.mp[2];
     const int tiles = ((d.fp[2] + 15) / 16) * ((d.fp[3] + 15) / 16);
     for (int t = 0; t < tiles; ++t) {
       px_brick.push_back(b);
@@ -670,10 +722,26 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
   std::stable_sort(order.begin(), order.end(),
                    [&](int32_t a, int32_t b) { return depth[(size_t)a] < depth[(size_t)b]; });
   HIP_TRY(h, h->bpartial.ensure(std::max<size_t>(partial_pixels, 1) * 20));
+  // the light-sheet row orders, recomputed when the light or a brick's map changes
+  if (h->light_rows) {
+    KeyBuilder key;
+    key.put(h->light_kind).put(h->light[0]).put(h->light[1]).put(h->light[2]);
+    for (const BrickDesc& d : desc) key.put(d.m.mp).put(d.m.msp).put(d.g.bounds);
+    if (key.bytes != h->brows_key) {
+      std::vector<uint32_t> all((size_t)row_total), one;
+      for (int b = 0; b < nb; ++b) {
+        light_sheet_rows(desc[(size_t)b].m, h->light_kind, h->light, desc[(size_t)b].g.bounds, one);
+        std::copy(one.begin(), one.end(), all.begin() + desc[(size_t)b].row_off);
+      }
+      HIP_TRY(h, hipStreamSynchronize(h->stream));
+      HIP_TRY(h, h->brows.ensure(sizeof(uint32_t) * all.size()));
+      HIP_TRY(h, hipMemcpy(h->brows.p, all.data(), sizeof(uint32_t) * all.size(), hipMemcpyHostToDevice));
+      h->brows_key.swap(key.bytes);
+    }
+  }
   std::vector<unsigned char> tab;
   const size_t o_desc = append(tab, desc.data(), desc.size());
-  const size_t o_mib = append(tab, mi_brick.data(), mi_brick.size());
-  const size_t o_mif = append(tab, mi_first.data(), mi_first.size());
+  const size_t o_mio = append(tab, item_off.data(), item_off.size());
   const size_t o_pxb = append(tab, px_brick.data(), px_brick.size());
   const size_t o_pxt = append(tab, px_tile.data(), px_tile.size());
   const size_t o_ord = append(tab, order.data(), order.size());
@@ -688,9 +756,9 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
   f = BrickFrame{};
   f.bricks = (const BrickDesc*)(base + o_desc);
   f.nbricks = nb;
-  f.map_item_brick = (const int32_t*)(base + o_mib);
-  f.map_item_first = (const int64_t*)(base + o_mif);
-  f.n_map_items = (int32_t)mi_brick.size();
+  f.map_item_off = (const int32_t*)(base + o_mio);
+  f.rows = h->light_rows ? (const uint32_t*)h->brows.p : nullptr;
+  f.n_map_items = item_off.back();
   f.px_item_brick = (const int32_t*)(base + o_pxb);
   f.px_item_tile = (const int32_t*)(base + o_pxt);
   f.n_px_items = (int32_t)px_brick.size();
@@ -701,22 +769,20 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
   f.partial_depth = (float*)((float4*)h->bpartial.p + partial_pixels);
   f.max_coords = max_coords;
   f.persistent_groups = h->persistent_groups;
+  f.xcd_ranges = h->xcd_ranges;
+  f.light_rows = h->light_rows;
+  f.march_xcd_tiles = h->march_xcd_tiles;
+  bool fast_ok = true;
+  for (const BrickDesc& d : desc)
+    fast_ok = fast_ok && d.g.bytes32 != 0 && d.m.bytes32 != 0 && d.g.field_stride[1] < (1 << 24) &&
+              d.g.field_stride[2] < (1 << 24) && d.m.ms[1] < (1 << 24) && d.m.ms[2] < (1 << 24);
+  f.map_fast = fast_ok && h->map_fast_fetch;
+  f.march_fast = fast_ok && h->march_fast_fetch;
   h->n_map_items = f.n_map_items;
   h->n_px_items = f.n_px_items;
   h->max_coords = max_coords;
   return BEAMS_OK;
 }
-
-struct KeyBuilder {
-  std::vector<unsigned char> bytes;
-  template <typename T>
-  KeyBuilder& put(const T& v)
-  {
-    const unsigned char* p = (const unsigned char*)&v;
-    bytes.insert(bytes.end(), p, p + sizeof(T));
-    return *this;
-  }
-};
 
 // Runs the frame's launches `issue(timed)` eagerly, or as a replay of a hipGraph captured
 // from them; the graph is re-captured whenever the key (all kernel arguments) changes.
@@ -935,6 +1001,7 @@ int beams_destroy(beams_renderer* h)
   h->btables.release();
   h->bpartial.release();
   h->bqueue.release();
+  h->brows.release();
   (void)hipStreamDestroy(h->stream);
   delete h;
   return BEAMS_OK;
@@ -1163,6 +1230,33 @@ int beams_set_persistent_groups(beams_renderer* h, int32_t groups)
   return BEAMS_OK;
 }
 
+int beams_set_schedule(beams_renderer* h, const beams_schedule* s)
+{
+  CHECK_HANDLE(h);
+  if (!s) return h->fail(BEAMS_ERR_INVALID, "null schedule");
+  if (s->persistent_groups < 1 || s->persistent_groups > 65536)
+    return h->fail(BEAMS_ERR_INVALID, "persistent groups must be in [1, 65536]");
+  h->persistent_groups = s->persistent_groups;
+  h->xcd_ranges = s->xcd_ranges != 0;
+  h->light_rows = s->light_rows != 0;
+  h->march_xcd_tiles = s->march_xcd_tiles != 0;
+  h->map_fast_fetch = s->map_fast_fetch != 0;
+  h->march_fast_fetch = s->march_fast_fetch != 0;
+  return BEAMS_OK;
+}
+
+int beams_get_schedule(const beams_renderer* h, beams_schedule* s)
+{
+  if (!h || !s) return BEAMS_ERR_INVALID;
+  s->persistent_groups = h->persistent_groups;
+  s->xcd_ranges = h->xcd_ranges;
+  s->light_rows = h->light_rows;
+  s->march_xcd_tiles = h->march_xcd_tiles;
+  s->map_fast_fetch = h->map_fast_fetch;
+  s->march_fast_fetch = h->march_fast_fetch;
+  return BEAMS_OK;
+}
+
 int beams_set_graph(beams_renderer* h, int32_t enabled)
 {
   CHECK_HANDLE(h);
@@ -1185,43 +1279,62 @@ int beams_autotune(beams_renderer* h, const beams_camera* cam, int32_t width, in
   const size_t npix = (size_t)width * (size_t)height;
   HIP_TRY(h, h->image.ensure(sizeof(float) * npix * 5));
   float* img = (float*)h->image.p;
-  std::vector<int32_t> cand;
-  if (h->bricks.empty())
-    cand.push_back(h->persistent_groups);
-  else
-    cand = { 256, 512, 1024, 2048, 4096, 8192 };
   hipEvent_t e0, e1;
   HIP_TRY(h, hipEventCreate(&e0));
   HIP_TRY(h, hipEventCreate(&e1));
-  float best = INFINITY;
-  int32_t bg = h->persistent_groups;
   int rc = BEAMS_OK;
-  for (int32_t g : cand) {
-    h->persistent_groups = g;
+  // mean frame time of schedule s (one untimed frame first), +inf on failure
+  auto time_schedule = [&](const beams_schedule& s) -> float {
+    if (rc) return INFINITY;
+    beams_set_schedule(h, &s);
     if ((rc = beams_render_image(h, cam, width, height, nullptr, img, img + 4 * npix, BEAMS_MEM_DEVICE)))
-      break;
+      return INFINITY;
     (void)hipEventRecord(e0, h->stream);
     for (int32_t k = 0; k < frames && !rc; ++k)
       rc = beams_render_image(h, cam, width, height, nullptr, img, img + 4 * npix, BEAMS_MEM_DEVICE);
-    if (rc) break;
+    if (rc) return INFINITY;
     (void)hipEventRecord(e1, h->stream);
     float ms = 0.0f;
     if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
       rc = h->fail(BEAMS_ERR_HIP, "autotune: event timing failed");
-      break;
+      return INFINITY;
     }
-    ms /= (float)frames;
-    if (ms < best) {
-      best = ms;
-      bg = g;
+    return ms / (float)frames;
+  };
+  beams_schedule best{};
+  beams_get_schedule(h, &best);
+  float best_t = INFINITY;
+  if (h->bricks.empty()) {
+    best_t = time_schedule(best);  // one block: no schedule knob applies
+  } else {
+    // coordinate descent: the workgroup count, then each switch flipped in turn
+    for (int32_t g : { 512, 1024, 2048, 4096, 8192 }) {
+      beams_schedule s = best;
+      s.persistent_groups = g;
+      const float t = time_schedule(s);
+      if (t < best_t) {
+        best_t = t;
+        best = s;
+      }
+    }
+    for (int knob = 0; knob < 5 && !rc; ++knob) {
+      beams_schedule s = best;
+      int32_t* const v[5] = { &s.xcd_ranges, &s.light_rows, &s.march_xcd_tiles, &s.map_fast_fetch,
+                              &s.march_fast_fetch };
+      *v[knob] = !*v[knob];
+      const float t = time_schedule(s);
+      if (t < best_t) {
+        best_t = t;
+        best = s;
+      }
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  h->persistent_groups = bg;
+  beams_set_schedule(h, &best);
   if (rc) return rc;
-  if (best_groups) *best_groups = bg;
-  if (best_ms) *best_ms = best;
+  if (best_groups) *best_groups = best.persistent_groups;
+  if (best_ms) *best_ms = best_t;
   return beams_reset_timings(h);
 }
 

@@ -20,7 +20,7 @@ namespace beams {
 namespace {
 
 // ---- phases 1 and 3: per-brick TransmittanceMapGenerator -------------------------------------
-template <int FDT, int MDT>
+template <int FDT, int MDT, bool FAST>
 __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
 {
   extern __shared__ float lds[];
@@ -34,10 +34,10 @@ __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
   queue_init(&queue);
   int staged = -1;
   for (;;) {
-    const int item = next_item(counters, &queue, f.n_map_items, chunk);
+    const int item = next_item(counters, &queue, f.n_map_items, chunk, f.xcd_ranges);
     if (item >= f.n_map_items) break;
     // the brick's parameters in scalar registers (the same brick for the whole workgroup)
-    const int b = __builtin_amdgcn_readfirstlane(f.map_item_brick[item]);
+    const int b = item_brick(f, item);
     const BrickDesc* bk = f.bricks + b;
     const GridParams g = load_uniform(&bk->g);
     const MapParams m = load_uniform(&bk->m);
@@ -48,15 +48,40 @@ __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
     staged = b;
     __syncthreads();
     const float* x[3] = { xs, xs + g.n[0], xs + g.n[0] + g.n[1] };
-    const int64_t v = f.map_item_first[item] + threadIdx.x;
-    if (v < bk->nverts) {
+    // the item's ordinal of this lane's vertex: x fastest, or along the brick's rows in
+    // light-sheet order (rows whose light paths share a sheet follow one another, so the
+    // workgroups in flight read a few sheets of the field instead of a slab of the brick)
+    // (a brick's map holds fewer than 2^31 vertices: 32-bit divisions)
+    const int64_t q = (int64_t)(item - f.map_item_off[b]) * kMapItem + threadIdx.x;
+    if (q < bk->nverts) {
+      const uint32_t mp0 = (uint32_t)m.mp[0], mp1 = (uint32_t)m.mp[1];
+      const uint32_t r = (uint32_t)q / mp0, i = (uint32_t)q - r * mp0;
+      uint32_t j, k;
+      if (f.light_rows) {
+        const uint32_t jk = f.rows[bk->row_off + r];
+        j = jk & 0xffffu;
+        k = jk >> 16;
+      } else {
+        k = r / mp1;
+        j = r - k * mp1;
+      }
+      const int64_t v = (int64_t)i + (int64_t)mp0 * ((int64_t)j + (int64_t)mp1 * (int64_t)k);
+      // ArrayPortalUniformPointCoordinates::Get, as map_vertex computes it
+      const float pos[3] = { m.morg[0] + m.msp[0] * (float)i, m.morg[1] + m.msp[1] * (float)j,
+                             m.morg[2] + m.msp[2] * (float)k };
       int64_t taken = 0;
       const float seed = seeded ? f.seeds[bk->seed_off + v] : 0.0f;
-      // a march seeded at opacity 1 returns 1 (each step adds (1 - 1) a = 0 for finite a);
-      // skipped unless steps are being counted
+      // A march seeded at opacity 1 returns 1 (each step adds (1 - 1) a = 0 for finite a), and
+      // one seeded at 0 repeats the phase-1 march, whose stored value is copied; both skipped
+      // unless steps are being counted.
       const bool opaque = seed == 1.0f && sp.alpha_finite && !sp.counters;
-      const float op = opaque ? 1.0f : march_vertex<FDT>(g, m, sp, a_tab, x, v, seed, &taken);
-      store_t<MDT>(seeded ? bk->map_final : m.map, v, op);
+      const bool unseeded = seeded && seed == 0.0f && !sp.counters;
+      if (unseeded) {
+        copy_t<MDT>(bk->map_final, m.map, v);
+      } else {
+        const float op = opaque ? 1.0f : march_vertex_at<FDT, false, FAST>(g, m, sp, a_tab, x, pos, seed, &taken);
+        store_t<MDT>(seeded ? bk->map_final : m.map, v, op);
+      }
       if (sp.counters) count_add(&sp.counters[1], (uint32_t)taken);
     }
   }
@@ -129,6 +154,85 @@ __device__ __forceinline__ bool may_cross(const MapParams& m, const float* c, fl
   return phi <= half * 1.001f + 2e-3f;
 }
 
+// Does triangle (A, B, C) meet the box [bx0, bx1] x [bx2, bx3] x [bx4, bx5]?  The separating
+// axis test of Akenine-Moeller: the box's face normals, the triangle's normal and the nine
+// cross products of box axes and triangle edges.  In FP64 on FP32 inputs, so that a test
+// against the inflated FP32 boxes (box32) keeps every box the exact segment test can hit.
+__device__ __forceinline__ bool tri_box(const float* bx, const float* A, const float* B, const float* C)
+{
+  double h[3], v0[3], v1[3], v2[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double lo = bx[2 * a], hi = bx[2 * a + 1], c = 0.5 * (lo + hi);
+    h[a] = 0.5 * (hi - lo);
+    v0[a] = (double)A[a] - c;
+    v1[a] = (double)B[a] - c;
+    v2[a] = (double)C[a] - c;
+    if (fmin(v0[a], fmin(v1[a], v2[a])) > h[a] || fmax(v0[a], fmax(v1[a], v2[a])) < -h[a]) return false;
+  }
+  const double* vs[3] = { v0, v1, v2 };
+  double e[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    e[0][a] = v1[a] - v0[a];
+    e[1][a] = v2[a] - v1[a];
+    e[2][a] = v0[a] - v2[a];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      // axis = unit(i) x e_k: components on the two other axes u = i+1, w = i+2
+      const int u = (i + 1) % 3, w = (i + 2) % 3;
+      const double au = -e[k][w], aw = e[k][u];
+      double pmin = 1e300, pmax = -1e300;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double pq = au * vs[q][u] + aw * vs[q][w];
+        pmin = fmin(pmin, pq);
+        pmax = fmax(pmax, pq);
+      }
+      const double r = h[u] * fabs(au) + h[w] * fabs(aw);
+      if (pmin > r || pmax < -r) return false;
+    }
+  }
+  const double n[3] = { e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2],
+                        e[0][0] * e[1][1] - e[0][1] * e[1][0] };
+  const double d = n[0] * v0[0] + n[1] * v0[1] + n[2] * v0[2];
+  return fabs(d) <= h[0] * fabs(n[0]) + h[1] * fabs(n[1]) + h[2] * fabs(n[2]);
+}
+
+// Row r of brick bd's map walk: the r-th row in light-sheet order, or x-fastest row r.
+__device__ __forceinline__ void row_of(const BrickFrame& f, const BrickDesc& bd, int64_t r, int64_t& j,
+                                       int64_t& k)
+{
+  if (f.light_rows) {
+    const uint32_t jk = f.rows[bd.row_off + r];
+    j = jk & 0xffffu;
+    k = jk >> 16;
+  } else {
+    j = r % bd.m.mp[1];
+    k = r / bd.m.mp[1];
+  }
+}
+
+// Can a light segment of a map vertex in x-run [xa, xb] of row (j, k) of brick map m meet box bx?
+// The vertices lie on the segment P0 P1 of the row, so their light segments lie in the
+// triangle (light, P0, P1) for a point light, or in the parallelogram (P0, P1, O1, O0) of their
+// up-light origins O = P - dir_far dir for a directional light.
+__device__ __forceinline__ bool run_may_cross(const MapParams& m, int64_t j, int64_t k, int64_t xa,
+                                              int64_t xb, const float* bx)
+{
+  const float y = m.morg[1] + m.msp[1] * (float)j, z = m.morg[2] + m.msp[2] * (float)k;
+  const float P0[3] = { m.morg[0] + m.msp[0] * (float)xa, y, z };
+  const float P1[3] = { m.morg[0] + m.msp[0] * (float)xb, y, z };
+  if (m.light_kind != 1) return tri_box(bx, m.light, P0, P1);
+  float O0[3], O1[3];
+  light_origin(m, P0, O0);
+  light_origin(m, P1, O1);
+  return tri_box(bx, P0, P1, O1) || tri_box(bx, P0, O1, O0);
+}
+
 // ---- phase 2: non-local hits (TransmittanceMap.h:342-453) + fetch (:505-508) + composite
 // (LightedVolumeRenderer.cxx:1214-1235), one thread per map vertex ------------------------------
 // Per work item the wave first keeps the bricks whose bounding sphere may meet a light segment
@@ -198,26 +302,40 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
   const int chunk = 1;
   queue_init(&queue);
   for (;;) {
-    const int item = next_item(f.queue + 1 * kQueueSlots, &queue, f.n_map_items, chunk);
+    const int item = next_item(f.queue + 1 * kQueueSlots, &queue, f.n_map_items, chunk, f.xcd_ranges);
     if (item >= f.n_map_items) break;
-    const int b = f.map_item_brick[item];
+    const int b = item_brick(f, item);
+    const int64_t item_first = (int64_t)(item - f.map_item_off[b]) * kMapItem;
     const BrickDesc& bd = f.bricks[b];
-    // candidate bricks of this item
+    // candidate bricks of this item: those the light segments of its vertices may cross.  An
+    // item of a few map rows is tested run by run against the triangles its light segments lie
+    // in; one of many short rows against the cone (or capsule) through its bounding sphere.
     int ncand = 0;
     {
-      const int64_t first = f.map_item_first[item];
-      const int64_t last = first + 255 < bd.nverts - 1 ? first + 255 : bd.nverts - 1;
+      const int64_t first = item_first;
+      const int64_t last = first + kMapItem - 1 < bd.nverts - 1 ? first + kMapItem - 1 : bd.nverts - 1;
+      const int64_t mp0 = bd.m.mp[0], r0 = first / mp0, r1 = last / mp0;
+      const bool runs = r1 - r0 < 4;
       float c[3], R;
-      item_sphere(bd.m, first, last, c, R);
+      // rows in light-sheet order are no longer neighbours: the whole brick bounds the item
+      item_sphere(bd.m, f.light_rows ? 0 : first, f.light_rows ? bd.nverts - 1 : last, c, R);
       for (int j0 = 0; j0 < f.nbricks; j0 += 64) {
         const int j = j0 + lane;
         bool keep = false;
         if (j < f.nbricks && j != b) {
           const float* bx = f.bricks[j].box32;
-          const float s[3] = { 0.5f * (bx[0] + bx[1]), 0.5f * (bx[2] + bx[3]), 0.5f * (bx[4] + bx[5]) };
-          const float e[3] = { bx[1] - bx[0], bx[3] - bx[2], bx[5] - bx[4] };
-          const float r = 0.5f * sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) * 1.0001f + 1e-6f;
-          keep = may_cross(bd.m, c, R, s, r);
+          if (runs) {
+            for (int64_t r = r0; r <= r1 && !keep; ++r) {
+              int64_t rj, rk;
+              row_of(f, bd, r, rj, rk);
+              keep = run_may_cross(bd.m, rj, rk, r == r0 ? first % mp0 : 0, r == r1 ? last % mp0 : mp0 - 1, bx);
+            }
+          } else {
+            const float s[3] = { 0.5f * (bx[0] + bx[1]), 0.5f * (bx[2] + bx[3]), 0.5f * (bx[4] + bx[5]) };
+            const float e[3] = { bx[1] - bx[0], bx[3] - bx[2], bx[5] - bx[4] };
+            const float rs = 0.5f * sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) * 1.0001f + 1e-6f;
+            keep = may_cross(bd.m, c, R, s, rs);
+          }
         }
         const unsigned long long mask = __ballot(keep);
         const int pos = ncand + __popcll(mask & ((1ull << lane) - 1ull));
@@ -242,10 +360,18 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
     const bool all = ncand > kMaxCandidates;  // too many to list: test every brick
     const int nloop = all ? f.nbricks : ncand;
     for (int sub = 0; sub < 4; ++sub) {
-      const int64_t v = f.map_item_first[item] + sub * 64 + lane;
-      if (v >= bd.nverts) continue;
+      const int64_t qv = item_first + sub * 64 + lane;
+      if (qv >= bd.nverts) continue;
+      const int64_t mp0 = bd.m.mp[0];
+      const int64_t r = qv / mp0, i = qv - r * mp0;
+      int64_t rj, rk;
+      row_of(f, bd, r, rj, rk);
+      const int64_t v = i + mp0 * (rj + bd.m.mp[1] * rk);
+      // map_vertex's position of vertex v
       float p[3], o[3];
-      map_vertex(bd.m, v, p);
+      p[0] = bd.m.morg[0] + bd.m.msp[0] * (float)i;
+      p[1] = bd.m.morg[1] + bd.m.msp[1] * (float)rj;
+      p[2] = bd.m.morg[2] + bd.m.msp[2] * (float)rk;
       light_origin(bd.m, p, o);
       const float dd[3] = { p[0] - o[0], p[1] - o[1], p[2] - o[2] };
       const float rs = 1.0f / __builtin_sqrtf(dot3(dd, dd));  // vtkm::Normalize
@@ -262,7 +388,14 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
         const int j = (int)(key & 0xffffffffull);
         const float t = bits_f32((uint32_t)(key >> 32));
         const float q[3] = { o[0] + dir[0] * t, o[1] + dir[1] * t, o[2] + dir[2] * t };
-        const float local = map_opacity<MDT>(f.bricks[j].m, q);
+        // the lanes usually fetch from the same brick: its map parameters then come in scalar
+        // registers (one load each for the wave, not one per lane)
+        const int j0 = __builtin_amdgcn_readfirstlane(j);
+        float local;
+        if (__ballot(j != j0) == 0)
+          local = map_opacity_uniform<MDT>(load_uniform(&f.bricks[j0].m), q);
+        else
+          local = map_opacity<MDT>(f.bricks[j].m, q);
         return op + (1.0f - op) * local;
       };
       unsigned long long top = 0;  // the largest key so far (hits[nh-1][lane])
@@ -318,13 +451,20 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
 }
 
 // ---- phase 4: CalcRayStart + Sampler per brick over its screen footprint ---------------------
-template <int FDT, int MDT, bool USE_MAP>
+template <int FDT, int MDT, bool USE_MAP, bool FAST>
 __global__ void __launch_bounds__(256) k_bricks_march(BrickFrame f, CameraParams cam)
 {
   // One workgroup per (brick, 16x16 screen tile) item.  Not persistent: a loop over items
   // keeps the camera and brick parameters live across the march and doubles the registers.
   extern __shared__ float4 lds4[];
-  const int item = blockIdx.x;
+  // march_xcd_tiles: workgroup w runs on XCD w mod 8 (workgroups are dealt to the XCDs round
+  // robin); it takes the (w / 8)-th item of that XCD's contiguous block of items, so each XCD's
+  // L2 serves one screen region of a brick instead of every eighth tile of all of them
+  int item = blockIdx.x;
+  if (f.march_xcd_tiles) {
+    const int n = (int)gridDim.x, xcd = (int)(blockIdx.x & 7u), k = (int)(blockIdx.x >> 3);
+    item = xcd * (n >> 3) + min(xcd, n & 7) + k;
+  }
   float4* tf = lds4;
   float* xs = (float*)(lds4 + f.bricks[0].s.ntf);
   // the item's brick is the same for the whole workgroup: its parameters are read with
@@ -352,7 +492,7 @@ __global__ void __launch_bounds__(256) k_bricks_march(BrickFrame f, CameraParams
   float minD = cam.near_d, maxD = cam.far_d, dist = 0.0f;
   calc_ray_start(g, org, dir, minD, dist, maxD);
   float color[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
-  shade<FDT, MDT, USE_MAP>(g, m, sp, tf, x, org, dir, minD, maxD, color);
+  shade<FDT, MDT, USE_MAP, false, FAST>(g, m, sp, tf, x, org, dir, minD, maxD, color);
   const int64_t pix = img_off + (int64_t)ly * fp2 + lx;
   f.partial[pix] = make_float4(color[0], color[1], color[2], color[3]);
   f.partial_depth[pix] = (minD == -1.0f) ? __builtin_inff() : minD;
@@ -436,7 +576,10 @@ hipError_t launch_bricks_map(const BrickFrame& f, int32_t field_dtype, int32_t m
   const size_t lds = sizeof(float) * (size_t)(ntf + 2 * f.max_coords);
   const dim3 grid((unsigned)f.persistent_groups);
   BRICK_DISPATCH_FIELD(field_dtype, BRICK_DISPATCH_MAP(map_dtype, {
-    hipLaunchKernelGGL((k_bricks_map<FDT, MDT>), grid, dim3(256), lds, st, f, seeded ? 1 : 0);
+    if (f.map_fast)
+      hipLaunchKernelGGL((k_bricks_map<FDT, MDT, true>), grid, dim3(256), lds, st, f, seeded ? 1 : 0);
+    else
+      hipLaunchKernelGGL((k_bricks_map<FDT, MDT, false>), grid, dim3(256), lds, st, f, seeded ? 1 : 0);
   }));
   return hipGetLastError();
 }
@@ -460,10 +603,14 @@ hipError_t launch_bricks_march(const BrickFrame& f, int32_t field_dtype, int32_t
   const size_t lds = sizeof(float4) * (size_t)ntf + sizeof(float) * 2 * (size_t)f.max_coords;
   const dim3 grid((unsigned)f.n_px_items);
   BRICK_DISPATCH_FIELD(field_dtype, BRICK_DISPATCH_MAP(map_dtype, {
-    if (use_map)
-      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, true>), grid, dim3(256), lds, st, f, c);
+    if (use_map && f.march_fast)
+      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, true, true>), grid, dim3(256), lds, st, f, c);
+    else if (use_map)
+      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, true, false>), grid, dim3(256), lds, st, f, c);
+    else if (f.march_fast)
+      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, false, true>), grid, dim3(256), lds, st, f, c);
     else
-      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, false>), grid, dim3(256), lds, st, f, c);
+      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, false, false>), grid, dim3(256), lds, st, f, c);
   }));
   return hipGetLastError();
 }

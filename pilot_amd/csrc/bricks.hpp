@@ -13,6 +13,7 @@ struct BrickDesc {
   int64_t nverts;      // map vertices
   int64_t seed_off;    // first vertex of this brick in the seed buffer
   int64_t img_off;     // first pixel of this brick's partial image
+  int64_t row_off;     // first entry of this brick's map rows in light-sheet order
   int32_t fp[4];       // screen footprint: x0, y0, width, height (pixels)
   float box32[6];      // bounds inflated for the conservative FP32 cull
 };
@@ -20,11 +21,16 @@ struct BrickDesc {
 // Work-queue counters per persistent phase (one per XCD).
 constexpr int kQueueSlots = 8;
 
+// Map work items: 256 vertices of one brick.  Brick b holds items map_item_off[b] ..
+// map_item_off[b+1] - 1; its item i covers its vertices 256 i .. 256 i + 255, x fastest or (the
+// map kernel with light_rows) in the light-sheet row order of `rows`.
+constexpr int kMapItem = 256;
+
 struct BrickFrame {
   const BrickDesc* bricks;
   int32_t nbricks;
-  const int32_t* map_item_brick;  // per map work item: brick
-  const int64_t* map_item_first;  // per map work item: first vertex
+  const int32_t* map_item_off;    // nbricks + 1 prefix sums of the bricks' map items
+  const uint32_t* rows;           // per brick: its map rows (j | k << 16) in light-sheet order
   int32_t n_map_items;
   const int32_t* px_item_brick;   // per march work item: brick
   const int32_t* px_item_tile;    // per march work item: 16x16 tile index in the footprint
@@ -36,6 +42,12 @@ struct BrickFrame {
   float* partial_depth;           // per-brick footprint entry distance
   int32_t max_coords;             // the largest brick's n0 + n1 + n2 (LDS: coordinates + reciprocal widths)
   int32_t persistent_groups;      // workgroups of the persistent launches
+  // schedule knobs (the same results whatever their values)
+  int32_t xcd_ranges;             // persistent queues: each XCD drains its own contiguous range first
+  int32_t light_rows;             // map kernel: rows in light-sheet order (rows != null)
+  int32_t march_xcd_tiles;        // march: tiles dealt to the XCDs in contiguous blocks
+  int32_t map_fast, march_fast;   // branch-free corner fetch in the map / march launches (every
+                                  // brick's field and map view below 2^32 bytes, strides below 2^24)
 };
 
 hipError_t launch_bricks_map(const BrickFrame& f, int32_t field_dtype, int32_t map_dtype,

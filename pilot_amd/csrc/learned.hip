@@ -415,9 +415,10 @@ __global__ void __launch_bounds__(kBlock) k_bricks_learned_map(LearnedParams p, 
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int item = blockIdx.x; item < f.n_map_items; item += gridDim.x) {
-    const BrickDesc& d = f.bricks[f.map_item_brick[item]];
+    const int b = item_brick(f, item);
+    const BrickDesc& d = f.bricks[b];
     const int64_t nv = d.nverts;
-    const int64_t id = f.map_item_first[item] + wave * 64 + lane;
+    const int64_t id = (int64_t)(item - f.map_item_off[b]) * kMapItem + wave * 64 + lane;
     const bool valid = id < nv;
     float v[3], feat[kFeat];
     map_vertex(d.m, valid ? id : nv - 1, v);
@@ -493,12 +494,12 @@ __global__ void __launch_bounds__(kBlock) k_bricks_exact_map(LearnedParams p, Br
   __shared__ ItemQueue queue;
   queue_init(&queue);
   for (;;) {
-    const int item = next_item(f.queue + 4 * kQueueSlots, &queue, f.n_map_items, 1);
+    const int item = next_item(f.queue + 4 * kQueueSlots, &queue, f.n_map_items, 1, f.xcd_ranges);
     if (item >= f.n_map_items) break;
-    const int b = __builtin_amdgcn_readfirstlane(f.map_item_brick[item]);
+    const int b = item_brick(f, item);
     if (!redo[b]) continue;
     const BrickDesc* bk = f.bricks + b;
-    const int64_t id = f.map_item_first[item] + threadIdx.x;
+    const int64_t id = (int64_t)(item - f.map_item_off[b]) * kMapItem + threadIdx.x;
     if (id >= bk->nverts) continue;
     const MapParams m = whole_volume_march(load_uniform(&bk->m), p.m);
     int64_t taken = 0;

@@ -42,6 +42,15 @@ template <> __device__ __forceinline__ void store_t<kF16>(void* p, int64_t i, fl
   ((uint16_t*)p)[i] = f32_to_f16(v);
 }
 
+// Element i of one stored map copied to another of the same storage type, bit for bit.
+template <int DT> __device__ __forceinline__ void copy_t(void* dst, const void* src, int64_t i)
+{
+  if (DT == kF32)
+    ((uint32_t*)dst)[i] = ((const uint32_t*)src)[i];
+  else
+    ((uint16_t*)dst)[i] = ((const uint16_t*)src)[i];
+}
+
 // The 8 corner scalars of a field cell (hex order 0:(i,j,k) 1:(i+1,j,k) 2:(i+1,j+1,k)
 // 3:(i,j+1,k), 4-7 the same at k+1).  A field view of fewer than 2^32 bytes is read through a
 // raw buffer resource: the cell's byte offset is the only per-lane address and the corners'
@@ -49,7 +58,10 @@ template <> __device__ __forceinline__ void store_t<kF16>(void* p, int64_t i, fl
 // lane of a wave reads the same view (the kernels march one grid, or one brick per
 // workgroup), which the constructor states to the compiler so that the resource stays in
 // scalar registers.
-template <int FDT>
+// FAST: the caller guarantees a view below 2^32 bytes and strides below 2^24, so the offset
+// takes full-rate 24-bit multiplies and the load has no branch (the conversions of FP16
+// corners can then fuse into the interpolation's first multiply-adds).
+template <int FDT, bool FAST = false>
 struct CornerFetch {
   __amdgpu_buffer_rsrc_t rsrc;
   int so[8];  // corner byte offsets
@@ -67,7 +79,7 @@ struct CornerFetch {
     sx = __builtin_amdgcn_readfirstlane(sx_);
     sy = __builtin_amdgcn_readfirstlane(sy_);
     sz = __builtin_amdgcn_readfirstlane(sz_);
-    w32 = bytes32 != 0;
+    w32 = FAST || bytes32 != 0;
     esz = FDT == kF32 ? 4 : 2;
     rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(field), (short)0, (int)bytes32, 0x00020000);
     // element offsets below 2^31, byte offsets below 2^32 when w32 (unsigned products)
@@ -85,11 +97,25 @@ struct CornerFetch {
   {
     if (FDT == kF32) return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, soff, 0));
     const uint16_t h = __builtin_amdgcn_raw_buffer_load_b16(rsrc, voff, soff, 0);
+    if (FAST && FDT == kF16) {
+      // a plain conversion: left to itself the compiler folds it into the interpolation's
+      // first multiply-adds as v_fma_mix_f32, which measured slower than convert + v_fma_f32
+      float r;
+      asm("v_cvt_f32_f16 %0, %1" : "=v"(r) : "v"((uint32_t)h));
+      return r;
+    }
     return FDT == kF16 ? f16_to_f32(h) : bf16_to_f32(h);
   }
   __device__ __forceinline__ void load(int ci, int cj, int ck, float* v) const
   {
-    if (w32) {
+    if (FAST) {
+      // cell indices below 2^20 (point dims at most 2^20), strides below 2^24
+      const uint32_t e = __umul24((uint32_t)ci, (uint32_t)sx) + __umul24((uint32_t)cj, (uint32_t)sy) +
+                         __umul24((uint32_t)ck, (uint32_t)sz);
+      const int off = (int)(e * (uint32_t)(FDT == kF32 ? 4 : 2));
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = buf(off, so[q]);
+    } else if (w32) {
       const int off = (int)((uint32_t)(ci * sx + cj * sy + ck * sz) * (uint32_t)esz);
 #pragma unroll
       for (int q = 0; q < 8; ++q) v[q] = buf(off, so[q]);
@@ -336,8 +362,8 @@ struct MapCellCache {
 // is not the cached one, issues the corner loads (the camera march calls it before the field's
 // own loads, so both sets are in flight together); map_cell_opacity interpolates in the cell.
 // Same result as map_opacity: the same corner values and parametric coordinates.
-template <int MDT>
-__device__ __forceinline__ void map_cell_update(const MapParams& m, const CornerFetch<MDT>& mf,
+template <int MDT, bool FAST = false>
+__device__ __forceinline__ void map_cell_update(const MapParams& m, const CornerFetch<MDT, FAST>& mf,
                                                 const float* p, MapCellCache& mc)
 {
   int cell[3];
@@ -387,7 +413,9 @@ __device__ __forceinline__ int walk_cell32(const float* x, int ncell, float v, f
 // first gives the same index for every finite product and keeps it 32-bit.
 __device__ __forceinline__ int tf_clamped_index(float t, float tfmax)
 {
-  return (int)fminf(fmaxf(t, 0.0f), tfmax);
+  // med3 clamps a finite t to [0, tfmax] in one instruction; a NaN t gives 0, as the
+  // fmax/fmin form does
+  return (int)__builtin_amdgcn_fmed3f(t, 0.0f, tfmax);
 }
 
 // TransmittanceMap.h:93-112, 166-229: trilinear map opacity at p.
@@ -420,6 +448,32 @@ __device__ __forceinline__ float map_opacity(const MapParams& m, const float* p)
   v[5] = load_t<MDT>(m.map, i0 + sxy + s0);
   v[6] = load_t<MDT>(m.map, i0 + sxy + s0 + sx);
   v[7] = load_t<MDT>(m.map, i0 + sxy + sx);
+  return interp_hex(v, pc);
+}
+
+// map_opacity for a map every lane of the wave reads (m in scalar registers): the same cell,
+// parametric coordinates and corner values, the corners read through a buffer resource with
+// 32-bit offsets when the map is below 2^32 bytes.
+template <int MDT>
+__device__ __forceinline__ float map_opacity_uniform(const MapParams& m, const float* p)
+{
+  if (m.bytes32 == 0) return map_opacity<MDT>(m, p);
+  float pc[3];
+  int cell[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int np = (int)m.mp[d];
+    float t = (p[d] - m.morg[d]) * m.minv[d];
+    if (t < 0.0f) t = 0.0f;
+    if (t >= (float)(np - 1)) t = (float)(np - 2);
+    cell[d] = (int)t;
+    float lo = m.morg[d] + m.msp[d] * (float)cell[d];
+    float hi = m.morg[d] + m.msp[d] * (float)(cell[d] + 1);
+    pc[d] = (p[d] - lo) / (hi - lo);
+  }
+  const CornerFetch<MDT> mf(m.map, m.bytes32, (int)m.ms[0], (int)m.ms[1], (int)m.ms[2]);
+  float v[8];
+  mf.load(cell[0], cell[1], cell[2], v);
   return interp_hex(v, pc);
 }
 

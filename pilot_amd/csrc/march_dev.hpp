@@ -59,14 +59,15 @@ __device__ __forceinline__ void count_add(unsigned long long* ctr, uint32_t v)
 // ---- TransmittanceMap.h:279-331: one map vertex's light march ----------------------------
 // a_tab: alpha / max alpha per TF entry; x: the block's coordinates (LDS).  Returns the
 // opacity; *taken receives the number of steps inside the volume.
-template <int FDT, bool BINS = false>
-__device__ __forceinline__ float march_vertex(const GridParams& g, const MapParams& m,
-                                              const ShadeParams& s, const float* a_tab,
-                                              const float* const* x, int64_t id, float seed_value,
-                                              int64_t* taken_out)
+// march_vertex_at marches from the map vertex at v (map_vertex's position of the vertex).
+// FAST: the field view and strides suit CornerFetch<FDT, true>.
+template <int FDT, bool BINS = false, bool FAST = false>
+__device__ __forceinline__ float march_vertex_at(const GridParams& g, const MapParams& m,
+                                                 const ShadeParams& s, const float* a_tab,
+                                                 const float* const* x, const float* v,
+                                                 float seed_value, int64_t* taken_out)
 {
-  float v[3], o[3], dir[3];
-  map_vertex(m, id, v);
+  float o[3], dir[3];
   light_origin(m, v, o);
   float dd[3] = { v[0] - o[0], v[1] - o[1], v[2] - o[2] };
   const float rs = 1.0f / __builtin_sqrtf(dot3(dd, dd));
@@ -87,7 +88,7 @@ __device__ __forceinline__ float march_vertex(const GridParams& g, const MapPara
   const int c0 = (int)(g.n[0] - 1), c1 = (int)(g.n[1] - 1), c2 = (int)(g.n[2] - 1);
   const int sx = (int)g.field_stride[0], sy = (int)g.field_stride[1], sz = (int)g.field_stride[2];
   const int rofs = coords_rofs(g);
-  const CornerFetch<FDT> corners(g.field, g.bytes32, sx, sy, sz);
+  const CornerFetch<FDT, FAST> corners(g.field, g.bytes32, sx, sy, sz);
   float op = seed_value;
   int64_t taken = 0;
   for (int64_t it = 0; it < kMapMaxSteps; ++it) {
@@ -124,8 +125,19 @@ __device__ __forceinline__ float march_vertex(const GridParams& g, const MapPara
   return op;
 }
 
+template <int FDT, bool BINS = false, bool FAST = false>
+__device__ __forceinline__ float march_vertex(const GridParams& g, const MapParams& m,
+                                              const ShadeParams& s, const float* a_tab,
+                                              const float* const* x, int64_t id, float seed_value,
+                                              int64_t* taken_out)
+{
+  float v[3];
+  map_vertex(m, id, v);
+  return march_vertex_at<FDT, BINS, FAST>(g, m, s, a_tab, x, v, seed_value, taken_out);
+}
+
 // ---- LightedVolumeRenderer.cxx:345-498 ----------------------------------------------------
-template <int FDT, int MDT, bool USE_MAP, bool BINS = false>
+template <int FDT, int MDT, bool USE_MAP, bool BINS = false, bool FAST = false>
 __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapParams& m,
                                              const ShadeParams& s, const float4* tf,
                                              const float* const* x, const float* org,
@@ -156,13 +168,13 @@ __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapPa
   // 32-bit cells and strides: set_field keeps a field's xy plane below 2^31 points
   const int sx = (int)g.field_stride[0], sy = (int)g.field_stride[1], sz = (int)g.field_stride[2];
   const int rofs = coords_rofs(g);
-  const CornerFetch<FDT> corners(g.field, g.bytes32, sx, sy, sz);
-  const CornerFetch<MDT> map_corners(m.map, m.bytes32, (int)m.ms[0], (int)m.ms[1], (int)m.ms[2]);
+  const CornerFetch<FDT, FAST> corners(g.field, g.bytes32, sx, sy, sz);
+  const CornerFetch<MDT, FAST> map_corners(m.map, m.bytes32, (int)m.ms[0], (int)m.ms[1], (int)m.ms[2]);
   const float sd = s.sample_dist;
   MapCellCache mcache;
   while (inside(g, p) && dist < maxD && ++steps < max_steps) {
     // the shadow map's cell first: its corner loads, if any, overlap the field's
-    if (USE_MAP) map_cell_update<MDT>(m, map_corners, p, mcache);
+    if (USE_MAP) map_cell_update<MDT, FAST>(m, map_corners, p, mcache);
     const float mint = fminf(tx, fminf(ty, tz));
     const float maxt = fmaxf(tx, fmaxf(ty, tz));
     if (maxt > 1.0f || mint < 0.0f) newCell = true;
@@ -329,7 +341,7 @@ __device__ __forceinline__ void stage_tf(const ShadeParams& s, float4* tf)
     tf[i] = make_float4(s.tf[4 * i], s.tf[4 * i + 1], s.tf[4 * i + 2], s.tf[4 * i + 3]);
 }
 
-template <int FDT, int MDT, bool USE_MAP, bool BINS = false>
+template <int FDT, int MDT, bool USE_MAP, bool BINS = false, bool FAST = false>
 __device__ __forceinline__ void shade(const GridParams& g, const MapParams& m,
                                       const ShadeParams& s, const float4* tf,
                                       const float* const* x, const float* org, const float* dir,
@@ -337,7 +349,7 @@ __device__ __forceinline__ void shade(const GridParams& g, const MapParams& m,
 {
   const int64_t n = (g.assoc == 1)
                       ? sample_cell<FDT>(g, s, tf, x, org, dir, minD, maxD, color)
-                      : sample_point<FDT, MDT, USE_MAP, BINS>(g, m, s, tf, x, org, dir, minD, maxD, color);
+                      : sample_point<FDT, MDT, USE_MAP, BINS, FAST>(g, m, s, tf, x, org, dir, minD, maxD, color);
   if (s.counters) count_add(&s.counters[0], (uint32_t)n);
 }
 

@@ -133,3 +133,38 @@ def test_bricks_small_persistent_grids(oracle_mod, groups):
             assert np.array_equal(r.GetBrickMap(b, nv, final=True), ref_maps[b][1]), f"brick {b} final"
     assert np.array_equal(img.reshape(-1, 4), ref_img)
     assert np.array_equal(depth.reshape(-1), ref_depth)
+
+
+@pytest.mark.parametrize("knobs", [(0, 0, 0, 0, 0), (1, 0, 0, 1, 0), (0, 1, 0, 0, 1), (0, 0, 1, 1, 1),
+                                   (1, 1, 1, 0, 0), (1, 1, 1, 1, 1)])
+def test_bricks_schedules_give_the_same_frame(oracle_mod, knobs):
+    """Every schedule of the brick launches (XCD ranges in the persistent queues, light-sheet
+    row order in the map launches, XCD-contiguous march tiles, the branch-free corner fetch of
+    the map and march launches) renders the oracle's local and final maps and image bit for
+    bit, under a point light off every axis and a directional light along an axis (the sheet
+    grouping's degenerate case)."""
+    xcd, rows, tiles, mapf, marchf = knobs
+    for light, pos in (("point", None), ("directional", (0.0, 0.0, -1.0))):
+        sc = scenes.make_scene("sched", 45, seed=23, nblobs=10, width=72, height=56, light=light)
+        if pos is not None:
+            sc.light_pos = np.float32(pos)
+        ranges = B.kd_bricks(sc.dims, 12, 29, 8, 24)
+        print(f"[bricks schedule {knobs} {light}]", flush=True)
+        with B.LightedVolumeRenderer() as r:
+            scenes.configure(r, sc)
+            r.SetBricks(ranges, ghost=1)
+            r.SetSchedule(persistent_groups=96, xcd_ranges=xcd, light_rows=rows, march_xcd_tiles=tiles,
+                          map_fast_fetch=mapf, march_fast_fetch=marchf)
+            assert r.GetSchedule() == {"persistent_groups": 96, "xcd_ranges": xcd, "light_rows": rows,
+                                       "march_xcd_tiles": tiles, "map_fast_fetch": mapf,
+                                       "march_fast_fetch": marchf}
+            img, depth = r.RenderImage(sc.camera, sc.width, sc.height)
+            maps = []
+            for b, rg in enumerate(ranges):
+                nv = int(np.prod(rg[1::2] - rg[0::2] + 1))
+                maps.append((r.GetBrickMap(b, nv, final=False), r.GetBrickMap(b, nv, final=True)))
+        (oimg, local, final, _), o, d = _oracle(oracle_mod, sc, ranges)
+        for b in range(len(ranges)):
+            assert np.array_equal(maps[b][0], local[b]), f"brick {b} local map"
+            assert np.array_equal(maps[b][1], final[b]), f"brick {b} final map"
+        assert np.array_equal(img.reshape(-1, 4), oimg)
