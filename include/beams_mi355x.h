@@ -145,6 +145,8 @@ typedef struct beams_schedule {
   int32_t map_fast_fetch;    /* 1: the map launches take the branch-free corner fetch (24-bit
                                 offsets) when every brick's field and map allow it              */
   int32_t march_fast_fetch;  /* 1: the same for the march launch                                */
+  int32_t map_pipeline;      /* 1: the map launches issue each light step's corner loads one
+                                step ahead                                                      */
 } beams_schedule;
 int beams_set_schedule(beams_renderer* h, const beams_schedule* s);
 int beams_get_schedule(const beams_renderer* h, beams_schedule* s);

@@ -141,7 +141,8 @@ struct beams_renderer {
   int32_t n_map_items = 0, n_px_items = 0, max_coords = 0;
   int32_t persistent_groups = 2048;
   // schedule knobs of the brick launches (the same results whatever their values)
-  int32_t xcd_ranges = 1, light_rows = 1, march_xcd_tiles = 1, map_fast_fetch = 1, march_fast_fetch = 1;
+  int32_t xcd_ranges = 1, light_rows = 1, march_xcd_tiles = 1, map_fast_fetch = 0, march_fast_fetch = 1;
+  int32_t map_pipeline = 0;
   DevBuf brows;                          // every brick's map rows in light-sheet order
   std::vector<unsigned char> brows_key;  // the light and brick maps the rows were ordered for
   int64_t btab_version = 0;
@@ -778,6 +779,7 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
               d.g.field_stride[2] < (1 << 24) && d.m.ms[1] < (1 << 24) && d.m.ms[2] < (1 << 24);
   f.map_fast = fast_ok && h->map_fast_fetch;
   f.march_fast = fast_ok && h->march_fast_fetch;
+  f.map_pipeline = h->map_pipeline;
   h->n_map_items = f.n_map_items;
   h->n_px_items = f.n_px_items;
   h->max_coords = max_coords;
@@ -1242,6 +1244,7 @@ int beams_set_schedule(beams_renderer* h, const beams_schedule* s)
   h->march_xcd_tiles = s->march_xcd_tiles != 0;
   h->map_fast_fetch = s->map_fast_fetch != 0;
   h->march_fast_fetch = s->march_fast_fetch != 0;
+  h->map_pipeline = s->map_pipeline != 0;
   return BEAMS_OK;
 }
 
@@ -1254,6 +1257,7 @@ int beams_get_schedule(const beams_renderer* h, beams_schedule* s)
   s->march_xcd_tiles = h->march_xcd_tiles;
   s->map_fast_fetch = h->map_fast_fetch;
   s->march_fast_fetch = h->march_fast_fetch;
+  s->map_pipeline = h->map_pipeline;
   return BEAMS_OK;
 }
 
@@ -1317,10 +1321,10 @@ int beams_autotune(beams_renderer* h, const beams_camera* cam, int32_t width, in
         best = s;
       }
     }
-    for (int knob = 0; knob < 5 && !rc; ++knob) {
+    for (int knob = 0; knob < 6 && !rc; ++knob) {
       beams_schedule s = best;
-      int32_t* const v[5] = { &s.xcd_ranges, &s.light_rows, &s.march_xcd_tiles, &s.map_fast_fetch,
-                              &s.march_fast_fetch };
+      int32_t* const v[6] = { &s.xcd_ranges, &s.light_rows, &s.march_xcd_tiles, &s.map_fast_fetch,
+                              &s.march_fast_fetch, &s.map_pipeline };
       *v[knob] = !*v[knob];
       const float t = time_schedule(s);
       if (t < best_t) {

@@ -20,7 +20,7 @@ namespace beams {
 namespace {
 
 // ---- phases 1 and 3: per-brick TransmittanceMapGenerator -------------------------------------
-template <int FDT, int MDT, bool FAST>
+template <int FDT, int MDT, bool FAST, bool PIPE>
 __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
 {
   extern __shared__ float lds[];
@@ -79,7 +79,9 @@ __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
       if (unseeded) {
         copy_t<MDT>(bk->map_final, m.map, v);
       } else {
-        const float op = opaque ? 1.0f : march_vertex_at<FDT, false, FAST>(g, m, sp, a_tab, x, pos, seed, &taken);
+        const float op = opaque ? 1.0f
+                       : PIPE ? march_vertex_pipe<FDT, FAST>(g, m, sp, a_tab, x, pos, seed, &taken)
+                              : march_vertex_at<FDT, false, FAST>(g, m, sp, a_tab, x, pos, seed, &taken);
         store_t<MDT>(seeded ? bk->map_final : m.map, v, op);
       }
       if (sp.counters) count_add(&sp.counters[1], (uint32_t)taken);
@@ -576,10 +578,14 @@ hipError_t launch_bricks_map(const BrickFrame& f, int32_t field_dtype, int32_t m
   const size_t lds = sizeof(float) * (size_t)(ntf + 2 * f.max_coords);
   const dim3 grid((unsigned)f.persistent_groups);
   BRICK_DISPATCH_FIELD(field_dtype, BRICK_DISPATCH_MAP(map_dtype, {
-    if (f.map_fast)
-      hipLaunchKernelGGL((k_bricks_map<FDT, MDT, true>), grid, dim3(256), lds, st, f, seeded ? 1 : 0);
+    if (f.map_fast && f.map_pipeline)
+      hipLaunchKernelGGL((k_bricks_map<FDT, MDT, true, true>), grid, dim3(256), lds, st, f, seeded ? 1 : 0);
+    else if (f.map_fast)
+      hipLaunchKernelGGL((k_bricks_map<FDT, MDT, true, false>), grid, dim3(256), lds, st, f, seeded ? 1 : 0);
+    else if (f.map_pipeline)
+      hipLaunchKernelGGL((k_bricks_map<FDT, MDT, false, true>), grid, dim3(256), lds, st, f, seeded ? 1 : 0);
     else
-      hipLaunchKernelGGL((k_bricks_map<FDT, MDT, false>), grid, dim3(256), lds, st, f, seeded ? 1 : 0);
+      hipLaunchKernelGGL((k_bricks_map<FDT, MDT, false, false>), grid, dim3(256), lds, st, f, seeded ? 1 : 0);
   }));
   return hipGetLastError();
 }

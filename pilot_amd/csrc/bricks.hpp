@@ -48,6 +48,7 @@ struct BrickFrame {
   int32_t march_xcd_tiles;        // march: tiles dealt to the XCDs in contiguous blocks
   int32_t map_fast, march_fast;   // branch-free corner fetch in the map / march launches (every
                                   // brick's field and map view below 2^32 bytes, strides below 2^24)
+  int32_t map_pipeline;           // map launches: next light step's loads in flight (march_vertex_pipe)
 };
 
 hipError_t launch_bricks_map(const BrickFrame& f, int32_t field_dtype, int32_t map_dtype,

@@ -106,6 +106,36 @@ struct CornerFetch {
     }
     return FDT == kF16 ? f16_to_f32(h) : bf16_to_f32(h);
   }
+  // The raw storage words of the 8 corners (FP32 bits, or FP16/BF16 bits in the low half),
+  // for callers that keep loads in flight and decode them later with decode().
+  __device__ __forceinline__ void load_raw(int ci, int cj, int ck, uint32_t* w) const
+  {
+    if (w32) {
+      const uint32_t e = FAST ? __umul24((uint32_t)ci, (uint32_t)sx) + __umul24((uint32_t)cj, (uint32_t)sy) +
+                                  __umul24((uint32_t)ck, (uint32_t)sz)
+                              : (uint32_t)(ci * sx + cj * sy + ck * sz);
+      const int off = (int)(e * (uint32_t)(FDT == kF32 ? 4 : 2));
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        w[q] = FDT == kF32 ? __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, so[q], 0)
+                           : (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(rsrc, off, so[q], 0);
+    } else {
+      const int64_t b = (int64_t)ci * sx + (int64_t)cj * sy + (int64_t)ck * sz;
+      const int64_t d[8] = { 0, sx, sx + sy, sy, sz, sz + sx, sz + sx + sy, sz + sy };
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        w[q] = FDT == kF32 ? ((const uint32_t*)field)[b + d[q]] : (uint32_t)((const uint16_t*)field)[b + d[q]];
+    }
+  }
+  __device__ __forceinline__ static float decode(uint32_t w)
+  {
+    if (FDT == kF32) return __builtin_bit_cast(float, w);
+    if (FDT == kBF16) return bf16_to_f32((uint16_t)w);
+    float r;
+    // a plain conversion (not folded into the interpolation as v_fma_mix_f32)
+    asm("v_cvt_f32_f16 %0, %1" : "=v"(r) : "v"(w));
+    return r;
+  }
   __device__ __forceinline__ void load(int ci, int cj, int ck, float* v) const
   {
     if (FAST) {
