@@ -317,10 +317,14 @@ int beams_mlp_train(beams_mlp* h, const float* x, const float* t, int64_t n, int
 /* Attach the network used by BEAMS_SHADOW_LEARNED (net NULL: the coarse optical depth alone).
  * fallback_mode (BEAMS_SHADOW_REF_MARCH or BEAMS_SHADOW_SWEEP) builds the map instead when the
  * largest opacity error at the check vertices (every check_stride-th map vertex per axis and
- * the last) exceeds error_budget.  With bricks the check and the fallback are per brick: each
- * brick's map is predicted from the whole volume (no non-local exchange), and a brick over
- * the budget is re-marched through the whole volume with the single-block light march.  The renderer keeps the pointer: destroy the renderer or
- * attach another network before destroying net. */
+ * the last) exceeds error_budget; a negative budget sends every map to the fallback (the
+ * learned frame's exact counterpart), a NaN one is refused.  The decision is taken on the
+ * device (the fallback's launches return at once when the check is within the budget), so a
+ * learned frame is captured in one graph like the exact ones.  The learned map approximates
+ * the single-block map (the whole volume's light march, step |size| / 128).  With bricks that
+ * is the single-block map at each brick's vertices, checked and rebuilt per brick: no
+ * non-local exchange and no per-brick light step, unlike the exact brick frame.  The renderer
+ * keeps the pointer: destroy the renderer or attach another network before destroying net. */
 int beams_set_learned(beams_renderer* h, const beams_mlp* net, int32_t fallback_mode,
                       float error_budget, int32_t check_stride);
 /* Training data from the current field, colormap and light: builds the exact map with
@@ -328,10 +332,11 @@ int beams_set_learned(beams_renderer* h, const beams_mlp* net, int32_t fallback_
  * (n x 32) and targets (n) of n map vertices drawn uniformly by SplitMix64(seed). */
 int beams_learned_samples(beams_renderer* h, int32_t mode, int64_t n, uint64_t seed,
                           float* features, float* targets, int32_t mem);
-/* The last learned map build: err[0] = largest |learned - exact| opacity at the check
- * vertices, err[1] = the same for the coarse optical depth alone (single block only, else -1);
- * *fell_back = the number of maps the fallback built (0 or 1 for the single block; with
- * bricks, the bricks over the budget, whose maps the whole volume's light march rebuilds). */
+/* The last learned map build (read back from the device: waits for the stream): err[0] =
+ * largest |learned - exact| opacity at the check vertices, err[1] = the same for the coarse
+ * optical depth alone (single block only, else -1); *fell_back = the number of maps the
+ * fallback built (0 or 1 for the single block; with bricks, the bricks over the budget,
+ * whose maps the whole volume's light march rebuilds).  -1, -1 and 0 before any. */
 int beams_get_learned_stats(beams_renderer* h, float err[2], int32_t* fell_back);
 
 #ifdef __cplusplus

@@ -64,6 +64,9 @@ typedef struct OrcRays {
   int orc_tmap_build_##SUF(const OrcScene* s, const void* seed, void* map_out, int nthr);  \
   int orc_tmap_build_range_##SUF(const OrcScene* s, const void* seed, void* map_out,      \
                                  int64_t begin, int64_t end, int nthr);                    \
+  int orc_map_points_##SUF(const OrcScene* s, void* const* pts);                           \
+  int orc_march_points_##SUF(const OrcScene* s, const void* const* pts, int64_t n,         \
+                             void* out, int nthr);                                         \
   int orc_map_estimate_##SUF(const OrcScene* s, const void* map, const void* point,        \
                              void* rgb_out, void* opacity_out);                            \
   int orc_calc_ray_start_##SUF(const OrcScene* s, OrcRays* rays, int nthr);                \

@@ -82,6 +82,8 @@ def load(path):
             "orc_tmap_build": [VP, VP, VP, I32],
             "orc_tmap_build_range": [VP, VP, VP, I64, I64, I32],
             "orc_map_estimate": [VP, VP, VP, VP, VP],
+            "orc_march_points": [VP, VP, I64, VP, I32],
+            "orc_map_points": [VP, VP],
             "orc_calc_ray_start": [VP, VP, I32],
             "orc_sample_rays": [VP, VP, VP, I32],
             "orc_render_rays": [VP, VP, VP, I32],
@@ -203,6 +205,28 @@ def nonlocal_seed_range(vols, maps, self_b, begin, end, out, nthreads=8):
     getattr(lib(), f"orc_nonlocal_seed_range_{twin}")(len(vols), arr, mptr, int(self_b), int(begin),
                                                       int(end), out.ctypes.data, nthreads)
     return out
+
+
+def march_points(vol, points, nthreads=8):
+    """The light march of vol (its bounds and light step) to each of the points (3 x n): the
+    map value a vertex at each point would get."""
+    dt = _dt(vol.twin)
+    pts = [np.ascontiguousarray(points[d], dt) for d in range(3)]
+    out = np.zeros(len(pts[0]), dt)
+    arr = (C.c_void_p * 3)(*[p.ctypes.data for p in pts])
+    getattr(lib(), f"orc_march_points_{vol.twin}")(C.byref(vol.s), arr, len(pts[0]), out.ctypes.data,
+                                                   nthreads)
+    return out
+
+
+def map_points(vol):
+    """The positions of vol's map vertices (x fastest) as 3 arrays: origin + spacing * i over
+    the block's bounds (TransmittanceMap.h:550-563)."""
+    dt = _dt(vol.twin)
+    pts = [np.zeros(vol.nverts, dt) for _ in range(3)]
+    arr = (C.c_void_p * 3)(*[p.ctypes.data for p in pts])
+    getattr(lib(), f"orc_map_points_{vol.twin}")(C.byref(vol.s), arr)
+    return pts
 
 
 def render_rays(vol, rays, nthreads=8):

@@ -233,12 +233,11 @@ static inline void FN(light_origin)(const FN(Geo) * g, int kind, const R* v, R* 
   }
 }
 
-/* ---- TransmittanceMap.h:279-331 (one vertex) ---------------------------------------- */
-static R FN(march_vertex)(const FN(Geo) * g, const R* field, int64_t nfield, int kind,
-                          int64_t id, R opacity)
+/* ---- TransmittanceMap.h:279-331 (the light march to one point v) ------------------------ */
+static R FN(march_point)(const FN(Geo) * g, const R* field, int64_t nfield, int kind, const R* v,
+                         R opacity)
 {
-  R v[3], o[3], dir[3];
-  FN(map_vertex)(g, id, v);
+  R o[3], dir[3];
   FN(light_origin)(g, kind, v, o);
   /* vtkm::Normal(dest - origin) = x * (1 / sqrt(|x|^2)) */
   R dd[3] = { v[0] - o[0], v[1] - o[1], v[2] - o[2] };
@@ -279,10 +278,53 @@ static R FN(march_vertex)(const FN(Geo) * g, const R* field, int64_t nfield, int
   return opacity;
 }
 
+static R FN(march_vertex)(const FN(Geo) * g, const R* field, int64_t nfield, int kind,
+                          int64_t id, R opacity)
+{
+  R v[3];
+  FN(map_vertex)(g, id, v);
+  return FN(march_point)(g, field, nfield, kind, v, opacity);
+}
+
 static int64_t FN(nfield)(const OrcScene* s)
 {
   if (s->assoc == 1) return (s->dims[0] - 1) * (s->dims[1] - 1) * (s->dims[2] - 1);
   return s->dims[0] * s->dims[1] * s->dims[2];
+}
+
+/* The positions of block s's map vertices (x fastest) into the x, y, z arrays pts[0..2]. */
+int FN(orc_map_points)(const OrcScene* s, void* const* pts)
+{
+  FN(Geo) g;
+  FN(geo_init)(&g, s);
+  const int64_t nv = g.mp[0] * g.mp[1] * g.mp[2];
+  for (int64_t id = 0; id < nv; ++id) {
+    R v[3];
+    FN(map_vertex)(&g, id, v);
+    for (int d = 0; d < 3; ++d) ((R*)pts[d])[id] = v[d];
+  }
+  return 0;
+}
+
+/* The light march of block s (its bounds, its light step |size| / 128) to n given points
+ * (x, y, z arrays): the map value a vertex at each point would get. */
+int FN(orc_march_points)(const OrcScene* s, const void* const* pts, int64_t n, void* out, int nthr)
+{
+  FN(Geo) g;
+  FN(geo_init)(&g, s);
+  const R* field = (const R*)s->field;
+  const int64_t nf = FN(nfield)(s);
+  const R* px = (const R*)pts[0];
+  const R* py = (const R*)pts[1];
+  const R* pz = (const R*)pts[2];
+  R* o = (R*)out;
+  (void)nthr;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthr > 0 ? nthr : 1)
+  for (int64_t i = 0; i < n; ++i) {
+    const R v[3] = { px[i], py[i], pz[i] };
+    o[i] = FN(march_point)(&g, field, nf, s->light_kind, v, (R)0);
+  }
+  return 0;
 }
 
 int FN(orc_tmap_build)(const OrcScene* s, const void* seed, void* map_out, int nthr)

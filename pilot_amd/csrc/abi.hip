@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -105,8 +106,10 @@ struct beams_renderer {
   int32_t check_stride = 8;
   DevBuf coarse;  // coarse extinction and optical depth, (kCoarseCells + 1)^3 each
   DevBuf lerr;    // check errors as float bits
-  float last_err[2] = { -1.0f, -1.0f };
-  int32_t fell_back = 0;
+  // the last learned build, whose statistics are read back on request: 0 none, 1 single block
+  // (lerr holds its two check errors), 2 bricks (bredo holds the fallback's work list)
+  int32_t learned_kind = 0;
+  float built_budget = 0.0f;
   // march
   float sample_distance = -1.0f;
   float alpha_stop = 1.0f;
@@ -136,7 +139,7 @@ struct beams_renderer {
   std::vector<Brick> brick_pool;  // storage of the decomposition a new field replaced
   int32_t ghost = 0;
   DevBuf bmap_local, bmap_final, bseeds, btables, bpartial, bqueue;
-  DevBuf bredo;  // learned bricks: per-brick fallback flags
+  DevBuf bredo;  // learned bricks: the fallback's work list (learned_redo_words)
   std::vector<unsigned char> btab_host;  // tables of the last frame (reused while unchanged)
   int32_t n_map_items = 0, n_px_items = 0, max_coords = 0;
   int32_t persistent_groups = 2048;
@@ -473,38 +476,51 @@ LearnedParams learned_params(beams_renderer* h, const GridParams& g, const Shade
   return p;
 }
 
-// The learned map: coarse optical depth, the network's correction per vertex, the sparse
-// exact check, and the fallback build when the check exceeds the budget (a host decision:
-// the check's two words are read back before the march is enqueued).
-int build_learned(beams_renderer* h, const GridParams& g, const ShadeParams& s)
+void learned_built(beams_renderer* h, int32_t kind)
+{
+  h->learned_kind = kind;
+  h->built_budget = h->error_budget;
+}
+
+// Device buffers of the learned map build and its fallback (allocated outside any capture).
+int prepare_learned(beams_renderer* h)
 {
   int rc = prepare_map(h, BEAMS_SHADOW_LEARNED);
-  if (rc) return rc;
+  if (!rc) rc = prepare_map(h, h->fallback_mode);
+  return rc;
+}
+
+// The learned map: coarse optical depth, the network's correction per vertex, the sparse exact
+// check, and the fallback build gated on the device by the check's largest error (it returns at
+// once when the error is within the budget), so the whole sequence is enqueued without a host
+// round trip and can be captured.  The fallback writes the learned map's layout (x fastest),
+// so the march reads the map the same way whichever build ran.
+int enqueue_learned(beams_renderer* h, const GridParams& g, const ShadeParams& s)
+{
   const MapParams m = map_params(h, g, BEAMS_SHADOW_LEARNED);
   const LearnedParams p = learned_params(h, g, s, m);
   HIP_TRY(h, launch_coarse(p, h->stream));
   HIP_TRY(h, launch_learned_map(p, h->stream));
   HIP_TRY(h, launch_zero_u64((unsigned long long*)h->lerr.p, 1, h->stream));
   HIP_TRY(h, launch_learned_check(p, h->stream));
-  unsigned int e[2] = { 0, 0 };
-  HIP_TRY(h, hipMemcpyAsync(e, h->lerr.p, sizeof(e), hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(h, hipStreamSynchronize(h->stream));
-  h->last_err[0] = bits_f32(e[0]);
-  h->last_err[1] = bits_f32(e[1]);
-  h->fell_back = !(h->last_err[0] <= h->error_budget);
-  h->built_mode = BEAMS_SHADOW_LEARNED;
-  if (h->fell_back) {
-    if ((rc = prepare_map(h, h->fallback_mode))) return rc;
-    if ((rc = enqueue_map(h, g, map_params(h, g, h->fallback_mode), s, h->fallback_mode))) return rc;
-    h->built_mode = h->fallback_mode;
-  }
-  h->map_valid = true;
-  return BEAMS_OK;
+  MapParams fm = map_params(h, g, h->fallback_mode);
+  for (int d = 0; d < 3; ++d) fm.ms[d] = m.ms[d];
+  fm.gate = (const unsigned int*)h->lerr.p;
+  fm.gate_budget = h->error_budget;
+  return enqueue_map(h, g, fm, s, h->fallback_mode);
 }
 
 int build_map(beams_renderer* h, const GridParams& g, const ShadeParams& s)
 {
-  if (h->shadow_mode == BEAMS_SHADOW_LEARNED) return build_learned(h, g, s);
+  if (h->shadow_mode == BEAMS_SHADOW_LEARNED) {
+    int rc = prepare_learned(h);
+    if (!rc) rc = enqueue_learned(h, g, s);
+    if (rc) return rc;
+    learned_built(h, 1);
+    h->built_mode = BEAMS_SHADOW_LEARNED;
+    h->map_valid = true;
+    return BEAMS_OK;
+  }
   int rc = prepare_map(h, h->shadow_mode);
   if (rc) return rc;
   MapParams m = map_params(h, g, h->shadow_mode);
@@ -819,57 +835,44 @@ int run_frame(beams_renderer* h, const std::vector<unsigned char>& key, Issue&& 
   return phase_mark(h, BEAMS_PHASE_FRAME, true);
 }
 
-// RenderOnDevice phases 1-4 for every brick + GlobalComposite + BlendBackground.
-// Learned bricks: every brick's final map from the whole volume's coarse optical depth and
-// the network, a per-brick sparse check against the whole volume's light march, and that
-// march for the bricks over the budget (timed as phases 1 and 3).  The whole field stays
-// resident beside the bricks' ghosted copies.
-int build_bricks_learned(beams_renderer* h, const BrickFrame& f)
+// Learned bricks (phases 1 and 3 of the frame): every brick's final map from the whole volume's
+// coarse optical depth and the network, a per-brick sparse check against the whole volume's
+// light march (the single-block map at the brick's vertices), the fallback's work list built on
+// the device from the checks, and that march for every vertex of the listed bricks.  The whole
+// field stays resident beside the bricks' ghosted copies.
+int prepare_bricks_learned(beams_renderer* h, int nb)
 {
-  const int nb = f.nbricks;
   const size_t nc = (size_t)(kCoarseCells + 1) * (kCoarseCells + 1) * (kCoarseCells + 1);
   HIP_TRY(h, h->coarse.ensure(2 * nc * sizeof(float) + 6 * (kCoarseCells + 1) * sizeof(int32_t)));
-  HIP_TRY(h, h->lerr.ensure(sizeof(float) * (size_t)nb));
-  HIP_TRY(h, h->bredo.ensure(sizeof(int32_t) * (size_t)nb));
+  HIP_TRY(h, h->lerr.ensure(sizeof(unsigned int) * (size_t)nb));
+  HIP_TRY(h, h->bredo.ensure(sizeof(int32_t) * learned_redo_words(nb)));
+  return BEAMS_OK;
+}
+
+LearnedParams bricks_learned_params(beams_renderer* h)
+{
   const GridParams g = grid_params(h);
   const float user_sd = h->sample_distance;
   ShadeParams s = shade_params(h, g);
   h->sample_distance = user_sd;
   s.counters = nullptr;
-  const LearnedParams p = learned_params(h, g, s, map_params(h, g, BEAMS_SHADOW_LEARNED));
+  return learned_params(h, g, s, map_params(h, g, BEAMS_SHADOW_LEARNED));
+}
+
+int enqueue_bricks_learned(beams_renderer* h, const BrickFrame& f, const LearnedParams& p,
+                           const std::function<int(int, bool)>& mark)
+{
   int rc;
-  if ((rc = phase_mark(h, BEAMS_PHASE_MAP, false))) return rc;
+  if ((rc = mark(BEAMS_PHASE_MAP, false))) return rc;
   HIP_TRY(h, launch_coarse(p, h->stream));
   HIP_TRY(h, launch_bricks_learned_map(p, f, h->map_dtype, h->stream));
-  HIP_TRY(h, launch_bricks_learned_check(p, f, h->map_dtype, (float*)h->lerr.p, h->stream));
-  if ((rc = phase_mark(h, BEAMS_PHASE_MAP, true))) return rc;
-  std::vector<float> err((size_t)nb);
-  HIP_TRY(h, hipMemcpyAsync(err.data(), h->lerr.p, sizeof(float) * (size_t)nb, hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(h, hipStreamSynchronize(h->stream));
-  std::vector<int32_t> redo((size_t)nb, 0);
-  int32_t count = 0;
-  float worst = 0.0f;
-  for (int b = 0; b < nb; ++b) {
-    worst = std::fmax(worst, err[(size_t)b]);
-    if (!(err[(size_t)b] <= h->error_budget)) {
-      redo[(size_t)b] = 1;
-      ++count;
-    }
-  }
-  h->last_err[0] = worst;
-  h->last_err[1] = -1.0f;
-  h->fell_back = count;
-  if (count) {
-    HIP_TRY(h, hipMemcpyAsync(h->bredo.p, redo.data(), sizeof(int32_t) * (size_t)nb, hipMemcpyHostToDevice,
-                              h->stream));
-    if ((rc = phase_mark(h, BEAMS_PHASE_REMAP, false))) return rc;
-    HIP_TRY(h, launch_zero_u32(f.queue + 4 * kQueueSlots, kQueueSlots, h->stream));
-    HIP_TRY(h, launch_bricks_exact_map(p, f, h->map_dtype, (const int32_t*)h->bredo.p, h->stream));
-    if ((rc = phase_mark(h, BEAMS_PHASE_REMAP, true))) return rc;
-    // the host vector must outlive the asynchronous copy
-    HIP_TRY(h, hipStreamSynchronize(h->stream));
-  }
-  return BEAMS_OK;
+  HIP_TRY(h, launch_bricks_learned_check(p, f, h->map_dtype, (unsigned int*)h->lerr.p, h->stream));
+  HIP_TRY(h, launch_learned_redo(f, (const unsigned int*)h->lerr.p, h->error_budget, (int32_t*)h->bredo.p,
+                                 h->stream));
+  if ((rc = mark(BEAMS_PHASE_MAP, true))) return rc;
+  if ((rc = mark(BEAMS_PHASE_REMAP, false))) return rc;
+  HIP_TRY(h, launch_bricks_exact_map(p, f, h->map_dtype, (const int32_t*)h->bredo.p, h->stream));
+  return mark(BEAMS_PHASE_REMAP, true);
 }
 
 int render_bricks(beams_renderer* h, const CameraParams& c, const float* background, float* rgba,
@@ -906,11 +909,18 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
   }
   o.use_bg = background ? 1 : 0;
   for (int k = 0; k < 4; ++k) o.bg[k] = background ? background[k] : 0.0f;
-  if (learned && (rc = build_bricks_learned(h, f))) return rc;
+  LearnedParams lp{};
+  if (learned) {
+    if ((rc = prepare_bricks_learned(h, f.nbricks))) return rc;
+    lp = bricks_learned_params(h);
+  }
   auto issue = [&](bool timed) -> int {
     int r;
-    auto mark = [&](int p, bool end) { return timed ? phase_mark(h, p, end) : BEAMS_OK; };
-    HIP_TRY(h, launch_zero_u32(f.queue, 4 * kQueueSlots, h->stream));
+    const std::function<int(int, bool)> mark = [&](int p, bool end) {
+      return timed ? phase_mark(h, p, end) : BEAMS_OK;
+    };
+    HIP_TRY(h, launch_zero_u32(f.queue, 5 * kQueueSlots, h->stream));
+    if (learned && (r = enqueue_bricks_learned(h, f, lp, mark))) return r;
     if (shadows && !learned) {
       if ((r = mark(BEAMS_PHASE_MAP, false))) return r;
       HIP_TRY(h, launch_bricks_map(f, h->dtype, h->map_dtype, ntf, false, h->stream));
@@ -932,7 +942,9 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
   KeyBuilder key;
   key.put(1).put(f).put(c).put(o).put(h->btab_version).put(shadows).put(learned).put(h->dtype)
      .put(h->map_dtype).put(ntf);
+  if (learned) key.put(lp).put(h->error_budget);
   if ((rc = run_frame(h, key.bytes, issue))) return rc;
+  if (learned) learned_built(h, 2);
   if (shadows) h->map_valid = true;
   if (mem == BEAMS_MEM_HOST) {
     HIP_TRY(h, hipMemcpyAsync(rgba, o.rgba, sizeof(float) * 4 * npix, hipMemcpyDeviceToHost, h->stream));
@@ -1566,16 +1578,12 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
   if ((rc = frame_begin(h))) return rc;
   GridParams g = grid_params(h);
   ShadeParams s = shade_params(h, g);
-  // The learned map is built eagerly (its fallback decision reads the check back); the exact
-  // modes' map build is part of the frame's launch sequence (and graph).
-  const bool learned = h->use_shadow && h->assoc == 0 && h->shadow_mode == BEAMS_SHADOW_LEARNED;
-  const bool with_map = h->use_shadow && h->assoc == 0 && !learned;
-  if (learned) {
-    if ((rc = phase_mark(h, BEAMS_PHASE_MAP, false))) return rc;
-    if ((rc = build_learned(h, g, s))) return rc;
-    if ((rc = phase_mark(h, BEAMS_PHASE_MAP, true))) return rc;
-  }
-  if (with_map && (rc = prepare_map(h, h->shadow_mode))) return rc;
+  // The map build (learned, with its check and gated fallback, or exact) is part of the frame's
+  // launch sequence and so of its graph.
+  const bool with_map = h->use_shadow && h->assoc == 0;
+  const bool learned = with_map && h->shadow_mode == BEAMS_SHADOW_LEARNED;
+  if (learned && (rc = prepare_learned(h))) return rc;
+  if (with_map && !learned && (rc = prepare_map(h, h->shadow_mode))) return rc;
   const size_t npix = (size_t)width * (size_t)height;
   ImageOut o{};
   if (mem == BEAMS_MEM_DEVICE) {
@@ -1588,13 +1596,13 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
   }
   o.use_bg = background ? 1 : 0;
   for (int k = 0; k < 4; ++k) o.bg[k] = background ? background[k] : 0.0f;
-  const MapParams m = map_params(h, g, learned ? h->built_mode : h->shadow_mode);
+  const MapParams m = map_params(h, g, h->shadow_mode);
   auto issue = [&](bool timed) -> int {
     int r;
     auto mark = [&](int p, bool end) { return timed ? phase_mark(h, p, end) : BEAMS_OK; };
     if (with_map) {
       if ((r = mark(BEAMS_PHASE_MAP, false))) return r;
-      if ((r = enqueue_map(h, g, m, s, h->shadow_mode))) return r;
+      if ((r = learned ? enqueue_learned(h, g, s) : enqueue_map(h, g, m, s, h->shadow_mode))) return r;
       if ((r = mark(BEAMS_PHASE_MAP, true))) return r;
     }
     if ((r = mark(BEAMS_PHASE_MARCH, false))) return r;
@@ -1603,10 +1611,13 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
   };
   KeyBuilder key;
   key.put(0).put(g).put(m).put(s).put(c).put(o).put(with_map);
+  if (learned)
+    key.put(h->fallback_mode).put(h->error_budget).put(h->check_stride).put(h->net ? mlp_device_params(h->net) : nullptr);
   if ((rc = run_frame(h, key.bytes, issue))) return rc;
   if (with_map) {
     h->map_valid = true;
     h->built_mode = h->shadow_mode;
+    if (learned) learned_built(h, 1);
   }
   if (mem == BEAMS_MEM_HOST) {
     HIP_TRY(h, hipMemcpyAsync(rgba, o.rgba, sizeof(float) * 4 * npix, hipMemcpyDeviceToHost, h->stream));
@@ -1653,7 +1664,7 @@ int beams_set_learned(beams_renderer* h, const beams_mlp* net, int32_t fallback_
   CHECK_HANDLE(h);
   if (fallback_mode != BEAMS_SHADOW_REF_MARCH && fallback_mode != BEAMS_SHADOW_SWEEP)
     return h->fail(BEAMS_ERR_UNSUPPORTED, "the fallback must be an exact mode");
-  if (!(error_budget >= 0.0f)) return h->fail(BEAMS_ERR_INVALID, "error budget must be >= 0");
+  if (error_budget != error_budget) return h->fail(BEAMS_ERR_INVALID, "error budget is NaN");
   if (check_stride < 1) return h->fail(BEAMS_ERR_INVALID, "check stride must be >= 1");
   h->net = net;
   h->fallback_mode = fallback_mode;
@@ -1704,11 +1715,25 @@ int beams_learned_samples(beams_renderer* h, int32_t mode, int64_t n, uint64_t s
 int beams_get_learned_stats(beams_renderer* h, float err[2], int32_t* fell_back)
 {
   CHECK_HANDLE(h);
-  if (err) {
-    err[0] = h->last_err[0];
-    err[1] = h->last_err[1];
+  float e[2] = { -1.0f, -1.0f };
+  int32_t fb = 0;
+  if (h->learned_kind) {
+    uint32_t w[2] = { 0, 0 };
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    HIP_TRY(h, hipMemcpy(w, h->learned_kind == 1 ? h->lerr.p : h->bredo.p, sizeof(w), hipMemcpyDeviceToHost));
+    e[0] = bits_f32(w[0]);
+    if (h->learned_kind == 1) {
+      e[1] = bits_f32(w[1]);
+      fb = !(e[0] <= h->built_budget);
+    } else {
+      fb = (int32_t)w[1];
+    }
   }
-  if (fell_back) *fell_back = h->fell_back;
+  if (err) {
+    err[0] = e[0];
+    err[1] = e[1];
+  }
+  if (fell_back) *fell_back = fb;
   return BEAMS_OK;
 }
 

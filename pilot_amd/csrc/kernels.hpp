@@ -42,7 +42,20 @@ struct MapParams {
   float light_color[3];
   float dir_far;       // directional-light origin distance
   uint32_t bytes32;    // bytes of the map array when below 2^32, else 0
+  // Optional gate of a fallback build: when set, the build runs only if the float stored (as
+  // bits) at *gate, a check's largest error, exceeds gate_budget (a NaN error exceeds it).
+  const unsigned int* gate;
+  float gate_budget;
 };
+
+// True when a gated build is to be skipped (uniform: every lane reads the same word, which an
+// earlier kernel wrote).
+__device__ __forceinline__ bool gate_closed(const MapParams& m)
+{
+  if (!m.gate) return false;
+  const float e = __uint_as_float((unsigned)__builtin_amdgcn_readfirstlane((int)*m.gate));
+  return e <= m.gate_budget;
+}
 
 // Bytes of a map array of mp[0] mp[1] mp[2] values of dtype when below 2^32, else 0.
 inline uint32_t map_bytes32(const int64_t* mp, int32_t dtype)

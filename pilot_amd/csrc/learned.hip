@@ -437,10 +437,13 @@ __device__ __forceinline__ MapParams whole_volume_march(const MapParams& brick_m
   return m;
 }
 
-// Per brick (one workgroup each): largest |learned - exact| over its check vertices, the exact
-// value being the whole volume's light march (the single-block reference march).
+// Per brick: largest |learned - exact| over its check vertices, the exact value being the
+// whole volume's light march (the single-block reference march).  `parts` workgroups share a
+// brick, each taking every parts-th block of 256 check vertices, and fold their maxima into
+// err[b] (non-negative floats order as their bits; a NaN's bits order above every number).
 template <int FDT, int MDT>
-__global__ void __launch_bounds__(kBlock) k_bricks_learned_check(LearnedParams p, BrickFrame f, float* err)
+__global__ void __launch_bounds__(kBlock) k_bricks_learned_check(LearnedParams p, BrickFrame f, unsigned int* err,
+                                                                 int32_t parts)
 {
   extern __shared__ float lds[];
   float* a_tab = lds;
@@ -450,13 +453,14 @@ __global__ void __launch_bounds__(kBlock) k_bricks_learned_check(LearnedParams p
   stage_coords(p.g, xs);
   __syncthreads();
   const float* x[3] = { xs, xs + p.g.n[0], xs + p.g.n[0] + p.g.n[1] };
-  const BrickDesc& d = f.bricks[blockIdx.x];
+  const int b = (int)blockIdx.x / parts, part = (int)blockIdx.x % parts;
+  const BrickDesc& d = f.bricks[b];
   const MapParams m = whole_volume_march(d.m, p.m);
   const int64_t n0 = check_axis(m.mp[0] - 1, p.check_stride);
   const int64_t n1 = check_axis(m.mp[1] - 1, p.check_stride);
   const int64_t n2 = check_axis(m.mp[2] - 1, p.check_stride);
   float worst = 0.0f;
-  for (int64_t q = threadIdx.x; q < n0 * n1 * n2; q += blockDim.x) {
+  for (int64_t q = (int64_t)part * kBlock + threadIdx.x; q < n0 * n1 * n2; q += (int64_t)parts * kBlock) {
     const int64_t qi[3] = { q % n0, (q / n0) % n1, q / (n0 * n1) };
     int64_t vi[3];
 #pragma unroll
@@ -467,23 +471,84 @@ __global__ void __launch_bounds__(kBlock) k_bricks_learned_check(LearnedParams p
     const int64_t id = vi[0] + m.mp[0] * (vi[1] + m.mp[1] * vi[2]);
     int64_t taken = 0;
     const float exact = march_vertex<FDT>(p.g, m, p.s, a_tab, x, id, 0.0f, &taken);
-    worst = fmaxf(worst, fabsf(load_t<MDT>(d.map_final, id) - exact));
+    const float e = fabsf(load_t<MDT>(d.map_final, id) - exact);
+    worst = (e > worst || e != e) ? e : worst;
   }
+  unsigned wb = f32_bits(worst);
 #pragma unroll
-  for (int mk = 32; mk >= 1; mk >>= 1) worst = fmaxf(worst, __shfl_xor(worst, mk, 64));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = worst;
+  for (int mk = 32; mk >= 1; mk >>= 1) wb = max(wb, (unsigned)__shfl_xor((int)wb, mk, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = bits_f32(wb);
   __syncthreads();
   if (threadIdx.x == 0) {
-    float w = red[0];
-    for (int i = 1; i < kBlock / 64; ++i) w = fmaxf(w, red[i]);
-    err[blockIdx.x] = w;
+    unsigned w = f32_bits(red[0]);
+    for (int i = 1; i < kBlock / 64; ++i) w = max(w, f32_bits(red[i]));
+    atomicMax(&err[b], w);
   }
 }
 
-// Fallback: the whole volume's light march for every vertex of the bricks flagged in redo.
-template <int FDT, int MDT>
-__global__ void __launch_bounds__(kBlock) k_bricks_exact_map(LearnedParams p, BrickFrame f, const int32_t* redo)
+// The fallback's work list from the per-brick checks (one workgroup; each thread takes a run
+// of consecutive bricks, and a scan over the threads' counts places every run's entries).
+__global__ void __launch_bounds__(kBlock) k_learned_redo(BrickFrame f, const unsigned int* err, float budget,
+                                                         int32_t* list)
 {
+  __shared__ int32_t cnt[kBlock + 1], items[kBlock + 1];
+  __shared__ unsigned worst[kBlock];
+  const int nb = f.nbricks, t = (int)threadIdx.x;
+  const int per = (nb + kBlock - 1) / kBlock;
+  const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+  int c = 0, it = 0;
+  unsigned w = 0;
+  for (int b = b0; b < b1; ++b) {
+    const unsigned e = err[b];
+    w = max(w, e);
+    if (!(bits_f32(e) <= budget)) {
+      ++c;
+      it += f.map_item_off[b + 1] - f.map_item_off[b];
+    }
+  }
+  cnt[t] = c;
+  items[t] = it;
+  worst[t] = w;
+  __syncthreads();
+  if (t == 0) {
+    int32_t sc = 0, si = 0;
+    unsigned sw = 0;
+    for (int i = 0; i < kBlock; ++i) {
+      const int32_t ci = cnt[i], ii = items[i];
+      cnt[i] = sc;
+      items[i] = si;
+      sc += ci;
+      si += ii;
+      sw = max(sw, worst[i]);
+    }
+    cnt[kBlock] = sc;
+    items[kBlock] = si;
+    list[0] = (int32_t)sw;
+    list[1] = sc;
+    list[2] = si;
+    list[3 + sc] = si;
+  }
+  __syncthreads();
+  int k = cnt[t], o = items[t];
+  for (int b = b0; b < b1; ++b) {
+    if (!(bits_f32(err[b]) <= budget)) {
+      list[3 + k] = o;
+      list[4 + nb + k] = b;
+      ++k;
+      o += f.map_item_off[b + 1] - f.map_item_off[b];
+    }
+  }
+}
+
+// Fallback: the whole volume's light march for every vertex of the listed bricks.  The list
+// was written by an earlier kernel, so its words are read through the scalar cache.
+template <int FDT, int MDT>
+__global__ void __launch_bounds__(kBlock) k_bricks_exact_map(LearnedParams p, BrickFrame f, const int32_t* list)
+{
+  typedef const __attribute__((address_space(4))) int32_t ConstInt;
+  ConstInt* lw = (ConstInt*)(const void*)list;
+  const int count = lw[1], nitems = lw[2];
+  if (nitems == 0) return;
   extern __shared__ float lds[];
   float* a_tab = lds;
   float* xs = lds + p.s.ntf;
@@ -494,12 +559,20 @@ __global__ void __launch_bounds__(kBlock) k_bricks_exact_map(LearnedParams p, Br
   __shared__ ItemQueue queue;
   queue_init(&queue);
   for (;;) {
-    const int item = next_item(f.queue + 4 * kQueueSlots, &queue, f.n_map_items, 1, f.xcd_ranges);
-    if (item >= f.n_map_items) break;
-    const int b = item_brick(f, item);
-    if (!redo[b]) continue;
+    const int item = __builtin_amdgcn_readfirstlane(next_item(f.queue + 4 * kQueueSlots, &queue, nitems, 1,
+                                                              f.xcd_ranges));
+    if (item >= nitems) break;
+    int lo = 0, hi = count - 1;  // the last listed brick whose first item is <= item
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (lw[3 + mid] <= item)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int b = lw[4 + f.nbricks + lo];
     const BrickDesc* bk = f.bricks + b;
-    const int64_t id = (int64_t)(item - f.map_item_off[b]) * kMapItem + threadIdx.x;
+    const int64_t id = (int64_t)(item - lw[3 + lo]) * kMapItem + threadIdx.x;
     if (id >= bk->nverts) continue;
     const MapParams m = whole_volume_march(load_uniform(&bk->m), p.m);
     int64_t taken = 0;
@@ -521,23 +594,34 @@ hipError_t launch_bricks_learned_map(const LearnedParams& p, const BrickFrame& f
 }
 
 hipError_t launch_bricks_learned_check(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
-                                       float* err, hipStream_t st)
+                                       unsigned int* err, hipStream_t st)
 {
+  hipError_t e = launch_zero_u32(err, f.nbricks, st);
+  if (e != hipSuccess) return e;
+  // about 2048 workgroups over the bricks (eight per CU)
+  const int parts = std::max(1, std::min(64, 2048 / std::max(1, f.nbricks)));
   const size_t lds = sizeof(float) * (size_t)(p.s.ntf + 2 * (p.g.n[0] + p.g.n[1] + p.g.n[2]));
   BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(map_dtype, {
-    hipLaunchKernelGGL((k_bricks_learned_check<FDT, MDT>), dim3((unsigned)f.nbricks), dim3(kBlock), lds,
-                       st, p, f, err);
+    hipLaunchKernelGGL((k_bricks_learned_check<FDT, MDT>), dim3((unsigned)(f.nbricks * parts)), dim3(kBlock),
+                       lds, st, p, f, err, (int32_t)parts);
   }));
   return hipGetLastError();
 }
 
+hipError_t launch_learned_redo(const BrickFrame& f, const unsigned int* err, float budget, int32_t* list,
+                               hipStream_t st)
+{
+  hipLaunchKernelGGL(k_learned_redo, dim3(1), dim3(kBlock), 0, st, f, err, budget, list);
+  return hipGetLastError();
+}
+
 hipError_t launch_bricks_exact_map(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
-                                   const int32_t* redo, hipStream_t st)
+                                   const int32_t* list, hipStream_t st)
 {
   const size_t lds = sizeof(float) * (size_t)(p.s.ntf + 2 * (p.g.n[0] + p.g.n[1] + p.g.n[2]));
   const dim3 grid((unsigned)std::max(1, std::min(f.n_map_items, f.persistent_groups)));
   BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(map_dtype, {
-    hipLaunchKernelGGL((k_bricks_exact_map<FDT, MDT>), grid, dim3(kBlock), lds, st, p, f, redo);
+    hipLaunchKernelGGL((k_bricks_exact_map<FDT, MDT>), grid, dim3(kBlock), lds, st, p, f, list);
   }));
   return hipGetLastError();
 }

@@ -54,14 +54,23 @@ hipError_t launch_learned_samples(const LearnedParams& p, const MapParams& em, i
 int64_t learned_check_count(const MapParams& m, int32_t stride);
 
 // Bricks: p describes the whole volume (field, coarse grids, light, map step); each brick's
-// map_final receives its learned map, err[b] its check, and the fallback re-marches the
-// bricks flagged in redo through the whole volume.
+// map_final receives its learned map and err[b] (float bits, zeroed by the launcher) its
+// check.  The fallback's work list is built on the device from the checks (launch_learned_redo,
+// no host round trip), and the fallback re-marches the listed bricks through the whole volume.
+//
+// Work list layout (int32, learned_redo_words(nb) words): [0] the largest check error (float
+// bits), [1] the number of bricks over the budget, [2] their map items, [3, 4 + count) the first
+// item of each listed brick in a contiguous renumbering (count + 1 entries), [4 + nb, 4 + nb +
+// count) the listed bricks in ascending order.
+inline size_t learned_redo_words(int nb) { return 4 + 2 * (size_t)nb; }
 hipError_t launch_bricks_learned_map(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
                                      hipStream_t st);
 hipError_t launch_bricks_learned_check(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
-                                       float* err, hipStream_t st);
+                                       unsigned int* err, hipStream_t st);
+hipError_t launch_learned_redo(const BrickFrame& f, const unsigned int* err, float budget, int32_t* list,
+                               hipStream_t st);
 hipError_t launch_bricks_exact_map(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
-                                   const int32_t* redo, hipStream_t st);
+                                   const int32_t* list, hipStream_t st);
 
 hipError_t launch_mlp_infer(const float* params, const float* x, int64_t n, float* y, hipStream_t st);
 hipError_t launch_mlp_grad(const float* params, const float* x, const float* t, int64_t n,

@@ -15,6 +15,7 @@ template <int FDT, int MDT, bool BINS>
 __global__ void __launch_bounds__(kBlock) k_tmap_ref(GridParams g, MapParams m, ShadeParams s,
                                                      const float* seed, int64_t nv)
 {
+  if (gate_closed(m)) return;
   extern __shared__ float lds[];
   float* a_tab = lds;          // alpha / max alpha per TF entry
   float* xs = lds + s.ntf;     // coordinates

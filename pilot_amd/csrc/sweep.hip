@@ -53,6 +53,7 @@ __device__ __forceinline__ float field_at(const GridParams& g, const float* cons
 template <int FDT, int MDT>
 __global__ void __launch_bounds__(256) k_sweep_slice(SweepParams sp, int32_t k, int32_t first)
 {
+  if (gate_closed(sp.m)) return;
   extern __shared__ float lds[];
   const GridParams& g = sp.g;
   const MapParams& m = sp.m;

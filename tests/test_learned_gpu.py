@@ -243,30 +243,113 @@ def test_single_brick_learned_equals_one_block(budget):
     assert s0[0] == s1[0] and s0[2] == s1[2] == (1 if budget == 0.0 else 0)
 
 
+def _check_index(mp, stride):
+    """The check vertices along one axis: every stride-th vertex and the last."""
+    return np.unique(np.r_[np.arange(0, mp, stride), mp - 1])
+
+
+def _oracle_bricks_image(O, sc, vols, maps):
+    """The oracle's sampler over each brick with the given maps, then its global composite."""
+    o, d = B.camera_rays(sc.camera, sc.width, sc.height)
+    partial, bounds = [], []
+    for v, m in zip(vols, maps):
+        rb = O.RayBuf(o, d, "f32")
+        O.sample_rays(v, m, rb)
+        partial.append(rb.rgba)
+        bounds += [float(v.coords[a][k]) for a in range(3) for k in (0, -1)]
+    return O.composite(np.array(bounds), np.asarray(sc.camera.eye, np.float64), partial, "f32")
+
+
 def test_bricks_learned_fallback_per_brick(oracle_mod):
-    """Many bricks: a zero budget re-marches every brick through the whole volume, a large one
-    none; the fallback frame stays close to the reference multi-rank frame."""
+    """Many bricks, each judged by its own check.  The learned bricks approximate the
+    single-block map at their vertices (the whole volume's light march, step |size| / 128), and
+    that march is their fallback, bit-equal to the oracle's march to the same points.  The
+    fallback's work list is built on the device: a budget between the bricks' check errors sends
+    exactly the bricks over it to the fallback and leaves the others' learned maps; a negative
+    budget sends every brick, and the frame over those maps equals the oracle's sampler and
+    composite.  The mixed frame is one captured graph whose replays equal the eager frame."""
+    O = oracle_mod
     sc = scenes.make_scene("lbricks", 49, seed=81, nblobs=12, width=96, height=72, light="point",
                            map_cells=(48, 48, 48))
     ranges = B.kd_bricks(sc.dims, 8, 12, 8, 32)
+    nb = len(ranges)
+    mps = ranges[:, 1::2] - ranges[:, 0::2] + 1
+    nverts = np.prod(mps, axis=1)
+    whole = O.volume_from_scene(sc, "f32")
+    vols = O.brick_volumes(sc, ranges)
+    exact = [O.march_points(whole, O.map_points(v)) for v in vols]
+    stride = 4
+
+    def run(net, budget, graph=False):
+        with B.LightedVolumeRenderer() as r:
+            scenes.configure(r, sc)
+            r.SetShadow(True, sc.map_cells, B.F32, mode=B.SHADOW_LEARNED)
+            r.SetLearned(net, fallback=B.SHADOW_REF_MARCH, error_budget=budget, check_stride=stride)
+            r.SetBricks(ranges, ghost=1)
+            r.SetGraph(graph)
+            print(f"[learned bricks] budget {budget} graph {graph}", flush=True)
+            imgs = [r.RenderImage(sc.camera, sc.width, sc.height)[0] for _ in range(3 if graph else 1)]
+            maps = [r.GetBrickMap(b, int(nverts[b]), final=True) for b in range(nb)]
+            return imgs, maps, r.GetLearnedStats()
+
+    def check_err(maps, b):
+        idx = np.ix_(*[_check_index(mps[b][a], stride) for a in (2, 1, 0)])
+        shape = tuple(mps[b][::-1])
+        return np.abs(maps[b].reshape(shape)[idx] - exact[b].reshape(shape)[idx]).max()
+
+    with B.ShadowMLP(seed=9) as net:
+        net.set_params(_random_params(91) * 0.2)
+        (img_l,), learned, st_l = run(net, 1.0)
+        errs = np.array([check_err(learned, b) for b in range(nb)])
+        assert st_l[0] == errs.max() and st_l[2] == 0
+        (img_x,), forced, st_x = run(net, -1.0)
+        assert st_x[2] == nb and st_x[0] == errs.max()
+        for b in range(nb):
+            assert np.array_equal(forced[b], exact[b]), f"brick {b}: fallback map"
+        oimg = _oracle_bricks_image(O, sc, vols, exact)
+        assert np.array_equal(img_x.reshape(-1, 4), oimg)
+        ordered = np.unique(errs)
+        assert len(ordered) >= 2
+        mid = float(ordered[len(ordered) // 2 - 1] + ordered[len(ordered) // 2]) / 2
+        (img_e,), mixed_e, st_e = run(net, mid)
+        imgs_g, mixed, st_m = run(net, mid, graph=True)
+    over = errs > np.float32(mid)
+    assert 0 < over.sum() < nb and st_m[2] == st_e[2] == over.sum()
+    for b in range(nb):
+        assert np.array_equal(mixed[b], exact[b] if over[b] else learned[b]), f"brick {b}: mixed"
+        assert np.array_equal(mixed_e[b], mixed[b])
+    for img in imgs_g:
+        assert np.array_equal(img, img_e)
     with B.LightedVolumeRenderer() as r:
         scenes.configure(r, sc)
         r.SetBricks(ranges, ghost=1)
         ref, _ = r.RenderImage(sc.camera, sc.width, sc.height)
-    with B.ShadowMLP(seed=9) as net:
-        res = {}
-        for budget in (1.0, 0.0):
+    err = np.abs(img_x - ref)
+    print(f"single-block map at the bricks vs the multi-rank exchange: mean {err.mean():.2e} "
+          f"max {err.max():.2e}; learned vs its fallback: max {np.abs(img_l - img_x).max():.2e}")
+
+
+@pytest.mark.parametrize("budget", [1.0, 0.0])
+def test_learned_frame_is_one_graph(oracle_mod, budget):
+    """Single block: the learned map, its check and the gated fallback are part of the frame's
+    graph; its replays equal the eager frame, and the zero budget's fallback map is the oracle's."""
+    O = oracle_mod
+    sc = _scene(63, light="point")
+    with B.ShadowMLP(seed=6) as net:
+        net.set_params(_random_params(64) * 0.2)
+        out = {}
+        for graph in (False, True):
             with B.LightedVolumeRenderer() as r:
                 scenes.configure(r, sc)
                 r.SetShadow(True, sc.map_cells, B.F32, mode=B.SHADOW_LEARNED)
-                r.SetLearned(net, fallback=B.SHADOW_REF_MARCH, error_budget=budget, check_stride=4)
-                r.SetBricks(ranges, ghost=1)
-                img, _ = r.RenderImage(sc.camera, sc.width, sc.height)
-                res[budget] = (img, r.GetLearnedStats(), r.GetTimings())
-    img1, st1, t1 = res[1.0]
-    img0, st0, t0 = res[0.0]
-    assert st1[2] == 0 and t1["remap"] == 0.0
-    assert st0[2] == len(ranges) and t0["remap"] > 0.0
-    err = np.abs(img0 - ref)
-    print(f"whole-volume fallback vs multi-rank exchange: mean {err.mean():.2e} max {err.max():.2e}")
-    assert err.mean() < 2e-3
+                r.SetLearned(net, fallback=B.SHADOW_REF_MARCH, error_budget=budget, check_stride=8)
+                r.SetGraph(graph)
+                imgs = [r.RenderImage(sc.camera, sc.width, sc.height)[0] for _ in range(3 if graph else 1)]
+                out[graph] = (imgs, r.GetMap(sc.map_cells), r.GetLearnedStats())
+    (e_imgs, e_map, e_st), (g_imgs, g_map, g_st) = out[False], out[True]
+    assert e_st == g_st and np.array_equal(e_map, g_map)
+    for img in g_imgs:
+        assert np.array_equal(img, e_imgs[0])
+    assert e_st[2] == (1 if budget == 0.0 else 0)
+    if budget == 0.0:
+        assert np.array_equal(g_map, O.tmap_build(O.volume_from_scene(sc, "f32")))
