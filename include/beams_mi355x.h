@@ -322,8 +322,10 @@ int beams_mlp_train(beams_mlp* h, const float* x, const float* t, int64_t n, int
  * device (the fallback's launches return at once when the check is within the budget), so a
  * learned frame is captured in one graph like the exact ones.  The learned map approximates
  * the single-block map (the whole volume's light march, step |size| / 128).  With bricks that
- * is the single-block map at each brick's vertices, checked and rebuilt per brick: no
- * non-local exchange and no per-brick light step, unlike the exact brick frame.  The renderer
+ * is the single-block map at each brick's vertices (no non-local exchange and no per-brick
+ * light step, unlike the exact brick frame), checked and rebuilt tile by tile: the check
+ * vertices cut each brick's map into tiles of up to check_stride^3 cells, and a tile is
+ * re-marched when the error at its own or a neighbouring tile's corners exceeds the budget.  The renderer
  * keeps the pointer: destroy the renderer or attach another network before destroying net. */
 int beams_set_learned(beams_renderer* h, const beams_mlp* net, int32_t fallback_mode,
                       float error_budget, int32_t check_stride);
@@ -334,9 +336,8 @@ int beams_learned_samples(beams_renderer* h, int32_t mode, int64_t n, uint64_t s
                           float* features, float* targets, int32_t mem);
 /* The last learned map build (read back from the device: waits for the stream): err[0] =
  * largest |learned - exact| opacity at the check vertices, err[1] = the same for the coarse
- * optical depth alone (single block only, else -1); *fell_back = the number of maps the
- * fallback built (0 or 1 for the single block; with bricks, the bricks over the budget,
- * whose maps the whole volume's light march rebuilds).  -1, -1 and 0 before any. */
+ * optical depth alone (single block only, else -1); *fell_back = what the fallback
+ * rebuilt (0 or 1 map for the single block; with bricks, the number of tiles).  -1, -1 and 0 before any. */
 int beams_get_learned_stats(beams_renderer* h, float err[2], int32_t* fell_back);
 
 #ifdef __cplusplus

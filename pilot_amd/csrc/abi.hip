@@ -107,7 +107,7 @@ struct beams_renderer {
   DevBuf coarse;  // coarse extinction and optical depth, (kCoarseCells + 1)^3 each
   DevBuf lerr;    // check errors as float bits
   // the last learned build, whose statistics are read back on request: 0 none, 1 single block
-  // (lerr holds its two check errors), 2 bricks (bredo holds the fallback's work list)
+  // (lerr holds its two check errors), 2 bricks (lb_list holds the fallback's tile list)
   int32_t learned_kind = 0;
   float built_budget = 0.0f;
   // march
@@ -139,7 +139,11 @@ struct beams_renderer {
   std::vector<Brick> brick_pool;  // storage of the decomposition a new field replaced
   int32_t ghost = 0;
   DevBuf bmap_local, bmap_final, bseeds, btables, bpartial, bqueue;
-  DevBuf bredo;  // learned bricks: the fallback's work list (learned_redo_words)
+  // learned bricks: check-slot and tile offsets (host copy and device table), per-check-vertex
+  // errors, and the fallback's tile list
+  std::vector<int32_t> lb_host;
+  DevBuf lb_tab, lb_err, lb_list;
+  LearnedBricks lb{};
   std::vector<unsigned char> btab_host;  // tables of the last frame (reused while unchanged)
   int32_t n_map_items = 0, n_px_items = 0, max_coords = 0;
   int32_t persistent_groups = 2048;
@@ -840,12 +844,45 @@ int run_frame(beams_renderer* h, const std::vector<unsigned char>& key, Issue&& 
 // light march (the single-block map at the brick's vertices), the fallback's work list built on
 // the device from the checks, and that march for every vertex of the listed bricks.  The whole
 // field stays resident beside the bricks' ghosted copies.
-int prepare_bricks_learned(beams_renderer* h, int nb)
+int prepare_bricks_learned(beams_renderer* h, const BrickFrame& f)
 {
+  const int nb = f.nbricks;
   const size_t nc = (size_t)(kCoarseCells + 1) * (kCoarseCells + 1) * (kCoarseCells + 1);
   HIP_TRY(h, h->coarse.ensure(2 * nc * sizeof(float) + 6 * (kCoarseCells + 1) * sizeof(int32_t)));
-  HIP_TRY(h, h->lerr.ensure(sizeof(unsigned int) * (size_t)nb));
-  HIP_TRY(h, h->bredo.ensure(sizeof(int32_t) * learned_redo_words(nb)));
+  std::vector<int32_t> tab(2 * ((size_t)nb + 1), 0);
+  int64_t slots = 0, tiles = 0;
+  for (int b = 0; b < nb; ++b) {
+    const auto& br = h->bricks[(size_t)b];
+    int64_t n = 1, t = 1;
+    for (int d = 0; d < 3; ++d) {
+      const int64_t c = check_count(br.cells[d], h->check_stride);
+      n *= c;
+      t *= c - 1;
+    }
+    slots += (n + 255) / 256 * 256;
+    tiles += t;
+    if (slots > INT32_MAX || tiles > INT32_MAX - 2)
+      return h->fail(BEAMS_ERR_UNSUPPORTED, "learned bricks: more than 2^31 check vertices or tiles");
+    tab[(size_t)b + 1] = (int32_t)slots;
+    tab[(size_t)nb + 2 + (size_t)b] = (int32_t)tiles;
+  }
+  if (tab != h->lb_host) {
+    HIP_TRY(h, h->lb_tab.ensure(sizeof(int32_t) * tab.size()));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));  // a replay in flight may read the table
+    HIP_TRY(h, hipMemcpy(h->lb_tab.p, tab.data(), sizeof(int32_t) * tab.size(), hipMemcpyHostToDevice));
+    h->lb_host = tab;
+  }
+  HIP_TRY(h, h->lb_err.ensure(sizeof(float) * (size_t)slots));
+  HIP_TRY(h, h->lb_list.ensure(sizeof(int32_t) * (2 + (size_t)tiles)));
+  LearnedBricks& lb = h->lb;
+  lb.chk_off = (const int32_t*)h->lb_tab.p;
+  lb.tile_off = lb.chk_off + nb + 1;
+  lb.verr = (float*)h->lb_err.p;
+  lb.list = (int32_t*)h->lb_list.p;
+  lb.n_chk_items = (int32_t)(slots / 256);
+  lb.n_tiles = (int32_t)tiles;
+  lb.stride = h->check_stride;
+  lb.budget = h->error_budget;
   return BEAMS_OK;
 }
 
@@ -866,12 +903,10 @@ int enqueue_bricks_learned(beams_renderer* h, const BrickFrame& f, const Learned
   if ((rc = mark(BEAMS_PHASE_MAP, false))) return rc;
   HIP_TRY(h, launch_coarse(p, h->stream));
   HIP_TRY(h, launch_bricks_learned_map(p, f, h->map_dtype, h->stream));
-  HIP_TRY(h, launch_bricks_learned_check(p, f, h->map_dtype, (unsigned int*)h->lerr.p, h->stream));
-  HIP_TRY(h, launch_learned_redo(f, (const unsigned int*)h->lerr.p, h->error_budget, (int32_t*)h->bredo.p,
-                                 h->stream));
+  HIP_TRY(h, launch_bricks_learned_check(p, f, h->map_dtype, h->lb, h->stream));
   if ((rc = mark(BEAMS_PHASE_MAP, true))) return rc;
   if ((rc = mark(BEAMS_PHASE_REMAP, false))) return rc;
-  HIP_TRY(h, launch_bricks_exact_map(p, f, h->map_dtype, (const int32_t*)h->bredo.p, h->stream));
+  HIP_TRY(h, launch_bricks_exact_tiles(p, f, h->map_dtype, h->lb, h->stream));
   return mark(BEAMS_PHASE_REMAP, true);
 }
 
@@ -911,7 +946,7 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
   for (int k = 0; k < 4; ++k) o.bg[k] = background ? background[k] : 0.0f;
   LearnedParams lp{};
   if (learned) {
-    if ((rc = prepare_bricks_learned(h, f.nbricks))) return rc;
+    if ((rc = prepare_bricks_learned(h, f))) return rc;
     lp = bricks_learned_params(h);
   }
   auto issue = [&](bool timed) -> int {
@@ -942,7 +977,7 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
   KeyBuilder key;
   key.put(1).put(f).put(c).put(o).put(h->btab_version).put(shadows).put(learned).put(h->dtype)
      .put(h->map_dtype).put(ntf);
-  if (learned) key.put(lp).put(h->error_budget);
+  if (learned) key.put(lp).put(h->lb);
   if ((rc = run_frame(h, key.bytes, issue))) return rc;
   if (learned) learned_built(h, 2);
   if (shadows) h->map_valid = true;
@@ -1000,6 +1035,9 @@ int beams_destroy(beams_renderer* h)
   h->coords.release();
   h->tf.release();
   h->map.release();
+  h->sweep_slices.release();
+  h->coarse.release();
+  h->lerr.release();
   h->rays.release();
   h->image.release();
   for (auto& f : h->frames)
@@ -1016,6 +1054,9 @@ int beams_destroy(beams_renderer* h)
   h->bpartial.release();
   h->bqueue.release();
   h->brows.release();
+  h->lb_tab.release();
+  h->lb_err.release();
+  h->lb_list.release();
   (void)hipStreamDestroy(h->stream);
   delete h;
   return BEAMS_OK;
@@ -1720,13 +1761,14 @@ int beams_get_learned_stats(beams_renderer* h, float err[2], int32_t* fell_back)
   if (h->learned_kind) {
     uint32_t w[2] = { 0, 0 };
     HIP_TRY(h, hipStreamSynchronize(h->stream));
-    HIP_TRY(h, hipMemcpy(w, h->learned_kind == 1 ? h->lerr.p : h->bredo.p, sizeof(w), hipMemcpyDeviceToHost));
-    e[0] = bits_f32(w[0]);
+    HIP_TRY(h, hipMemcpy(w, h->learned_kind == 1 ? h->lerr.p : h->lb_list.p, sizeof(w), hipMemcpyDeviceToHost));
     if (h->learned_kind == 1) {
+      e[0] = bits_f32(w[0]);
       e[1] = bits_f32(w[1]);
       fb = !(e[0] <= h->built_budget);
     } else {
-      fb = (int32_t)w[1];
+      e[0] = bits_f32(w[1]);
+      fb = (int32_t)w[0];
     }
   }
   if (err) {

@@ -336,10 +336,6 @@ __global__ void __launch_bounds__(kBlock) k_learned_map(LearnedParams p, int32_t
 }
 
 // ---- the sparse exact check ----------------------------------------------------------------
-__device__ __forceinline__ int64_t check_axis(int64_t cells, int32_t stride)
-{
-  return cells / stride + 1 + (cells % stride != 0 ? 1 : 0);
-}
 
 template <int FDT, int MDT>
 __global__ void __launch_bounds__(kBlock) k_learned_check(LearnedParams p, int64_t ncheck)
@@ -354,8 +350,8 @@ __global__ void __launch_bounds__(kBlock) k_learned_check(LearnedParams p, int64
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= ncheck) return;
   const MapParams& m = p.m;
-  const int64_t n0 = check_axis(m.mp[0] - 1, p.check_stride);
-  const int64_t n1 = check_axis(m.mp[1] - 1, p.check_stride);
+  const int64_t n0 = check_count(m.mp[0] - 1, p.check_stride);
+  const int64_t n1 = check_count(m.mp[1] - 1, p.check_stride);
   const int64_t qi[3] = { q % n0, (q / n0) % n1, q / (n0 * n1) };
   int64_t vi[3];
 #pragma unroll
@@ -437,117 +433,127 @@ __device__ __forceinline__ MapParams whole_volume_march(const MapParams& brick_m
   return m;
 }
 
-// Per brick: largest |learned - exact| over its check vertices, the exact value being the
-// whole volume's light march (the single-block reference march).  `parts` workgroups share a
-// brick, each taking every parts-th block of 256 check vertices, and fold their maxima into
-// err[b] (non-negative floats order as their bits; a NaN's bits order above every number).
+// The brick of slot `v` in a table of nb + 1 ascending offsets: the last b with off[b] <= v.
+__device__ __forceinline__ int brick_of(const int32_t* off, int nb, int v)
+{
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= v)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Per check vertex: |learned - exact|, the exact value being the whole volume's light march
+// (the single-block reference march) to the brick's vertex.  One item = 256 check slots of one
+// brick (each brick's slots are padded to whole items); the largest error over all bricks goes
+// to list[1] (non-negative floats order as their bits, a NaN's bits above every number).
 template <int FDT, int MDT>
-__global__ void __launch_bounds__(kBlock) k_bricks_learned_check(LearnedParams p, BrickFrame f, unsigned int* err,
-                                                                 int32_t parts)
+__global__ void __launch_bounds__(kBlock) k_bricks_learned_check(LearnedParams p, BrickFrame f, LearnedBricks lb)
 {
   extern __shared__ float lds[];
   float* a_tab = lds;
   float* xs = lds + p.s.ntf;
-  __shared__ float red[kBlock / 64];
+  __shared__ unsigned red[kBlock / 64];
   stage_alpha(p.s, a_tab);
   stage_coords(p.g, xs);
   __syncthreads();
   const float* x[3] = { xs, xs + p.g.n[0], xs + p.g.n[0] + p.g.n[1] };
-  const int b = (int)blockIdx.x / parts, part = (int)blockIdx.x % parts;
-  const BrickDesc& d = f.bricks[b];
-  const MapParams m = whole_volume_march(d.m, p.m);
-  const int64_t n0 = check_axis(m.mp[0] - 1, p.check_stride);
-  const int64_t n1 = check_axis(m.mp[1] - 1, p.check_stride);
-  const int64_t n2 = check_axis(m.mp[2] - 1, p.check_stride);
-  float worst = 0.0f;
-  for (int64_t q = (int64_t)part * kBlock + threadIdx.x; q < n0 * n1 * n2; q += (int64_t)parts * kBlock) {
-    const int64_t qi[3] = { q % n0, (q / n0) % n1, q / (n0 * n1) };
-    int64_t vi[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const int64_t i = qi[a] * p.check_stride;
-      vi[a] = i < m.mp[a] - 1 ? i : m.mp[a] - 1;
+  typedef const __attribute__((address_space(4))) int32_t ConstInt;
+  ConstInt* coff = (ConstInt*)(const void*)lb.chk_off;
+  unsigned worst = 0;
+  for (int item = blockIdx.x; item < lb.n_chk_items; item += gridDim.x) {
+    const int slot0 = item * kBlock;
+    int lo = 0, hi = f.nbricks - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (coff[mid] <= slot0)
+        lo = mid;
+      else
+        hi = mid - 1;
     }
-    const int64_t id = vi[0] + m.mp[0] * (vi[1] + m.mp[1] * vi[2]);
-    int64_t taken = 0;
-    const float exact = march_vertex<FDT>(p.g, m, p.s, a_tab, x, id, 0.0f, &taken);
-    const float e = fabsf(load_t<MDT>(d.map_final, id) - exact);
-    worst = (e > worst || e != e) ? e : worst;
-  }
-  unsigned wb = f32_bits(worst);
+    const int b = __builtin_amdgcn_readfirstlane(lo);
+    const BrickDesc* bk = f.bricks + b;
+    const MapParams m = whole_volume_march(load_uniform(&bk->m), p.m);
+    const int64_t n0 = check_count(m.mp[0] - 1, lb.stride), n1 = check_count(m.mp[1] - 1, lb.stride);
+    const int64_t n2 = check_count(m.mp[2] - 1, lb.stride);
+    const int64_t q = (int64_t)(slot0 - coff[b]) + threadIdx.x;
+    if (q < n0 * n1 * n2) {
+      const int64_t qi[3] = { q % n0, (q / n0) % n1, q / (n0 * n1) };
+      int64_t vi[3];
 #pragma unroll
-  for (int mk = 32; mk >= 1; mk >>= 1) wb = max(wb, (unsigned)__shfl_xor((int)wb, mk, 64));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = bits_f32(wb);
+      for (int a = 0; a < 3; ++a) {
+        const int64_t i = qi[a] * lb.stride;
+        vi[a] = i < m.mp[a] - 1 ? i : m.mp[a] - 1;
+      }
+      const int64_t id = vi[0] + m.mp[0] * (vi[1] + m.mp[1] * vi[2]);
+      int64_t taken = 0;
+      const float exact = march_vertex<FDT>(p.g, m, p.s, a_tab, x, id, 0.0f, &taken);
+      const float e = fabsf(load_t<MDT>(bk->map_final, id) - exact);
+      lb.verr[slot0 + threadIdx.x] = e;
+      worst = max(worst, f32_bits(e));
+    }
+  }
+#pragma unroll
+  for (int mk = 32; mk >= 1; mk >>= 1) worst = max(worst, (unsigned)__shfl_xor((int)worst, mk, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = worst;
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned w = f32_bits(red[0]);
-    for (int i = 1; i < kBlock / 64; ++i) w = max(w, f32_bits(red[i]));
-    atomicMax(&err[b], w);
+    unsigned w = red[0];
+    for (int i = 1; i < kBlock / 64; ++i) w = max(w, red[i]);
+    atomicMax((unsigned*)&lb.list[1], w);
   }
 }
 
-// The fallback's work list from the per-brick checks (one workgroup; each thread takes a run
-// of consecutive bricks, and a scan over the threads' counts places every run's entries).
-__global__ void __launch_bounds__(kBlock) k_learned_redo(BrickFrame f, const unsigned int* err, float budget,
-                                                         int32_t* list)
+// One thread per tile: the largest error over the check vertices of the tile and of its 26
+// neighbours (the 4 x 4 x 4 check vertices from one before its lower corner to one past its
+// upper corner, clamped to the brick) against the budget: an error inside a tile whose own
+// corners pass shows, most often, at a neighbour's corners.  The tiles over the budget are
+// appended to the list (one atomic per wave).
+__global__ void __launch_bounds__(kBlock) k_learned_tiles(BrickFrame f, LearnedBricks lb)
 {
-  __shared__ int32_t cnt[kBlock + 1], items[kBlock + 1];
-  __shared__ unsigned worst[kBlock];
-  const int nb = f.nbricks, t = (int)threadIdx.x;
-  const int per = (nb + kBlock - 1) / kBlock;
-  const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
-  int c = 0, it = 0;
-  unsigned w = 0;
-  for (int b = b0; b < b1; ++b) {
-    const unsigned e = err[b];
-    w = max(w, e);
-    if (!(bits_f32(e) <= budget)) {
-      ++c;
-      it += f.map_item_off[b + 1] - f.map_item_off[b];
+  const int g = (int)(blockIdx.x * kBlock + threadIdx.x);
+  bool fail = false;
+  if (g < lb.n_tiles) {
+    const int b = brick_of(lb.tile_off, f.nbricks, g);
+    const int64_t* mp = f.bricks[b].m.mp;
+    const int n0 = (int)check_count(mp[0] - 1, lb.stride), n1 = (int)check_count(mp[1] - 1, lb.stride);
+    const int t = g - lb.tile_off[b];
+    const int ti = t % (n0 - 1), tj = (t / (n0 - 1)) % (n1 - 1), tk = t / ((n0 - 1) * (n1 - 1));
+    const float* e = lb.verr + lb.chk_off[b];
+    const int n2 = (int)check_count(mp[2] - 1, lb.stride);
+    unsigned w = 0;
+    for (int c = 0; c < 64; ++c) {
+      const int ci = min(max(ti - 1 + (c & 3), 0), n0 - 1), cj = min(max(tj - 1 + ((c >> 2) & 3), 0), n1 - 1);
+      const int ck = min(max(tk - 1 + (c >> 4), 0), n2 - 1);
+      w = max(w, f32_bits(e[ci + n0 * (cj + n1 * ck)]));
     }
+    fail = !(bits_f32(w) <= lb.budget);
   }
-  cnt[t] = c;
-  items[t] = it;
-  worst[t] = w;
-  __syncthreads();
-  if (t == 0) {
-    int32_t sc = 0, si = 0;
-    unsigned sw = 0;
-    for (int i = 0; i < kBlock; ++i) {
-      const int32_t ci = cnt[i], ii = items[i];
-      cnt[i] = sc;
-      items[i] = si;
-      sc += ci;
-      si += ii;
-      sw = max(sw, worst[i]);
-    }
-    cnt[kBlock] = sc;
-    items[kBlock] = si;
-    list[0] = (int32_t)sw;
-    list[1] = sc;
-    list[2] = si;
-    list[3 + sc] = si;
-  }
-  __syncthreads();
-  int k = cnt[t], o = items[t];
-  for (int b = b0; b < b1; ++b) {
-    if (!(bits_f32(err[b]) <= budget)) {
-      list[3 + k] = o;
-      list[4 + nb + k] = b;
-      ++k;
-      o += f.map_item_off[b + 1] - f.map_item_off[b];
-    }
+  const unsigned long long mask = __ballot(fail);
+  if (mask) {
+    const int lane = (int)(threadIdx.x & 63), leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = (int)atomicAdd((unsigned*)&lb.list[0], (unsigned)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+    if (fail) lb.list[2 + base + rank] = g;
   }
 }
 
-// Fallback: the whole volume's light march for every vertex of the listed bricks.  The list
-// was written by an earlier kernel, so its words are read through the scalar cache.
+// Fallback: the whole volume's light march for every vertex of the listed tiles (one tile per
+// work item).  The list was written by earlier kernels, so its words go through the scalar cache.
 template <int FDT, int MDT>
-__global__ void __launch_bounds__(kBlock) k_bricks_exact_map(LearnedParams p, BrickFrame f, const int32_t* list)
+__global__ void __launch_bounds__(kBlock) k_bricks_exact_tiles(LearnedParams p, BrickFrame f, LearnedBricks lb)
 {
   typedef const __attribute__((address_space(4))) int32_t ConstInt;
-  ConstInt* lw = (ConstInt*)(const void*)list;
-  const int count = lw[1], nitems = lw[2];
+  ConstInt* lw = (ConstInt*)(const void*)lb.list;
+  ConstInt* toff = (ConstInt*)(const void*)lb.tile_off;
+  const int nitems = lw[0];
   if (nitems == 0) return;
   extern __shared__ float lds[];
   float* a_tab = lds;
@@ -558,25 +564,37 @@ __global__ void __launch_bounds__(kBlock) k_bricks_exact_map(LearnedParams p, Br
   const float* x[3] = { xs, xs + p.g.n[0], xs + p.g.n[0] + p.g.n[1] };
   __shared__ ItemQueue queue;
   queue_init(&queue);
+  const int s = lb.stride;
   for (;;) {
     const int item = __builtin_amdgcn_readfirstlane(next_item(f.queue + 4 * kQueueSlots, &queue, nitems, 1,
                                                               f.xcd_ranges));
     if (item >= nitems) break;
-    int lo = 0, hi = count - 1;  // the last listed brick whose first item is <= item
+    const int g = lw[2 + item];
+    int lo = 0, hi = f.nbricks - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (lw[3 + mid] <= item)
+      if (toff[mid] <= g)
         lo = mid;
       else
         hi = mid - 1;
     }
-    const int b = lw[4 + f.nbricks + lo];
+    const int b = __builtin_amdgcn_readfirstlane(lo);
     const BrickDesc* bk = f.bricks + b;
-    const int64_t id = (int64_t)(item - lw[3 + lo]) * kMapItem + threadIdx.x;
-    if (id >= bk->nverts) continue;
     const MapParams m = whole_volume_march(load_uniform(&bk->m), p.m);
-    int64_t taken = 0;
-    store_t<MDT>(bk->map_final, id, march_vertex<FDT>(p.g, m, p.s, a_tab, x, id, 0.0f, &taken));
+    const int mp0 = (int)m.mp[0], mp1 = (int)m.mp[1], mp2 = (int)m.mp[2];
+    const int n0 = (int)check_count(mp0 - 1, s), n1 = (int)check_count(mp1 - 1, s);
+    const int n2 = (int)check_count(mp2 - 1, s);
+    const int t = g - toff[b];
+    const int ti = t % (n0 - 1), tj = (t / (n0 - 1)) % (n1 - 1), tk = t / ((n0 - 1) * (n1 - 1));
+    const int i0 = ti * s, j0 = tj * s, k0 = tk * s;
+    const int c0 = (ti == n0 - 2 ? mp0 : i0 + s) - i0, c1 = (tj == n1 - 2 ? mp1 : j0 + s) - j0;
+    const int c2 = (tk == n2 - 2 ? mp2 : k0 + s) - k0;
+    for (int v = threadIdx.x; v < c0 * c1 * c2; v += kBlock) {
+      const int i = i0 + v % c0, j = j0 + (v / c0) % c1, k = k0 + v / (c0 * c1);
+      const int64_t id = i + (int64_t)mp0 * (j + (int64_t)mp1 * k);
+      int64_t taken = 0;
+      store_t<MDT>(bk->map_final, id, march_vertex<FDT>(p.g, m, p.s, a_tab, x, id, 0.0f, &taken));
+    }
   }
 }
 
@@ -594,34 +612,28 @@ hipError_t launch_bricks_learned_map(const LearnedParams& p, const BrickFrame& f
 }
 
 hipError_t launch_bricks_learned_check(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
-                                       unsigned int* err, hipStream_t st)
+                                       const LearnedBricks& lb, hipStream_t st)
 {
-  hipError_t e = launch_zero_u32(err, f.nbricks, st);
+  hipError_t e = launch_zero_u32((unsigned int*)lb.list, 2, st);
   if (e != hipSuccess) return e;
-  // about 2048 workgroups over the bricks (eight per CU)
-  const int parts = std::max(1, std::min(64, 2048 / std::max(1, f.nbricks)));
   const size_t lds = sizeof(float) * (size_t)(p.s.ntf + 2 * (p.g.n[0] + p.g.n[1] + p.g.n[2]));
+  const dim3 grid((unsigned)std::max(1, std::min(lb.n_chk_items, 2048)));
   BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(map_dtype, {
-    hipLaunchKernelGGL((k_bricks_learned_check<FDT, MDT>), dim3((unsigned)(f.nbricks * parts)), dim3(kBlock),
-                       lds, st, p, f, err, (int32_t)parts);
+    hipLaunchKernelGGL((k_bricks_learned_check<FDT, MDT>), grid, dim3(kBlock), lds, st, p, f, lb);
   }));
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_learned_tiles, dim3((unsigned)((lb.n_tiles + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                     f, lb);
   return hipGetLastError();
 }
 
-hipError_t launch_learned_redo(const BrickFrame& f, const unsigned int* err, float budget, int32_t* list,
-                               hipStream_t st)
-{
-  hipLaunchKernelGGL(k_learned_redo, dim3(1), dim3(kBlock), 0, st, f, err, budget, list);
-  return hipGetLastError();
-}
-
-hipError_t launch_bricks_exact_map(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
-                                   const int32_t* list, hipStream_t st)
+hipError_t launch_bricks_exact_tiles(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
+                                     const LearnedBricks& lb, hipStream_t st)
 {
   const size_t lds = sizeof(float) * (size_t)(p.s.ntf + 2 * (p.g.n[0] + p.g.n[1] + p.g.n[2]));
-  const dim3 grid((unsigned)std::max(1, std::min(f.n_map_items, f.persistent_groups)));
+  const dim3 grid((unsigned)std::max(1, std::min(lb.n_tiles, f.persistent_groups)));
   BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(map_dtype, {
-    hipLaunchKernelGGL((k_bricks_exact_map<FDT, MDT>), grid, dim3(kBlock), lds, st, p, f, list);
+    hipLaunchKernelGGL((k_bricks_exact_tiles<FDT, MDT>), grid, dim3(kBlock), lds, st, p, f, lb);
   }));
   return hipGetLastError();
 }
@@ -629,10 +641,7 @@ hipError_t launch_bricks_exact_map(const LearnedParams& p, const BrickFrame& f, 
 int64_t learned_check_count(const MapParams& m, int32_t stride)
 {
   int64_t c = 1;
-  for (int d = 0; d < 3; ++d) {
-    const int64_t cells = m.mp[d] - 1;
-    c *= cells / stride + 1 + (cells % stride != 0 ? 1 : 0);
-  }
+  for (int d = 0; d < 3; ++d) c *= check_count(m.mp[d] - 1, stride);
   return c;
 }
 

@@ -53,24 +53,36 @@ hipError_t launch_learned_samples(const LearnedParams& p, const MapParams& em, i
                                   uint64_t seed, float* x, float* t, hipStream_t st);
 int64_t learned_check_count(const MapParams& m, int32_t stride);
 
-// Bricks: p describes the whole volume (field, coarse grids, light, map step); each brick's
-// map_final receives its learned map and err[b] (float bits, zeroed by the launcher) its
-// check.  The fallback's work list is built on the device from the checks (launch_learned_redo,
-// no host round trip), and the fallback re-marches the listed bricks through the whole volume.
-//
-// Work list layout (int32, learned_redo_words(nb) words): [0] the largest check error (float
-// bits), [1] the number of bricks over the budget, [2] their map items, [3, 4 + count) the first
-// item of each listed brick in a contiguous renumbering (count + 1 entries), [4 + nb, 4 + nb +
-// count) the listed bricks in ascending order.
-inline size_t learned_redo_words(int nb) { return 4 + 2 * (size_t)nb; }
+// Bricks: p describes the whole volume (field, coarse grids, light, map step), and each
+// brick's map_final receives its learned map.  The check vertices of a brick (every stride-th
+// map vertex per axis, and the last) cut its map into tiles of up to stride^3 cells: tile t
+// along an axis holds the vertices [t s, (t + 1) s) and the last tile also the last vertex.
+// The check writes |learned - exact| per check vertex; a tile is listed on the device when the
+// largest error over its own and its 26 neighbours' corners exceeds the budget (a NaN exceeds
+// it), and the fallback re-marches the listed tiles' vertices through the whole volume.  No host round trip: the launches
+// can be captured in one graph.
+struct LearnedBricks {
+  const int32_t* chk_off;   // nbricks + 1: first check slot of each brick (counts padded to 256)
+  const int32_t* tile_off;  // nbricks + 1: first tile of each brick
+  float* verr;              // per check slot: |learned - exact|
+  int32_t* list;            // [0] tiles listed, [1] largest check error (float bits), [2, ...) the tiles
+  int32_t n_chk_items;      // chk_off[nbricks] / 256
+  int32_t n_tiles;          // tile_off[nbricks]
+  int32_t stride;
+  float budget;
+};
+// Check vertices along an axis of `cells` cells: every stride-th vertex and the last.
+BEAMS_HD int64_t check_count(int64_t cells, int32_t stride)
+{
+  return cells / stride + 1 + (cells % stride != 0 ? 1 : 0);
+}
 hipError_t launch_bricks_learned_map(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
                                      hipStream_t st);
+// Zeroes the list's two counters, then the check and the tile listing.
 hipError_t launch_bricks_learned_check(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
-                                       unsigned int* err, hipStream_t st);
-hipError_t launch_learned_redo(const BrickFrame& f, const unsigned int* err, float budget, int32_t* list,
-                               hipStream_t st);
-hipError_t launch_bricks_exact_map(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
-                                   const int32_t* list, hipStream_t st);
+                                       const LearnedBricks& lb, hipStream_t st);
+hipError_t launch_bricks_exact_tiles(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
+                                     const LearnedBricks& lb, hipStream_t st);
 
 hipError_t launch_mlp_infer(const float* params, const float* x, int64_t n, float* y, hipStream_t st);
 hipError_t launch_mlp_grad(const float* params, const float* x, const float* t, int64_t n,

@@ -218,7 +218,7 @@ def test_learned_gain_over_coarse_only(oracle_mod):
     assert np.all(out[:, 0] < 1.25 * out[:, 1])
 
 
-@pytest.mark.parametrize("budget", [1.0, 0.0])
+@pytest.mark.parametrize("budget", [1.0, -1.0])
 def test_single_brick_learned_equals_one_block(budget):
     """One brick covering the field: the per-brick learned map (and its whole-volume fallback)
     equals the single-block learned map (and its reference-march fallback) bit for bit."""
@@ -240,12 +240,32 @@ def test_single_brick_learned_equals_one_block(budget):
     (i0, m0, s0), (i1, m1, s1) = out[False], out[True]
     assert np.array_equal(m0, m1)
     assert np.array_equal(i0, i1)
-    assert s0[0] == s1[0] and s0[2] == s1[2] == (1 if budget == 0.0 else 0)
+    # the single block falls back as a whole, the brick tile by tile (8^3 tiles of 4^3 cells)
+    assert s0[0] == s1[0] and s0[2] == (1 if budget < 0 else 0) and s1[2] == (512 if budget < 0 else 0)
 
 
 def _check_index(mp, stride):
     """The check vertices along one axis: every stride-th vertex and the last."""
     return np.unique(np.r_[np.arange(0, mp, stride), mp - 1])
+
+
+def _tile_errors(learned, exact, mp, stride):
+    """A brick's check tiles (z, y, x): the largest |learned - exact| over the check vertices
+    of each tile and its 26 neighbours (check indices t - 1 .. t + 2 per axis, clamped), and
+    the tile of every map vertex per axis (tile t holds the vertices [t s, (t + 1) s), the last
+    tile also the last vertex)."""
+    idx = [_check_index(mp[a], stride) for a in (2, 1, 0)]
+    shape = tuple(mp[::-1])
+    e = np.abs(learned.reshape(shape)[np.ix_(*idx)] - exact.reshape(shape)[np.ix_(*idx)])
+    ep = np.pad(e, 1, mode="edge")
+    nt = tuple(n - 1 for n in e.shape)
+    t = np.zeros(nt, e.dtype)
+    for dz in range(4):
+        for dy in range(4):
+            for dx in range(4):
+                t = np.maximum(t, ep[dz:dz + nt[0], dy:dy + nt[1], dx:dx + nt[2]])
+    tiles_of = [np.minimum(np.arange(mp[a]) // stride, len(i) - 2) for a, i in zip((2, 1, 0), idx)]
+    return e.max(), t, tiles_of
 
 
 def _oracle_bricks_image(O, sc, vols, maps):
@@ -260,14 +280,16 @@ def _oracle_bricks_image(O, sc, vols, maps):
     return O.composite(np.array(bounds), np.asarray(sc.camera.eye, np.float64), partial, "f32")
 
 
-def test_bricks_learned_fallback_per_brick(oracle_mod):
-    """Many bricks, each judged by its own check.  The learned bricks approximate the
-    single-block map at their vertices (the whole volume's light march, step |size| / 128), and
-    that march is their fallback, bit-equal to the oracle's march to the same points.  The
-    fallback's work list is built on the device: a budget between the bricks' check errors sends
-    exactly the bricks over it to the fallback and leaves the others' learned maps; a negative
-    budget sends every brick, and the frame over those maps equals the oracle's sampler and
-    composite.  The mixed frame is one captured graph whose replays equal the eager frame."""
+def test_bricks_learned_fallback_per_tile(oracle_mod):
+    """Many bricks, checked tile by tile.  The learned bricks approximate the single-block map
+    at their vertices (the whole volume's light march, step |size| / 128); that march is their
+    fallback, bit-equal to the oracle's march to the same points.  The check vertices (every
+    4th vertex per axis and the last) cut each brick into tiles, and the fallback's tile list is
+    built on the device: a budget between the tiles' errors (over their own and their
+    neighbours' corners) re-marches exactly the tiles over it and leaves the others' learned
+    values; a negative budget re-marches every tile, and
+    the frame over those maps equals the oracle's sampler and composite.  The mixed frame is one
+    captured graph whose replays equal the eager frame."""
     O = oracle_mod
     sc = scenes.make_scene("lbricks", 49, seed=81, nblobs=12, width=96, height=72, light="point",
                            map_cells=(48, 48, 48))
@@ -292,32 +314,33 @@ def test_bricks_learned_fallback_per_brick(oracle_mod):
             maps = [r.GetBrickMap(b, int(nverts[b]), final=True) for b in range(nb)]
             return imgs, maps, r.GetLearnedStats()
 
-    def check_err(maps, b):
-        idx = np.ix_(*[_check_index(mps[b][a], stride) for a in (2, 1, 0)])
-        shape = tuple(mps[b][::-1])
-        return np.abs(maps[b].reshape(shape)[idx] - exact[b].reshape(shape)[idx]).max()
-
     with B.ShadowMLP(seed=9) as net:
         net.set_params(_random_params(91) * 0.2)
         (img_l,), learned, st_l = run(net, 1.0)
-        errs = np.array([check_err(learned, b) for b in range(nb)])
-        assert st_l[0] == errs.max() and st_l[2] == 0
+        tiles = [_tile_errors(learned[b], exact[b], mps[b], stride) for b in range(nb)]
+        worst = max(t[0] for t in tiles)
+        ntiles = sum(t[1].size for t in tiles)
+        assert st_l[0] == worst and st_l[2] == 0
         (img_x,), forced, st_x = run(net, -1.0)
-        assert st_x[2] == nb and st_x[0] == errs.max()
+        assert st_x[2] == ntiles and st_x[0] == worst
         for b in range(nb):
             assert np.array_equal(forced[b], exact[b]), f"brick {b}: fallback map"
         oimg = _oracle_bricks_image(O, sc, vols, exact)
         assert np.array_equal(img_x.reshape(-1, 4), oimg)
-        ordered = np.unique(errs)
-        assert len(ordered) >= 2
+        ordered = np.unique(np.concatenate([t[1].ravel() for t in tiles]))
         mid = float(ordered[len(ordered) // 2 - 1] + ordered[len(ordered) // 2]) / 2
         (img_e,), mixed_e, st_e = run(net, mid)
         imgs_g, mixed, st_m = run(net, mid, graph=True)
-    over = errs > np.float32(mid)
-    assert 0 < over.sum() < nb and st_m[2] == st_e[2] == over.sum()
+    over = 0
     for b in range(nb):
-        assert np.array_equal(mixed[b], exact[b] if over[b] else learned[b]), f"brick {b}: mixed"
+        _, terr, (tz, ty, tx) = tiles[b]
+        fail = terr > np.float32(mid)
+        over += int(fail.sum())
+        vfail = fail[np.ix_(tz, ty, tx)].ravel()
+        want = np.where(vfail, exact[b], learned[b])
+        assert np.array_equal(mixed[b], want), f"brick {b}: mixed"
         assert np.array_equal(mixed_e[b], mixed[b])
+    assert 0 < over < ntiles and st_m[2] == st_e[2] == over
     for img in imgs_g:
         assert np.array_equal(img, img_e)
     with B.LightedVolumeRenderer() as r:
