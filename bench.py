@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--series-size", type=int, default=1024, help="points per axis of the time series")
     ap.add_argument("--series-image", type=int, default=2048, help="time-series image width = height")
     ap.add_argument("--cpu-runs", type=int, default=5, help="timed runs of each CPU mode (median)")
+    ap.add_argument("--schedule", default=None,
+                    help="time-series brick schedule as JSON (the knobs of SetSchedule): no trials or "
+                         "autotuning, so that separate profiling passes run the same kernels")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device 5 times (the byte-counter calibration run)")
     return ap.parse_args()
@@ -292,7 +295,7 @@ def series_bricks(n):
     return ranges, ranges[:, 1::2] - ranges[:, 0::2]  # a map cell per field cell
 
 
-def series_exact(steps, warmup, n, img, timesteps, keep_first=True):
+def series_exact(steps, warmup, n, img, timesteps, keep_first=True, schedule=None):
     """The exact-shadow frames of the time series.  One renderer per timestep holds that
     timestep's field in its bricks (inputs resident in HBM before timing); every frame rebuilds
     the maps (phase-1 local maps, the non-local exchange, the seeded re-march), marches every
@@ -309,7 +312,7 @@ def series_exact(steps, warmup, n, img, timesteps, keep_first=True):
     rgba = DeviceArray((img, img, 4), np.float32)
     depth = DeviceArray((img, img), np.float32)
     rs, samples, light_steps, phase_rows, first = [], [], [], [], None
-    schedule, sched_ms = None, {}
+    sched_ms = {}
     for ts in range(timesteps):
         sc = series_scene(ts, n, img)
         r = B.LightedVolumeRenderer()
@@ -826,7 +829,7 @@ def main():
     n, img, T = args.series_size, args.series_image, max(1, min(8, args.series_timesteps))
     nb = SERIES["nbricks"]
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    ex = series_exact(args.steps, args.warmup, n, img, T)
+    ex = series_exact(args.steps, args.warmup, n, img, T, schedule=json.loads(args.schedule) if args.schedule else None)
     sec = ex["sec"]
     map_bytes, field_bytes, maps_bytes = series_map_bytes(ex["ranges"], ex["cells"], 1, n)
     map_ms = ex["phase"]["map"]
