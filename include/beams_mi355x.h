@@ -151,6 +151,17 @@ typedef struct beams_schedule {
 int beams_set_schedule(beams_renderer* h, const beams_schedule* s);
 int beams_get_schedule(const beams_renderer* h, beams_schedule* s);
 
+/* Order in which the bricks' partial images are composited (bricks only).
+ * BEAMS_ORDER_CENTER_DISTANCE (default) is the reference's: ascending |camera - block centre|
+ * (MapperLightedVolume.cxx:222-236), which can put a farther brick first when bricks differ
+ * in size.  BEAMS_ORDER_KD_FRONT_TO_BACK (an extension) is a visibility order of a kd
+ * decomposition: split the bricks by an axis-aligned plane that no brick straddles, the
+ * camera's side first, recursively; a frame fails with BEAMS_ERR_UNSUPPORTED when some set of
+ * bricks has no such plane (beams_gen_kd_bricks always has one). */
+#define BEAMS_ORDER_CENTER_DISTANCE 0
+#define BEAMS_ORDER_KD_FRONT_TO_BACK 1
+int beams_set_composite_order(beams_renderer* h, int32_t order);
+
 /* Frames of beams_render_image replay a hipGraph captured from their launches; the graph
  * is captured again whenever a kernel argument changes (camera, outputs, scene state). */
 int beams_set_graph(beams_renderer* h, int32_t enabled);

@@ -80,6 +80,8 @@ typedef struct OrcRays {
   int orc_nonlocal_seed_range_##SUF(int nblocks, const OrcScene* scenes,                   \
                                     const void* const* maps, int self, int64_t begin,      \
                                     int64_t end, void* seed_out, int nthr);                \
+  int orc_composite_order_##SUF(int nblocks, const int* order, const void* const* images,   \
+                                int64_t npix, void* out);                                 \
   int orc_composite_##SUF(int nblocks, const double* block_bounds, const double* cam_pos,  \
                           const void* const* images, int64_t npix, void* out);
 

@@ -91,6 +91,7 @@ def load(path):
             "orc_nonlocal_seed": [I32, VP, VP, I32, VP, I32],
             "orc_nonlocal_seed_range": [I32, VP, VP, I32, I64, I64, VP, I32],
             "orc_composite": [I32, VP, VP, VP, I64, VP],
+            "orc_composite_order": [I32, VP, VP, I64, VP],
         }
         for name, args in sig.items():
             getattr(L, f"{name}_{tw}").argtypes = args
@@ -284,17 +285,56 @@ def map_estimate(vol, map_, point):
     return rgb, op[0]
 
 
-def composite(block_bounds, cam_pos, images, twin="f32"):
+def composite(block_bounds, cam_pos, images, twin="f32", order=None):
+    """GlobalComposite (MapperLightedVolume.cxx:215-303): the blocks' images over-composited in
+    ascending |camera - block centre|, or in the given order (first = nearest)."""
     dt = _dt(twin)
     imgs = [np.ascontiguousarray(im, dt) for im in images]
     npix = imgs[0].size // 4
     out = np.zeros_like(imgs[0])
+    ptrs = (C.c_void_p * len(imgs))(*[im.ctypes.data for im in imgs])
+    if order is not None:
+        o = np.ascontiguousarray(order, np.int32)
+        assert sorted(o.tolist()) == list(range(len(imgs)))
+        getattr(lib(), f"orc_composite_order_{twin}")(len(imgs), o.ctypes.data, ptrs, C.c_int64(npix),
+                                                      out.ctypes.data)
+        return out
     bb = np.ascontiguousarray(block_bounds, np.float64)
     cp = np.ascontiguousarray(cam_pos, np.float64)
-    ptrs = (C.c_void_p * len(imgs))(*[im.ctypes.data for im in imgs])
     getattr(lib(), f"orc_composite_{twin}")(len(imgs), bb.ctypes.data, cp.ctypes.data, ptrs,
                                             C.c_int64(npix), out.ctypes.data)
     return out
+
+
+def kd_order(block_bounds, eye):
+    """A front-to-back visibility order of boxes (rows lo0, hi0, lo1, hi1, lo2, hi2) forming a kd
+    decomposition (an extension; the reference composites by centre distance): of the
+    axis-aligned planes at box faces that no box straddles and that leave boxes on both
+    sides, the one splitting the set most evenly, the eye's side first, recursively.  None
+    when some subset has no such plane."""
+    bb = np.asarray(block_bounds, np.float64).reshape(-1, 6)
+
+    def rec(ids):
+        if len(ids) <= 1:
+            return list(ids)
+        best = None
+        for a in range(3):
+            lo, hi = bb[ids, 2 * a], bb[ids, 2 * a + 1]
+            for plane in np.unique(lo):
+                below, above = hi <= plane, lo >= plane
+                if np.all(below | above) and below.any() and above.any():
+                    score = abs(int(below.sum()) - int(above.sum()))
+                    if best is None or score < best[0]:
+                        best = (score, a, plane, [i for i, k in zip(ids, below) if k],
+                                [i for i, k in zip(ids, above) if k])
+        if best is None:
+            return None
+        _, a, plane, low, high = best
+        first, second = (low, high) if eye[a] < plane else (high, low)
+        r1, r2 = rec(first), rec(second)
+        return None if r1 is None or r2 is None else r1 + r2
+
+    return rec(list(range(len(bb))))
 
 
 # ---- many blocks (the reference's one block per MPI rank) ------------------------------------

@@ -849,6 +849,26 @@ int FN(orc_nonlocal_seed)(int nblocks, const OrcScene* scenes, const void* const
 }
 
 /* ---- MapperLightedVolume.cxx:215-303 --------------------------------------------------- */
+/* GlobalComposite's over operator over an explicit block order (first = nearest). */
+int FN(orc_composite_order)(int nblocks, const int* order, const void* const* images, int64_t npix, void* out)
+{
+  R* o = (R*)out;
+  memcpy(o, images[order[0]], sizeof(R) * 4 * (size_t)npix);
+  for (int i = 1; i < nblocks; ++i) {
+    const R* b = (const R*)images[order[i]];
+    for (int64_t j = 0; j < npix; ++j) {
+      R* a = o + 4 * j;
+      const R* q = b + 4 * j;
+      R ia = (R)1 - a[3];
+      a[0] = a[0] + q[0] * ia;
+      a[1] = a[1] + q[1] * ia;
+      a[2] = a[2] + q[2] * ia;
+      a[3] = a[3] + q[3] * ia;
+    }
+  }
+  return 0;
+}
+
 int FN(orc_composite)(int nblocks, const double* block_bounds, const double* cam_pos,
                       const void* const* images, int64_t npix, void* out)
 {
@@ -873,21 +893,7 @@ int FN(orc_composite)(int nblocks, const double* block_bounds, const double* cam
     }
     order[j + 1] = t;
   }
-  R* o = (R*)out;
-  const R* first = (const R*)images[order[0]];
-  memcpy(o, first, sizeof(R) * 4 * (size_t)npix);
-  for (int i = 1; i < nblocks; ++i) {
-    const R* b = (const R*)images[order[i]];
-    for (int64_t j = 0; j < npix; ++j) {
-      R* a = o + 4 * j;
-      const R* q = b + 4 * j;
-      R ia = (R)1 - a[3];
-      a[0] = a[0] + q[0] * ia;
-      a[1] = a[1] + q[1] * ia;
-      a[2] = a[2] + q[2] * ia;
-      a[3] = a[3] + q[3] * ia;
-    }
-  }
+  FN(orc_composite_order)(nblocks, order, images, npix, out);
   free(order);
   free(depth);
   return 0;

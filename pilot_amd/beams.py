@@ -21,6 +21,7 @@ LIGHT_POINT, LIGHT_DIRECTIONAL = 0, 1
 SHADOW_REF_MARCH = 0
 SHADOW_SWEEP = 1
 SHADOW_LEARNED = 2
+ORDER_CENTER_DISTANCE, ORDER_KD_FRONT_TO_BACK = 0, 1
 MLP_FEATURES = 32
 PHASES = ("map", "exchange", "remap", "march", "composite", "frame")
 
@@ -81,6 +82,7 @@ SIGNATURES = {
     "beams_get_brick_map": (C.c_int, [_P, _I32, _I32, _PF]),
     "beams_set_persistent_groups": (C.c_int, [_P, _I32]),
     "beams_set_graph": (C.c_int, [_P, _I32]),
+    "beams_set_composite_order": (C.c_int, [_P, _I32]),
     "beams_autotune": (C.c_int, [_P, C.POINTER(Camera), _I32, _I32, _I32, C.POINTER(C.c_int32), _PF]),
     "beams_set_scalar_range": (C.c_int, [_P, _D, _D]),
     "beams_set_colormap": (C.c_int, [_P, _PF, _I64]),
@@ -361,6 +363,10 @@ class LightedVolumeRenderer:
 
     def SetGraph(self, enabled):
         self._c(lib().beams_set_graph(self._h, int(bool(enabled))))
+
+    def SetCompositeOrder(self, order):
+        """ORDER_CENTER_DISTANCE (the reference's) or ORDER_KD_FRONT_TO_BACK (an extension)."""
+        self._c(lib().beams_set_composite_order(self._h, int(order)))
 
     def Autotune(self, camera, width, height, frames=3):
         g = C.c_int32()

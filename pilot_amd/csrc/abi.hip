@@ -11,6 +11,7 @@
 #include "learned.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -150,6 +151,7 @@ struct beams_renderer {
   // schedule knobs of the brick launches (the same results whatever their values)
   int32_t xcd_ranges = 1, light_rows = 1, march_xcd_tiles = 1, map_fast_fetch = 0, march_fast_fetch = 1;
   int32_t map_pipeline = 0;
+  int32_t composite_order = BEAMS_ORDER_CENTER_DISTANCE;
   DevBuf brows;                          // every brick's map rows in light-sheet order
   std::vector<unsigned char> brows_key;  // the light and brick maps the rows were ordered for
   int64_t btab_version = 0;
@@ -536,6 +538,36 @@ int build_map(beams_renderer* h, const GridParams& g, const ShadeParams& s)
 
 // ---- bricks ---------------------------------------------------------------------------------
 
+// Front-to-back visibility order of boxes (lo/hi per axis) that form a kd decomposition: an
+// axis-aligned plane that no box of `set` straddles (boxes sorted by their low face; the first
+// prefix whose highest high face is at or below the next box's low face), the eye's side
+// first, then each side recursively.  False when some subset has no such plane.
+bool kd_front_to_back(const std::vector<std::array<double, 6>>& boxes, const double eye[3],
+                      std::vector<int32_t> set, std::vector<int32_t>& out)
+{
+  if (set.size() <= 1) {
+    out.insert(out.end(), set.begin(), set.end());
+    return true;
+  }
+  for (int a = 0; a < 3; ++a) {
+    std::stable_sort(set.begin(), set.end(), [&](int32_t p, int32_t q) {
+      return boxes[(size_t)p][(size_t)(2 * a)] < boxes[(size_t)q][(size_t)(2 * a)];
+    });
+    double hi = -INFINITY;
+    for (size_t k = 0; k + 1 < set.size(); ++k) {
+      hi = std::max(hi, boxes[(size_t)set[k]][(size_t)(2 * a + 1)]);
+      const double plane = boxes[(size_t)set[k + 1]][(size_t)(2 * a)];
+      if (hi <= plane) {
+        std::vector<int32_t> low(set.begin(), set.begin() + (long)k + 1), high(set.begin() + (long)k + 1, set.end());
+        const bool low_first = eye[a] < plane;
+        return kd_front_to_back(boxes, eye, low_first ? low : high, out) &&
+               kd_front_to_back(boxes, eye, low_first ? high : low, out);
+      }
+    }
+  }
+  return false;
+}
+
 // Screen rectangle (x0, y0, w, h) that contains every pixel whose camera ray can meet the
 // box: the bounding box of the projected corners plus 2 pixels, or the whole image when a
 // corner is not in front of the camera.
@@ -740,8 +772,20 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
     depth[(size_t)b] = std::sqrt(dx * dx + dy * dy + dz * dz);
     order[(size_t)b] = b;
   }
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int32_t a, int32_t b) { return depth[(size_t)a] < depth[(size_t)b]; });
+  if (h->composite_order == BEAMS_ORDER_KD_FRONT_TO_BACK) {
+    std::vector<std::array<double, 6>> boxes((size_t)nb);
+    for (int b = 0; b < nb; ++b)
+      for (int k = 0; k < 6; ++k) boxes[(size_t)b][(size_t)k] = desc[(size_t)b].g.bounds[k];
+    order.clear();
+    std::vector<int32_t> all((size_t)nb);
+    for (int b = 0; b < nb; ++b) all[(size_t)b] = b;
+    if (!kd_front_to_back(boxes, c.eye, all, order))
+      return h->fail(BEAMS_ERR_UNSUPPORTED, "kd composite order: the bricks are not a kd decomposition "
+                                            "(some set has no axis-aligned plane that no brick straddles)");
+  } else {
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return depth[(size_t)a] < depth[(size_t)b]; });
+  }
   HIP_TRY(h, h->bpartial.ensure(std::max<size_t>(partial_pixels, 1) * 20));
   // the light-sheet row orders, recomputed when the light or a brick's map changes
   if (h->light_rows) {
@@ -1311,6 +1355,15 @@ int beams_get_schedule(const beams_renderer* h, beams_schedule* s)
   s->map_fast_fetch = h->map_fast_fetch;
   s->march_fast_fetch = h->march_fast_fetch;
   s->map_pipeline = h->map_pipeline;
+  return BEAMS_OK;
+}
+
+int beams_set_composite_order(beams_renderer* h, int32_t order)
+{
+  CHECK_HANDLE(h);
+  if (order != BEAMS_ORDER_CENTER_DISTANCE && order != BEAMS_ORDER_KD_FRONT_TO_BACK)
+    return h->fail(BEAMS_ERR_INVALID, "unknown composite order");
+  h->composite_order = order;
   return BEAMS_OK;
 }
 

@@ -168,3 +168,35 @@ def test_bricks_schedules_give_the_same_frame(oracle_mod, knobs):
             assert np.array_equal(maps[b][0], local[b]), f"brick {b} local map"
             assert np.array_equal(maps[b][1], final[b]), f"brick {b} final map"
         assert np.array_equal(img.reshape(-1, 4), oimg)
+
+
+def test_bricks_kd_composite_order_matches_oracle(oracle_mod):
+    """The kd front-to-back composite order (an extension): the bricks' partial images
+    over-composited in the oracle's kd visibility order, bit for bit; bricks that are no kd
+    decomposition are refused."""
+    O = oracle_mod
+    sc = scenes.make_scene("bricks", 49, seed=33, nblobs=12, width=96, height=72, light="point")
+    ranges = B.kd_bricks(sc.dims, 10, 17, 6, 32)
+    with B.LightedVolumeRenderer() as r:
+        scenes.configure(r, sc)
+        r.SetBricks(ranges, ghost=1)
+        r.SetCompositeOrder(B.ORDER_KD_FRONT_TO_BACK)
+        print("[bricks kd order]", flush=True)
+        img, _ = r.RenderImage(sc.camera, sc.width, sc.height)
+    (_, _, _, partial), o, d = _oracle(O, sc, ranges)
+    vols = O.brick_volumes(sc, ranges)
+    bb = [[float(v.coords[a][k]) for a in range(3) for k in (0, -1)] for v in vols]
+    order = O.kd_order(bb, np.asarray(sc.camera.eye, np.float64))
+    assert order is not None
+    oimg = O.composite(None, None, partial, order=order)
+    assert np.array_equal(img.reshape(-1, 4), oimg)
+    pin = np.array([[0, 20, 0, 10, 0, 10], [20, 30, 0, 20, 0, 10], [10, 30, 20, 30, 0, 10],
+                    [0, 10, 10, 30, 0, 10], [10, 20, 10, 20, 0, 10]])
+    sc2 = scenes.make_scene("pinwheel", 31, seed=3, nblobs=4, width=32, height=32, light="point")
+    with B.LightedVolumeRenderer() as r:
+        scenes.configure(r, sc2)
+        r.SetBricks(pin, ghost=1)
+        r.RenderImage(sc2.camera, 32, 32)  # the reference order renders it
+        r.SetCompositeOrder(B.ORDER_KD_FRONT_TO_BACK)
+        with pytest.raises(B.BeamsError, match="kd decomposition"):
+            r.RenderImage(sc2.camera, 32, 32)
