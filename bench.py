@@ -161,9 +161,10 @@ def learned_scene(seed, n, img, **kw):
     from pilot_amd import scenes
 
     kw.setdefault("map_cells", (n, n, n))
+    kw.setdefault("device", True)
     return scenes.make_scene("learned", n, seed=seed, nblobs=64, noise_octaves=3, noise_amp=0.2,
                              light="point", width=img, height=img, field_dtype=B.F16,
-                             map_dtype=B.F16, device=True, **kw)
+                             map_dtype=B.F16, **kw)
 
 
 def train_closure(n, seeds, per_field, epochs, batch=4096, lr=1e-3):
@@ -275,15 +276,16 @@ SERIES = dict(n=1024, img=2048, nbricks=128, timesteps=8, seed=9, brick_seed=5, 
               max_cells=320)
 
 
-def series_scene(ts, n, img):
+def series_scene(ts, n, img, device=True):
     """Timestep ts of the drifting field: 64 blobs moving 0.01 box/step + 3-octave value noise
-    (amplitude 0.2) whose phase advances 0.25/step, generated in HBM as FP16; FP16 maps."""
+    (amplitude 0.2) whose phase advances 0.25/step, generated in HBM as FP16 (device) or on the
+    host; FP16 maps."""
     from pilot_amd import beams as B
     from pilot_amd import scenes
 
     return scenes.make_scene("series", n, seed=SERIES["seed"], nblobs=64, noise_octaves=3,
                              noise_amp=0.2, light="point", width=img, height=img,
-                             field_dtype=B.F16, map_dtype=B.F16, device=True, time=float(ts),
+                             field_dtype=B.F16, map_dtype=B.F16, device=device, time=float(ts),
                              drift_speed=0.01, noise_phase_rate=0.25)
 
 
@@ -321,7 +323,7 @@ def series_exact(steps, warmup, n, img, timesteps, keep_first=True, schedule=Non
         if schedule is None:
             # the schedule of the brick launches: the frame with every switch off and with every
             # switch on, then the autotuner's pick (coordinate descent), used for every timestep
-            knobs = [k for k in B.LightedVolumeRenderer.SCHEDULE_KNOBS if k != "persistent_groups"]
+            knobs = [k for k in B.LightedVolumeRenderer.SCHEDULE_KNOBS if k not in ("persistent_groups", "sweep_block")]
             trials = [("all switches off", {k: 0 for k in knobs}), ("all switches on", {k: 1 for k in knobs})]
             trials += [(f"all on but {k}", {q: int(q != k) for q in knobs}) for k in knobs]
             for label, sw in trials:
@@ -635,6 +637,132 @@ def precision_report(n, img, threads, net=None):
     }
 
 
+def fp64_report(threads, net, learned_n=512, learned_img=2048, tile=256, small=161):
+    """RGBA error against the FP64 oracle (the reference algorithm in FP64 on the exact field,
+    FP64 rays) for the workloads beyond the precision frame:
+    - the 512^3 single-block learned workload (exact reference march; learned with and without
+      its fallback) on a 256^2 tile of its 2048^2 frame (the one with the most partly
+      transparent, lit pixels): the FP64 map is built at the vertices inside the bounding box of
+      the tile rays' entry and exit points (the rays' samples look up no other vertex);
+    - the brick workloads (time-series recipe, 128 bricks; batched-bricks recipe, 96 bricks) at
+      161^3 with 256^2 frames: the FP64 multi-rank pipeline, and for the learned bricks their
+      FP64 counterpart (the single-block map at the bricks' vertices)."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from pilot_amd import beams as B
+    from pilot_amd import scenes
+
+    out = {}
+    # ---- single block, tile
+    sc = learned_scene(2001, learned_n, learned_img, device=False)
+    vol = O.volume_from_scene(sc, "f64")
+    o, d, o64, d64 = B.camera_rays(sc.camera, learned_img, learned_img, want64=True)
+    modes = [("exact reference march", B.SHADOW_REF_MARCH, None)]
+    if net is not None:
+        modes += [("learned, no fallback", B.SHADOW_LEARNED, 1e30),
+                  ("learned, budget 2e-3", B.SHADOW_LEARNED, 2e-3)]
+    images = {}
+    for label, mode, budget in modes:
+        with B.LightedVolumeRenderer() as r:
+            scenes.configure(r, sc)
+            r.SetShadow(True, sc.map_cells, B.F16, mode=mode)
+            if budget is not None:
+                r.SetLearned(net, fallback=B.SHADOW_REF_MARCH, error_budget=budget, check_stride=8)
+            img, _ = r.RenderImage(sc.camera, learned_img, learned_img)
+            images[label] = img.reshape(-1, 4)
+    # the tile with the most partly transparent, lit pixels of the exact frame (much of the
+    # frame is background or the fully shadowed core, where every path agrees)
+    ex = images["exact reference march"].reshape(learned_img, learned_img, 4)
+    nt = learned_img // tile
+    score = [(((ex[ty * tile:(ty + 1) * tile, tx * tile:(tx + 1) * tile, 3] > 0.05) &
+               (ex[ty * tile:(ty + 1) * tile, tx * tile:(tx + 1) * tile, :3].max(axis=2) > 0.05)).sum(), ty, tx)
+             for ty in range(nt) for tx in range(nt)]
+    _, ty, tx = max(score)
+    idx = ((np.arange(ty * tile, (ty + 1) * tile)[:, None] * learned_img) +
+           np.arange(tx * tile, (tx + 1) * tile)[None, :]).ravel()
+    to, td = o64[:, idx], d64[:, idx]
+    lo = np.array([c[0] for c in sc.coords64]); hi = np.array([c[-1] for c in sc.coords64])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t0 = (lo[:, None] - to) / td
+        t1 = (hi[:, None] - to) / td
+    tin = np.maximum(np.nanmax(np.minimum(t0, t1), axis=0), 0.0)
+    tout = np.nanmin(np.maximum(t0, t1), axis=0)
+    hit = tout > tin
+    pts = np.concatenate([to[:, hit] + td[:, hit] * tin[hit], to[:, hit] + td[:, hit] * tout[hit]], axis=1)
+    cells = np.array(sc.map_cells)
+    sp = (hi - lo) / cells
+    vlo = np.clip(np.floor((pts.min(axis=1) - lo) / sp).astype(int) - 1, 0, cells)
+    vhi = np.clip(np.ceil((pts.max(axis=1) - lo) / sp).astype(int) + 1, 0, cells)
+    ax = [lo[a] + sp[a] * np.arange(vlo[a], vhi[a] + 1) for a in range(3)]
+    Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    t_map = time.perf_counter()
+    vals = O.march_points(vol, [X.ravel(), Y.ravel(), Z.ravel()], threads)
+    mp = cells + 1
+    full = np.full(int(np.prod(mp)), np.nan)
+    ii, jj, kk = np.meshgrid(np.arange(vlo[2], vhi[2] + 1), np.arange(vlo[1], vhi[1] + 1),
+                             np.arange(vlo[0], vhi[0] + 1), indexing="ij")
+    full[(kk + mp[0] * (jj + mp[1] * ii)).ravel()] = vals
+    rb = O.RayBuf(to, td, "f64")
+    O.sample_rays(vol, full, rb, threads)
+    ref = rb.rgba.reshape(-1, 4)
+    assert np.isfinite(ref).all(), "a tile ray looked up a map vertex outside the built box"
+    t_map = time.perf_counter() - t_map
+    rows = {}
+    for label, _, _ in modes:
+        rows[label] = image_error(images[label][idx], ref)
+        print(f"[bench] fp64 {learned_n}^3 tile {label}: {rows[label]}", file=sys.stderr, flush=True)
+    lit = int(((ref[:, 3] > 0.05) & (ref[:, :3].max(axis=1) > 0.05)).sum())
+    out[f"{learned_n}^3 FP16 single block, {learned_img}^2, {tile}^2 tile"] = {
+        "tile_origin_px": [int(tx * tile), int(ty * tile)], "tile_lit_px": lit,
+        "rgba_err_vs_fp64_oracle": rows, "fp64_map_vertices": int(vals.size), "oracle_s": t_map}
+    # ---- bricks at a reduced size
+    recipes = [("time-series recipe, 128 bricks", series_scene(0, small, tile, device=False),
+                B.kd_bricks((small,) * 3, 128, SERIES["brick_seed"], SERIES["min_cells"] * small // 1024,
+                            SERIES["max_cells"] * small // 1024)),
+               ("batched-bricks recipe, 96 bricks",
+                scenes.make_scene("bricks", small, seed=3, nblobs=64, noise_octaves=3, noise_amp=0.2,
+                                  light="point", width=tile, height=tile, field_dtype=B.F16, map_dtype=B.F16),
+                B.kd_bricks((small,) * 3, 96, 3, 32 * small // 512, 160 * small // 512))]
+    for label, scb, ranges in recipes:
+        cells_b = ranges[:, 1::2] - ranges[:, 0::2]
+        o, d, o64, d64 = B.camera_rays(scb.camera, tile, tile, want64=True)
+        vols64 = O.brick_volumes(scb, ranges, twin="f64", map_cells=cells_b)
+        t_o = time.perf_counter()
+        ref, _, _, _ = O.render_bricks(vols64, o64, d64, list(scb.camera.eye), nthreads=threads)
+        rows = {}
+        with B.LightedVolumeRenderer() as r:
+            scenes.configure(r, scb)
+            r.SetBricks(ranges, ghost=1, map_cells=cells_b)
+            img, _ = r.RenderImage(scb.camera, tile, tile)
+        rows["exact (multi-rank)"] = image_error(img.reshape(-1, 4), ref)
+        if net is not None:
+            whole = O.volume_from_scene(scb, "f64")
+            maps = [O.march_points(whole, O.map_points(v), threads) for v in vols64]
+            partial, bounds = [], []
+            for v, m in zip(vols64, maps):
+                rb = O.RayBuf(o64, d64, "f64")
+                O.sample_rays(v, m, rb, threads)
+                partial.append(rb.rgba)
+                bounds += [float(v.coords[a][k]) for a in range(3) for k in (0, -1)]
+            lref = O.composite(np.array(bounds), np.asarray(scb.camera.eye, np.float64), partial, "f64")
+            with B.LightedVolumeRenderer() as r:
+                scenes.configure(r, scb)
+                r.SetShadow(True, scb.map_cells, B.F16, mode=B.SHADOW_LEARNED)
+                r.SetLearned(net, fallback=B.SHADOW_REF_MARCH, error_budget=SERIES_LEARNED_BUDGET, check_stride=8)
+                r.SetBricks(ranges, ghost=1, map_cells=cells_b)
+                img_l, _ = r.RenderImage(scb.camera, tile, tile)
+            rows["learned (budget %g) vs its FP64 counterpart (single-block map at the bricks)"
+                 % SERIES_LEARNED_BUDGET] = image_error(img_l.reshape(-1, 4), lref)
+            rows["learned vs the FP64 multi-rank frame"] = image_error(img_l.reshape(-1, 4), ref)
+        t_o = time.perf_counter() - t_o
+        for k, v in rows.items():
+            print(f"[bench] fp64 {small}^3 {label} {k}: {v}", file=sys.stderr, flush=True)
+        out[f"{small}^3 FP16 {label}, {tile}^2 frame"] = {"rgba_err_vs_fp64_oracle": rows, "oracle_s": t_o}
+    return out
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the byte-counter runs summarised in
     profiles/traffic.json (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the counters' reading
@@ -920,6 +1048,7 @@ def main():
         line["workloads"]["series_learned"] = sl
     if not args.no_precision:
         line["precision"] = precision_report(args.size, args.image, threads, net)
+        line["precision"]["other_workloads"] = fp64_report(threads, net)
     if net is not None:
         net.close()
     print(json.dumps(line))

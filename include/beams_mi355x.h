@@ -147,6 +147,10 @@ typedef struct beams_schedule {
   int32_t march_fast_fetch;  /* 1: the same for the march launch                                */
   int32_t map_pipeline;      /* 1: the map launches issue each light step's corner loads one
                                 step ahead                                                      */
+  int32_t sweep_block;       /* 1: the exact sweep advances several slices per launch from an
+                                LDS halo (one sigma pass first); 0: one launch per slice.  The
+                                maps are bit-identical; a light too oblique for the halo always
+                                takes one launch per slice                                      */
 } beams_schedule;
 int beams_set_schedule(beams_renderer* h, const beams_schedule* s);
 int beams_get_schedule(const beams_renderer* h, beams_schedule* s);

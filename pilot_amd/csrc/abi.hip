@@ -98,6 +98,7 @@ struct beams_renderer {
   int32_t map_dtype = BEAMS_F32;
   DevBuf map;
   DevBuf sweep_slices;  // (tau, sigma) ping-pong slices of the exact sweep
+  DevBuf sweep_sigma;   // per-vertex extinction of the blocked sweep
   bool map_valid = false;
   int32_t built_mode = BEAMS_SHADOW_REF_MARCH;  // the mode (and so the layout) of the map held
   // learned closure
@@ -151,6 +152,7 @@ struct beams_renderer {
   // schedule knobs of the brick launches (the same results whatever their values)
   int32_t xcd_ranges = 1, light_rows = 1, march_xcd_tiles = 1, map_fast_fetch = 0, march_fast_fetch = 1;
   int32_t map_pipeline = 0;
+  int32_t sweep_block = 1;
   int32_t composite_order = BEAMS_ORDER_CENTER_DISTANCE;
   DevBuf brows;                          // every brick's map rows in light-sheet order
   std::vector<unsigned char> brows_key;  // the light and brick maps the rows were ordered for
@@ -431,6 +433,7 @@ int prepare_map(beams_renderer* h, int32_t mode)
       return h->fail(BEAMS_ERR_UNSUPPORTED, "the sweep needs a light outside the volume's bounds");
     const size_t ns = (size_t)(h->map_cells[a1] + 1) * (size_t)(h->map_cells[a2] + 1);
     HIP_TRY(h, h->sweep_slices.ensure(2 * ns * sizeof(float2)));
+    if (h->sweep_block) HIP_TRY(h, h->sweep_sigma.ensure(ns * (size_t)(h->map_cells[axis] + 1) * sizeof(float)));
   }
   return BEAMS_OK;
 }
@@ -450,7 +453,8 @@ int enqueue_map(beams_renderer* h, const GridParams& g, const MapParams& m, cons
     const size_t ns = (size_t)m.mp[sp.a1] * (size_t)m.mp[sp.a2];
     sp.prev = (float2*)h->sweep_slices.p;
     sp.cur = sp.prev + ns;
-    HIP_TRY(h, launch_sweep(sp, h->stream));
+    const SweepPlan plan = sweep_plan(sp, h->bounds);
+    HIP_TRY(h, launch_sweep(sp, plan, h->sweep_block ? (float*)h->sweep_sigma.p : nullptr, h->stream));
   } else {
     HIP_TRY(h, launch_tmap_ref(g, m, s, nullptr, h->stream));
   }
@@ -1080,6 +1084,7 @@ int beams_destroy(beams_renderer* h)
   h->tf.release();
   h->map.release();
   h->sweep_slices.release();
+  h->sweep_sigma.release();
   h->coarse.release();
   h->lerr.release();
   h->rays.release();
@@ -1342,6 +1347,7 @@ int beams_set_schedule(beams_renderer* h, const beams_schedule* s)
   h->map_fast_fetch = s->map_fast_fetch != 0;
   h->march_fast_fetch = s->march_fast_fetch != 0;
   h->map_pipeline = s->map_pipeline != 0;
+  h->sweep_block = s->sweep_block != 0;
   return BEAMS_OK;
 }
 
@@ -1355,6 +1361,7 @@ int beams_get_schedule(const beams_renderer* h, beams_schedule* s)
   s->map_fast_fetch = h->map_fast_fetch;
   s->march_fast_fetch = h->march_fast_fetch;
   s->map_pipeline = h->map_pipeline;
+  s->sweep_block = h->sweep_block;
   return BEAMS_OK;
 }
 
@@ -1704,7 +1711,7 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
     return mark(BEAMS_PHASE_MARCH, true);
   };
   KeyBuilder key;
-  key.put(0).put(g).put(m).put(s).put(c).put(o).put(with_map);
+  key.put(0).put(g).put(m).put(s).put(c).put(o).put(with_map).put(h->sweep_block);
   if (learned)
     key.put(h->fallback_mode).put(h->error_budget).put(h->check_stride).put(h->net ? mlp_device_params(h->net) : nullptr);
   if ((rc = run_frame(h, key.bytes, issue))) return rc;

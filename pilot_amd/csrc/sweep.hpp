@@ -18,7 +18,18 @@ struct SweepParams {
   float2* cur;        // (tau, sigma) of the current slice
 };
 
-// Enqueues one launch per slice on st.
-hipError_t launch_sweep(SweepParams sp, hipStream_t st);
+// Launch shape of a sweep: the halo R (vertices a light ray can move across one slice, plus
+// one), slices per blocked launch nt, the tile width, their LDS, and whether the blocked shape
+// applies.
+struct SweepPlan {
+  int32_t R, nt, tile;
+  size_t lds;
+  bool blocked;
+};
+SweepPlan sweep_plan(const SweepParams& sp, const double* bounds);
+
+// Enqueues the sweep on st: blocked (sigma pass, then ceil(n0 / nt) launches) when plan.blocked
+// and sig (one float per map vertex) is given, else one launch per slice.
+hipError_t launch_sweep(SweepParams sp, const SweepPlan& plan, float* sig, hipStream_t st);
 
 }  // namespace beams

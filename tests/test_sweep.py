@@ -107,6 +107,33 @@ def test_gpu_sweep_matches_restatement(oracle_mod, light):
     assert np.array_equal(rays.rgba, rb.rgba)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("light,mdt,cells", [("point", B.F32, (40, 40, 40)), ("directional", B.F16, (40, 40, 40)),
+                                             ("point", B.F32, (23, 61, 37)), ("oblique", B.F32, (40, 40, 40))])
+def test_gpu_blocked_sweep_equals_per_slice(oracle_mod, light, mdt, cells):
+    """The blocked sweep (a sigma pass, then several slices per launch from an LDS halo) and
+    one launch per slice give the same map bit for bit: ragged maps (the halo is sized per
+    axis from the map spacing), FP16 storage, and a light so close to a face that the halo
+    cannot fit (both take one launch per slice)."""
+    kind = "point" if light == "oblique" else light
+    sc = scenes.make_scene("sweepblk", 41, seed=39, nblobs=12, noise_octaves=3, noise_amp=0.2,
+                           light=kind, width=64, height=48, map_cells=cells)
+    if light == "oblique":
+        sc.light_pos = np.float32([sc.coords[0][-1] * 0.5, sc.coords[1][-1] * 0.5, -0.002 * sc.coords[2][-1]])
+    maps = {}
+    for blocked in (1, 0):
+        with B.LightedVolumeRenderer() as r:
+            scenes.configure(r, sc)
+            r.SetShadow(True, cells, mdt, mode=B.SHADOW_SWEEP)
+            r.SetSchedule(sweep_block=blocked)
+            print(f"[sweep blocked={blocked} {light} {cells}]", flush=True)
+            img, _ = r.RenderImage(sc.camera, sc.width, sc.height)
+            maps[blocked] = (r.GetMap(cells), img)
+    assert np.array_equal(maps[1][0], maps[0][0])
+    assert np.array_equal(maps[1][1], maps[0][1])
+    assert maps[1][0].max() > 0.1
+
+
 @pytest.mark.parametrize("kind,light", [(1, (0.3, -0.45, 0.84)), (0, (1.9, 0.7, -0.8))])
 def test_sweep_converges_to_the_continuum_depth(kind, light):
     """On a smooth field and TF the sweep's opacity approaches the continuum light-path depth
