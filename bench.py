@@ -323,7 +323,8 @@ def series_exact(steps, warmup, n, img, timesteps, keep_first=True, schedule=Non
         if schedule is None:
             # the schedule of the brick launches: the frame with every switch off and with every
             # switch on, then the autotuner's pick (coordinate descent), used for every timestep
-            knobs = [k for k in B.LightedVolumeRenderer.SCHEDULE_KNOBS if k not in ("persistent_groups", "sweep_block")]
+            knobs = [k for k in B.LightedVolumeRenderer.SCHEDULE_KNOBS
+                     if k not in ("persistent_groups", "sweep_block", "march_groups_per_cu")]
             trials = [("all switches off", {k: 0 for k in knobs}), ("all switches on", {k: 1 for k in knobs})]
             trials += [(f"all on but {k}", {q: int(q != k) for q in knobs}) for k in knobs]
             for label, sw in trials:

@@ -151,6 +151,8 @@ typedef struct beams_schedule {
                                 LDS halo (one sigma pass first); 0: one launch per slice.  The
                                 maps are bit-identical; a light too oblique for the halo always
                                 takes one launch per slice                                      */
+  int32_t march_groups_per_cu; /* 1..16: at most that many march workgroups share a CU (a larger
+                                  share of the L2 per screen tile); 0: no cap                   */
 } beams_schedule;
 int beams_set_schedule(beams_renderer* h, const beams_schedule* s);
 int beams_get_schedule(const beams_renderer* h, beams_schedule* s);

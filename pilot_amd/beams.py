@@ -343,7 +343,8 @@ class LightedVolumeRenderer:
         self._c(lib().beams_set_persistent_groups(self._h, int(groups)))
 
     SCHEDULE_KNOBS = ("persistent_groups", "xcd_ranges", "light_rows", "march_xcd_tiles",
-                      "map_fast_fetch", "march_fast_fetch", "map_pipeline", "sweep_block")
+                      "map_fast_fetch", "march_fast_fetch", "map_pipeline", "sweep_block",
+                      "march_groups_per_cu")
 
     def SetSchedule(self, **knobs):
         """Brick-launch schedule knobs (SCHEDULE_KNOBS; results are the same under every

@@ -153,6 +153,7 @@ struct beams_renderer {
   int32_t xcd_ranges = 1, light_rows = 1, march_xcd_tiles = 1, map_fast_fetch = 0, march_fast_fetch = 1;
   int32_t map_pipeline = 0;
   int32_t sweep_block = 1;
+  int32_t march_groups_per_cu = 0;
   int32_t composite_order = BEAMS_ORDER_CENTER_DISTANCE;
   DevBuf brows;                          // every brick's map rows in light-sheet order
   std::vector<unsigned char> brows_key;  // the light and brick maps the rows were ordered for
@@ -848,6 +849,7 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
   f.map_fast = fast_ok && h->map_fast_fetch;
   f.march_fast = fast_ok && h->march_fast_fetch;
   f.map_pipeline = h->map_pipeline;
+  f.march_groups_per_cu = h->march_groups_per_cu;
   h->n_map_items = f.n_map_items;
   h->n_px_items = f.n_px_items;
   h->max_coords = max_coords;
@@ -1340,6 +1342,8 @@ int beams_set_schedule(beams_renderer* h, const beams_schedule* s)
   if (!s) return h->fail(BEAMS_ERR_INVALID, "null schedule");
   if (s->persistent_groups < 1 || s->persistent_groups > 65536)
     return h->fail(BEAMS_ERR_INVALID, "persistent groups must be in [1, 65536]");
+  if (s->march_groups_per_cu < 0 || s->march_groups_per_cu > 16)
+    return h->fail(BEAMS_ERR_INVALID, "march groups per CU must be in [0, 16]");
   h->persistent_groups = s->persistent_groups;
   h->xcd_ranges = s->xcd_ranges != 0;
   h->light_rows = s->light_rows != 0;
@@ -1348,6 +1352,7 @@ int beams_set_schedule(beams_renderer* h, const beams_schedule* s)
   h->march_fast_fetch = s->march_fast_fetch != 0;
   h->map_pipeline = s->map_pipeline != 0;
   h->sweep_block = s->sweep_block != 0;
+  h->march_groups_per_cu = s->march_groups_per_cu;
   return BEAMS_OK;
 }
 
@@ -1362,6 +1367,7 @@ int beams_get_schedule(const beams_renderer* h, beams_schedule* s)
   s->march_fast_fetch = h->march_fast_fetch;
   s->map_pipeline = h->map_pipeline;
   s->sweep_block = h->sweep_block;
+  s->march_groups_per_cu = h->march_groups_per_cu;
   return BEAMS_OK;
 }
 

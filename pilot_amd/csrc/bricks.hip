@@ -11,6 +11,8 @@
 // The reference routes 32-byte hit records through rank 0 with MPI; here the hits of a
 // vertex live in LDS for the duration of its loop and are never written to memory.
 #include "bricks.hpp"
+
+#include <algorithm>
 #include "workqueue.hpp"
 #include "camera.hpp"
 #include "march_dev.hpp"
@@ -606,7 +608,10 @@ hipError_t launch_bricks_march(const BrickFrame& f, int32_t field_dtype, int32_t
                                int32_t ntf, bool use_map, const CameraParams& c, hipStream_t st)
 {
   if (f.n_px_items == 0) return hipSuccess;
-  const size_t lds = sizeof(float4) * (size_t)ntf + sizeof(float) * 2 * (size_t)f.max_coords;
+  size_t lds = sizeof(float4) * (size_t)ntf + sizeof(float) * 2 * (size_t)f.max_coords;
+  // march_groups_per_cu > 0: LDS padded so that at most that many workgroups share a CU (a
+  // larger share of the XCD's L2 for each tile's rays)
+  if (f.march_groups_per_cu > 0) lds = std::max(lds, (size_t)(160 * 1024) / (size_t)f.march_groups_per_cu - 512);
   const dim3 grid((unsigned)f.n_px_items);
   BRICK_DISPATCH_FIELD(field_dtype, BRICK_DISPATCH_MAP(map_dtype, {
     if (use_map && f.march_fast)

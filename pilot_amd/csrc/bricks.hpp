@@ -49,6 +49,7 @@ struct BrickFrame {
   int32_t map_fast, march_fast;   // branch-free corner fetch in the map / march launches (every
                                   // brick's field and map view below 2^32 bytes, strides below 2^24)
   int32_t map_pipeline;           // map launches: next light step's loads in flight (march_vertex_pipe)
+  int32_t march_groups_per_cu;    // march launch: at most this many workgroups per CU (0: no cap)
 };
 
 hipError_t launch_bricks_map(const BrickFrame& f, int32_t field_dtype, int32_t map_dtype,
