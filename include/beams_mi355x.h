@@ -335,14 +335,15 @@ int beams_mlp_train(beams_mlp* h, const float* x, const float* t, int64_t n, int
  * fallback_mode (BEAMS_SHADOW_REF_MARCH or BEAMS_SHADOW_SWEEP) builds the map instead when the
  * largest opacity error at the check vertices (every check_stride-th map vertex per axis and
  * the last) exceeds error_budget; a negative budget sends every map to the fallback (the
- * learned frame's exact counterpart), a NaN one is refused.  The decision is taken on the
+ * learned frame's exact counterpart), a NaN one is refused.  With the reference-march fallback
+ * the map is rebuilt tile by tile (below); the sweep fallback rebuilds the whole map.  The decision is taken on the
  * device (the fallback's launches return at once when the check is within the budget), so a
  * learned frame is captured in one graph like the exact ones.  The learned map approximates
  * the single-block map (the whole volume's light march, step |size| / 128).  With bricks that
  * is the single-block map at each brick's vertices (no non-local exchange and no per-brick
- * light step, unlike the exact brick frame), checked and rebuilt tile by tile: the check
- * vertices cut each brick's map into tiles of up to check_stride^3 cells, and a tile is
- * re-marched when the error at its own or a neighbouring tile's corners exceeds the budget.  The renderer
+ * light step, unlike the exact brick frame).  The check vertices cut each map (the single
+ * block's, or each brick's) into tiles of up to check_stride^3 cells, and a tile is re-marched
+ * when the error at its own or a neighbouring tile's corners exceeds the budget.  The renderer
  * keeps the pointer: destroy the renderer or attach another network before destroying net. */
 int beams_set_learned(beams_renderer* h, const beams_mlp* net, int32_t fallback_mode,
                       float error_budget, int32_t check_stride);
@@ -354,7 +355,8 @@ int beams_learned_samples(beams_renderer* h, int32_t mode, int64_t n, uint64_t s
 /* The last learned map build (read back from the device: waits for the stream): err[0] =
  * largest |learned - exact| opacity at the check vertices, err[1] = the same for the coarse
  * optical depth alone (single block only, else -1); *fell_back = what the fallback
- * rebuilt (0 or 1 map for the single block; with bricks, the number of tiles).  -1, -1 and 0 before any. */
+ * rebuilt: the number of check tiles re-marched by the reference march (bricks, and the single
+ * block with that fallback), or 0 or 1 map (the single block with the sweep fallback).  -1, -1 and 0 before any. */
 int beams_get_learned_stats(beams_renderer* h, float err[2], int32_t* fell_back);
 
 #ifdef __cplusplus

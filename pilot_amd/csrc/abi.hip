@@ -146,6 +146,13 @@ struct beams_renderer {
   std::vector<int32_t> lb_host;
   DevBuf lb_tab, lb_err, lb_list;
   LearnedBricks lb{};
+  // learned single block: its fallback tiles through the brick kernels, as a frame of one brick
+  // whose map is the whole volume's (device descriptor, queue, tables, errors, tile list)
+  std::vector<unsigned char> sb_host;
+  DevBuf sb_desc, sb_queue, sb_tab, sb_err, sb_list;
+  LearnedBricks sb_lb{};
+  BrickFrame sb_f{};
+  bool sb_tiles = false;  // the last single-block learned build used the tile fallback
   std::vector<unsigned char> btab_host;  // tables of the last frame (reused while unchanged)
   int32_t n_map_items = 0, n_px_items = 0, max_coords = 0;
   int32_t persistent_groups = 2048;
@@ -494,11 +501,63 @@ void learned_built(beams_renderer* h, int32_t kind)
 }
 
 // Device buffers of the learned map build and its fallback (allocated outside any capture).
-int prepare_learned(beams_renderer* h)
+// With the reference-march fallback, the single block is re-marched tile by tile like a brick:
+// a one-brick frame (its descriptor, queue counters and tables uploaded when they change).
+int prepare_learned(beams_renderer* h, const GridParams& g, const ShadeParams& s)
 {
   int rc = prepare_map(h, BEAMS_SHADOW_LEARNED);
   if (!rc) rc = prepare_map(h, h->fallback_mode);
-  return rc;
+  if (rc) return rc;
+  h->sb_tiles = h->fallback_mode == BEAMS_SHADOW_REF_MARCH;
+  if (!h->sb_tiles) return BEAMS_OK;
+  BrickDesc d{};
+  d.g = g;
+  d.m = map_params(h, g, BEAMS_SHADOW_LEARNED);
+  d.s = s;
+  d.s.counters = nullptr;
+  d.map_final = h->map.p;
+  d.nverts = d.m.mp[0] * d.m.mp[1] * d.m.mp[2];
+  int64_t slots = 1, tiles = 1;
+  for (int a = 0; a < 3; ++a) {
+    const int64_t c = check_count(h->map_cells[a], h->check_stride);
+    slots *= c;
+    tiles *= c - 1;
+  }
+  slots = (slots + 255) / 256 * 256;
+  if (slots > INT32_MAX || tiles > INT32_MAX - 2)
+    return h->fail(BEAMS_ERR_UNSUPPORTED, "learned map: more than 2^31 check vertices or tiles");
+  const int32_t tab[4] = { 0, (int32_t)slots, 0, (int32_t)tiles };
+  std::vector<unsigned char> host(sizeof(d) + sizeof(tab));
+  std::memcpy(host.data(), &d, sizeof(d));
+  std::memcpy(host.data() + sizeof(d), tab, sizeof(tab));
+  HIP_TRY(h, h->sb_desc.ensure(sizeof(BrickDesc)));
+  HIP_TRY(h, h->sb_tab.ensure(sizeof(tab)));
+  HIP_TRY(h, h->sb_queue.ensure(5 * kQueueSlots * sizeof(unsigned int)));
+  HIP_TRY(h, h->sb_err.ensure(sizeof(float) * (size_t)slots));
+  HIP_TRY(h, h->sb_list.ensure(sizeof(int32_t) * (2 + (size_t)tiles)));
+  if (host != h->sb_host) {
+    HIP_TRY(h, hipStreamSynchronize(h->stream));  // a replay in flight may read them
+    HIP_TRY(h, hipMemcpy(h->sb_desc.p, &d, sizeof(d), hipMemcpyHostToDevice));
+    HIP_TRY(h, hipMemcpy(h->sb_tab.p, tab, sizeof(tab), hipMemcpyHostToDevice));
+    h->sb_host.swap(host);
+  }
+  BrickFrame& f = h->sb_f;
+  f = BrickFrame{};
+  f.bricks = (const BrickDesc*)h->sb_desc.p;
+  f.nbricks = 1;
+  f.queue = (unsigned int*)h->sb_queue.p;
+  f.persistent_groups = h->persistent_groups;
+  f.xcd_ranges = h->xcd_ranges;
+  LearnedBricks& lb = h->sb_lb;
+  lb.chk_off = (const int32_t*)h->sb_tab.p;
+  lb.tile_off = lb.chk_off + 2;
+  lb.verr = (float*)h->sb_err.p;
+  lb.list = (int32_t*)h->sb_list.p;
+  lb.n_chk_items = (int32_t)(slots / 256);
+  lb.n_tiles = (int32_t)tiles;
+  lb.stride = h->check_stride;
+  lb.budget = h->error_budget;
+  return BEAMS_OK;
 }
 
 // The learned map: coarse optical depth, the network's correction per vertex, the sparse exact
@@ -509,11 +568,21 @@ int prepare_learned(beams_renderer* h)
 int enqueue_learned(beams_renderer* h, const GridParams& g, const ShadeParams& s)
 {
   const MapParams m = map_params(h, g, BEAMS_SHADOW_LEARNED);
-  const LearnedParams p = learned_params(h, g, s, m);
+  LearnedParams p = learned_params(h, g, s, m);
+  p.verr = h->sb_tiles ? h->sb_lb.verr : nullptr;
   HIP_TRY(h, launch_coarse(p, h->stream));
   HIP_TRY(h, launch_learned_map(p, h->stream));
   HIP_TRY(h, launch_zero_u64((unsigned long long*)h->lerr.p, 1, h->stream));
   HIP_TRY(h, launch_learned_check(p, h->stream));
+  if (h->sb_tiles) {
+    // the tiles over the budget, re-marched with the reference march (a one-brick frame)
+    HIP_TRY(h, launch_zero_u32((unsigned int*)h->sb_lb.list, 2, h->stream));
+    HIP_TRY(h, launch_zero_u32(h->sb_f.queue + 4 * kQueueSlots, kQueueSlots, h->stream));
+    HIP_TRY(h, launch_learned_tiles(h->sb_f, h->sb_lb, h->stream));
+    return launch_bricks_exact_tiles(p, h->sb_f, h->map_dtype, h->sb_lb, h->stream) == hipSuccess
+               ? BEAMS_OK
+               : h->fail(BEAMS_ERR_HIP, "launch_bricks_exact_tiles failed");
+  }
   MapParams fm = map_params(h, g, h->fallback_mode);
   for (int d = 0; d < 3; ++d) fm.ms[d] = m.ms[d];
   fm.gate = (const unsigned int*)h->lerr.p;
@@ -524,7 +593,7 @@ int enqueue_learned(beams_renderer* h, const GridParams& g, const ShadeParams& s
 int build_map(beams_renderer* h, const GridParams& g, const ShadeParams& s)
 {
   if (h->shadow_mode == BEAMS_SHADOW_LEARNED) {
-    int rc = prepare_learned(h);
+    int rc = prepare_learned(h, g, s);
     if (!rc) rc = enqueue_learned(h, g, s);
     if (rc) return rc;
     learned_built(h, 1);
@@ -1108,6 +1177,11 @@ int beams_destroy(beams_renderer* h)
   h->lb_tab.release();
   h->lb_err.release();
   h->lb_list.release();
+  h->sb_desc.release();
+  h->sb_queue.release();
+  h->sb_tab.release();
+  h->sb_err.release();
+  h->sb_list.release();
   (void)hipStreamDestroy(h->stream);
   delete h;
   return BEAMS_OK;
@@ -1689,7 +1763,7 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
   // launch sequence and so of its graph.
   const bool with_map = h->use_shadow && h->assoc == 0;
   const bool learned = with_map && h->shadow_mode == BEAMS_SHADOW_LEARNED;
-  if (learned && (rc = prepare_learned(h))) return rc;
+  if (learned && (rc = prepare_learned(h, g, s))) return rc;
   if (with_map && !learned && (rc = prepare_map(h, h->shadow_mode))) return rc;
   const size_t npix = (size_t)width * (size_t)height;
   ImageOut o{};
@@ -1719,7 +1793,8 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
   KeyBuilder key;
   key.put(0).put(g).put(m).put(s).put(c).put(o).put(with_map).put(h->sweep_block);
   if (learned)
-    key.put(h->fallback_mode).put(h->error_budget).put(h->check_stride).put(h->net ? mlp_device_params(h->net) : nullptr);
+    key.put(h->fallback_mode).put(h->error_budget).put(h->check_stride).put(h->net ? mlp_device_params(h->net) : nullptr)
+       .put(h->sb_lb).put(h->sb_f);
   if ((rc = run_frame(h, key.bytes, issue))) return rc;
   if (with_map) {
     h->map_valid = true;
@@ -1832,6 +1907,11 @@ int beams_get_learned_stats(beams_renderer* h, float err[2], int32_t* fell_back)
       e[0] = bits_f32(w[0]);
       e[1] = bits_f32(w[1]);
       fb = !(e[0] <= h->built_budget);
+      if (h->sb_tiles) {
+        uint32_t t = 0;
+        HIP_TRY(h, hipMemcpy(&t, h->sb_list.p, sizeof(t), hipMemcpyDeviceToHost));
+        fb = (int32_t)t;
+      }
     } else {
       e[0] = bits_f32(w[1]);
       fb = (int32_t)w[0];

@@ -367,6 +367,7 @@ __global__ void __launch_bounds__(kBlock) k_learned_check(LearnedParams p, int64
   const float tc = fminf(coarse_lerp(p, p.coarse_tau, v), kTauMax);
   const float coarse = 1.0f - expf(-tc);
   const float learned = load_t<MDT>(m.map, id);
+  if (p.verr) p.verr[q] = fabsf(learned - exact);
   atomicMax(&p.err[0], f32_bits(fabsf(learned - exact)));
   atomicMax(&p.err[1], f32_bits(fabsf(coarse - exact)));
 }
@@ -622,6 +623,11 @@ hipError_t launch_bricks_learned_check(const LearnedParams& p, const BrickFrame&
     hipLaunchKernelGGL((k_bricks_learned_check<FDT, MDT>), grid, dim3(kBlock), lds, st, p, f, lb);
   }));
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  return launch_learned_tiles(f, lb, st);
+}
+
+hipError_t launch_learned_tiles(const BrickFrame& f, const LearnedBricks& lb, hipStream_t st)
+{
   hipLaunchKernelGGL(k_learned_tiles, dim3((unsigned)((lb.n_tiles + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
                      f, lb);
   return hipGetLastError();

@@ -40,6 +40,7 @@ struct LearnedParams {
   float inv_h;          // 1 / map step
   const float* params;  // network parameters (FP32 packed); null: coarse optical depth only
   unsigned int* err;    // [0] max |learned - exact|, [1] max |coarse only - exact| (float bits)
+  float* verr;          // optional: |learned - exact| per check vertex (the check tiles' input)
   int32_t check_stride; // check vertices: every check_stride-th map index per axis + the last
   int32_t field_small;  // the field's storage spans less than 4 GiB (32-bit byte offsets)
 };
@@ -83,6 +84,9 @@ hipError_t launch_bricks_learned_check(const LearnedParams& p, const BrickFrame&
                                        const LearnedBricks& lb, hipStream_t st);
 hipError_t launch_bricks_exact_tiles(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
                                      const LearnedBricks& lb, hipStream_t st);
+// The tile listing alone (the list's counters zeroed by the caller).  A single block uses it,
+// and the fallback above, as a frame of one brick whose map is the whole volume's.
+hipError_t launch_learned_tiles(const BrickFrame& f, const LearnedBricks& lb, hipStream_t st);
 
 hipError_t launch_mlp_infer(const float* params, const float* x, int64_t n, float* y, hipStream_t st);
 hipError_t launch_mlp_grad(const float* params, const float* x, const float* t, int64_t n,

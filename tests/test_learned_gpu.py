@@ -164,12 +164,12 @@ def test_learned_fallback_is_the_exact_path(oracle_mod):
         with B.LightedVolumeRenderer() as r:
             scenes.configure(r, sc)
             r.SetShadow(True, sc.map_cells, B.F32, mode=B.SHADOW_LEARNED)
-            r.SetLearned(net, fallback=B.SHADOW_REF_MARCH, error_budget=0.0, check_stride=8)
+            r.SetLearned(net, fallback=B.SHADOW_REF_MARCH, error_budget=-1.0, check_stride=8)
             o, d = B.camera_rays(sc.camera, sc.width, sc.height)
             rays = r.Render(B.Rays(o, d))
             gmap = r.GetMap(sc.map_cells)
             e_learned, _, fell = r.GetLearnedStats()
-    assert fell and e_learned > 0
+    assert fell == 2 ** 3 and e_learned > 0  # every check tile (16 cells per axis at stride 8)
     rb = O.RayBuf(o, d, "f32")
     O.render_rays(O.volume_from_scene(sc, "f32"), rb)
     assert np.array_equal(gmap, O.tmap_build(O.volume_from_scene(sc, "f32")))
@@ -240,8 +240,8 @@ def test_single_brick_learned_equals_one_block(budget):
     (i0, m0, s0), (i1, m1, s1) = out[False], out[True]
     assert np.array_equal(m0, m1)
     assert np.array_equal(i0, i1)
-    # the single block falls back as a whole, the brick tile by tile (8^3 tiles of 4^3 cells)
-    assert s0[0] == s1[0] and s0[2] == (1 if budget < 0 else 0) and s1[2] == (512 if budget < 0 else 0)
+    # both re-march tile by tile (8^3 tiles of 4^3 cells)
+    assert s0[0] == s1[0] and s0[2] == s1[2] == (512 if budget < 0 else 0)
 
 
 def _check_index(mp, stride):
@@ -352,10 +352,11 @@ def test_bricks_learned_fallback_per_tile(oracle_mod):
           f"max {err.max():.2e}; learned vs its fallback: max {np.abs(img_l - img_x).max():.2e}")
 
 
-@pytest.mark.parametrize("budget", [1.0, 0.0])
+@pytest.mark.parametrize("budget", [1.0, -1.0])
 def test_learned_frame_is_one_graph(oracle_mod, budget):
-    """Single block: the learned map, its check and the gated fallback are part of the frame's
-    graph; its replays equal the eager frame, and the zero budget's fallback map is the oracle's."""
+    """Single block: the learned map, its check, the tile list and the fallback are part of the
+    frame's graph; its replays equal the eager frame, and the forced fallback's map is the
+    oracle's."""
     O = oracle_mod
     sc = _scene(63, light="point")
     with B.ShadowMLP(seed=6) as net:
@@ -373,6 +374,41 @@ def test_learned_frame_is_one_graph(oracle_mod, budget):
     assert e_st == g_st and np.array_equal(e_map, g_map)
     for img in g_imgs:
         assert np.array_equal(img, e_imgs[0])
-    assert e_st[2] == (1 if budget == 0.0 else 0)
-    if budget == 0.0:
+    assert e_st[2] == (2 ** 3 if budget < 0 else 0)
+    if budget < 0:
         assert np.array_equal(g_map, O.tmap_build(O.volume_from_scene(sc, "f32")))
+
+
+def test_learned_single_block_tiles(oracle_mod):
+    """Single block, a budget between its tiles' errors: exactly the tiles over it (own or
+    neighbouring corners) hold the reference march's values, bit-equal to the oracle's map,
+    and the others keep the learned values; a ragged map (check tiles of up to 4^3 cells)."""
+    O = oracle_mod
+    sc = _scene(65, n=41, cells=37)
+    sc.map_cells = (37, 22, 29)
+    mp = np.array(sc.map_cells) + 1
+    stride = 4
+    exact = O.tmap_build(O.volume_from_scene(sc, "f32"))
+    with B.ShadowMLP(seed=6) as net:
+        net.set_params(_random_params(66) * 0.2)
+        maps = {}
+        for budget in ("learned", "mid"):
+            with B.LightedVolumeRenderer() as r:
+                scenes.configure(r, sc)
+                r.SetShadow(True, sc.map_cells, B.F32, mode=B.SHADOW_LEARNED)
+                if budget == "mid":
+                    _, terr, _ = _tile_errors(maps["learned"][0], exact, mp, stride)
+                    ordered = np.unique(terr)
+                    b = float(ordered[len(ordered) // 2 - 1] + ordered[len(ordered) // 2]) / 2
+                else:
+                    b = 1.0
+                r.SetLearned(net, fallback=B.SHADOW_REF_MARCH, error_budget=b, check_stride=stride)
+                r.RenderImage(sc.camera, sc.width, sc.height)
+                maps[budget] = (r.GetMap(sc.map_cells), r.GetLearnedStats(), b)
+    learned, (_, _, fell_l), _ = maps["learned"]
+    mixed, (_, _, fell_m), b = maps["mid"]
+    _, terr, (tz, ty, tx) = _tile_errors(learned, exact, mp, stride)
+    fail = terr > np.float32(b)
+    assert fell_l == 0 and fell_m == int(fail.sum()) and 0 < fail.sum() < fail.size
+    want = np.where(fail[np.ix_(tz, ty, tx)].ravel(), exact, learned)
+    assert np.array_equal(mixed, want)
