@@ -332,8 +332,10 @@ def series_exact(steps, warmup, n, img, timesteps, keep_first=True, schedule=Non
                      if k not in ("persistent_groups", "sweep_block", "march_groups_per_cu")]
             trials = [("all switches off", {k: 0 for k in knobs}), ("all switches on", {k: 1 for k in knobs})]
             trials += [(f"all on but {k}", {q: int(q != k) for q in knobs}) for k in knobs]
+            sched_of = {}
             for label, sw in trials:
                 r.SetSchedule(persistent_groups=2048, **sw)
+                sched_of[label] = r.GetSchedule()
                 r.RenderImage(sc.camera, img, img, rgba=rgba, depth=depth)
                 synchronize()
                 t0 = time.perf_counter()
@@ -341,8 +343,12 @@ def series_exact(steps, warmup, n, img, timesteps, keep_first=True, schedule=Non
                     r.RenderImage(sc.camera, img, img, rgba=rgba, depth=depth)
                 synchronize()
                 sched_ms[label] = (time.perf_counter() - t0) / 2 * 1e3
+            # the autotuner's descent starts from the fastest trial; the fastest of all is kept
+            best_trial = min(sched_of, key=lambda k: sched_ms[k])
+            r.SetSchedule(**sched_of[best_trial])
             _, sched_ms["autotuned"] = r.Autotune(sc.camera, img, img, frames=2)
-            schedule = r.GetSchedule()
+            sched_of["autotuned"] = r.GetSchedule()
+            schedule = sched_of[min(sched_of, key=lambda k: sched_ms[k])]
             print(f"[bench] series schedules: {sched_ms}, picked {schedule}", file=sys.stderr, flush=True)
         r.SetSchedule(**schedule)
         # eager frames with per-phase events (the first one also warms up)

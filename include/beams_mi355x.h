@@ -153,6 +153,9 @@ typedef struct beams_schedule {
                                 takes one launch per slice                                      */
   int32_t march_groups_per_cu; /* 1..16: at most that many march workgroups share a CU (a larger
                                   share of the L2 per screen tile); 0: no cap                   */
+  int32_t march_blocked;     /* 1: the brick march reads 4 x 4 x 4-blocked copies of the bricks'
+                                fields (made with the bricks) and final maps (made after the map
+                                phases), so a wave's samples touch fewer cache lines            */
 } beams_schedule;
 int beams_set_schedule(beams_renderer* h, const beams_schedule* s);
 int beams_get_schedule(const beams_renderer* h, beams_schedule* s);
@@ -173,7 +176,8 @@ int beams_set_composite_order(beams_renderer* h, int32_t order);
 int beams_set_graph(beams_renderer* h, int32_t enabled);
 
 /* Times `frames` frames of beams_render_image for candidate schedules of the brick launches
- * (coordinate descent: the persistent workgroup count, then each switch of beams_schedule),
+ * (coordinate descent from the current schedule: the persistent workgroup count, then each
+ * switch of beams_schedule but sweep_block, a flip kept only when it gains more than 0.5%),
  * keeps the fastest (beams_get_schedule reads it) and reports its workgroup count and time. */
 int beams_autotune(beams_renderer* h, const beams_camera* cam, int32_t width, int32_t height,
                    int32_t frames, int32_t* best_groups, float* best_ms);

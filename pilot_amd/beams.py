@@ -344,7 +344,7 @@ class LightedVolumeRenderer:
 
     SCHEDULE_KNOBS = ("persistent_groups", "xcd_ranges", "light_rows", "march_xcd_tiles",
                       "map_fast_fetch", "march_fast_fetch", "map_pipeline", "sweep_block",
-                      "march_groups_per_cu")
+                      "march_groups_per_cu", "march_blocked")
 
     def SetSchedule(self, **knobs):
         """Brick-launch schedule knobs (SCHEDULE_KNOBS; results are the same under every

@@ -135,12 +135,19 @@ struct beams_renderer {
     int64_t slo[3], sdims[3];  // stored region: owned points + ghost layers
     int64_t cells[3];          // map cells
     DevBuf field;
+    DevBuf field_blk;  // 4 x 4 x 4-blocked copy of the owned points (the blocked march's)
     int64_t nverts = 0, seed_off = 0;
+    void release()
+    {
+      field.release();
+      field_blk.release();
+    }
   };
   std::vector<Brick> bricks;
   std::vector<Brick> brick_pool;  // storage of the decomposition a new field replaced
   int32_t ghost = 0;
   DevBuf bmap_local, bmap_final, bseeds, btables, bpartial, bqueue;
+  DevBuf bmap_blk;  // 4 x 4 x 4-blocked copies of the final maps (the blocked march's)
   // learned bricks: check-slot and tile offsets (host copy and device table), per-check-vertex
   // errors, and the fallback's tile list
   std::vector<int32_t> lb_host;
@@ -157,10 +164,11 @@ struct beams_renderer {
   int32_t n_map_items = 0, n_px_items = 0, max_coords = 0;
   int32_t persistent_groups = 2048;
   // schedule knobs of the brick launches (the same results whatever their values)
-  int32_t xcd_ranges = 1, light_rows = 1, march_xcd_tiles = 1, map_fast_fetch = 0, march_fast_fetch = 1;
-  int32_t map_pipeline = 0;
+  int32_t xcd_ranges = 1, light_rows = 1, march_xcd_tiles = 0, map_fast_fetch = 1, march_fast_fetch = 1;
+  int32_t map_pipeline = 1;
   int32_t sweep_block = 1;
   int32_t march_groups_per_cu = 0;
+  int32_t march_blocked = 1;
   int32_t composite_order = BEAMS_ORDER_CENTER_DISTANCE;
   DevBuf brows;                          // every brick's map rows in light-sheet order
   std::vector<unsigned char> brows_key;  // the light and brick maps the rows were ordered for
@@ -754,7 +762,8 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
   std::vector<BrickDesc> desc((size_t)nb);
   std::vector<int32_t> item_off((size_t)nb + 1, 0), px_brick, px_tile, order((size_t)nb);
   std::vector<float> depth((size_t)nb);
-  int64_t row_total = 0;
+  int64_t row_total = 0, blk_total = 0;
+  bool blk_ok = true;
   total_verts = 0;
   partial_pixels = 0;
   int32_t max_coords = 0;
@@ -813,6 +822,23 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
     m.dir_far = 4.0f * whole.ext_mag;  // one light origin for all bricks
     m.map = (unsigned char*)h->bmap_local.p + msz * (size_t)br.seed_off;
     d.map_final = (unsigned char*)h->bmap_final.p + msz * (size_t)br.seed_off;
+    {
+      int64_t ybs, zbs;
+      blocked_strides(g.n, ybs, zbs);
+      d.field_blk = br.field_blk.p;
+      d.fblk_ybs = ybs;
+      d.fblk_zbs = zbs;
+      const uint64_t fb = (uint64_t)esz * (uint64_t)blocked_count(g.n);
+      d.fblk_bytes = fb < ((uint64_t)1 << 32) ? (uint32_t)fb : 0u;
+      blocked_strides(m.mp, ybs, zbs);
+      d.map_blk = (unsigned char*)h->bmap_blk.p + msz * (size_t)blk_total;
+      d.mblk_ybs = ybs;
+      d.mblk_zbs = zbs;
+      const uint64_t mb = (uint64_t)msz * (uint64_t)blocked_count(m.mp);
+      d.mblk_bytes = mb < ((uint64_t)1 << 32) ? (uint32_t)mb : 0u;
+      blk_total += blocked_count(m.mp);
+      blk_ok = blk_ok && d.fblk_bytes != 0 && d.mblk_bytes != 0;
+    }
     // per-rank Render() state: its own extent sets the mesh epsilon (and the default step)
     ShadeParams& s = d.s;
     s = s0;
@@ -919,6 +945,7 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
   f.march_fast = fast_ok && h->march_fast_fetch;
   f.map_pipeline = h->map_pipeline;
   f.march_groups_per_cu = h->march_groups_per_cu;
+  f.march_blocked = h->march_blocked && blk_ok;
   h->n_map_items = f.n_map_items;
   h->n_px_items = f.n_px_items;
   h->max_coords = max_coords;
@@ -1042,6 +1069,12 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
   for (const auto& br : h->bricks) nv_total += br.nverts;
   HIP_TRY(h, h->bmap_local.ensure((size_t)nv_total * msz));
   HIP_TRY(h, h->bmap_final.ensure((size_t)nv_total * msz));
+  int64_t blk_total = 0;
+  for (const auto& br : h->bricks) {
+    const int64_t mp[3] = { br.cells[0] + 1, br.cells[1] + 1, br.cells[2] + 1 };
+    blk_total += blocked_count(mp);
+  }
+  HIP_TRY(h, h->bmap_blk.ensure((size_t)blk_total * msz));
   HIP_TRY(h, h->bseeds.ensure((size_t)nv_total * sizeof(float)));
   HIP_TRY(h, h->bqueue.ensure(5 * kQueueSlots * sizeof(unsigned int)));
   if ((rc = frame_begin(h))) return rc;
@@ -1087,6 +1120,7 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
       if ((r = mark(BEAMS_PHASE_REMAP, true))) return r;
     }
     if ((r = mark(BEAMS_PHASE_MARCH, false))) return r;
+    if (shadows && f.march_blocked) HIP_TRY(h, launch_block_maps(f, h->map_dtype, h->stream));
     HIP_TRY(h, launch_bricks_march(f, h->dtype, h->map_dtype, ntf, shadows, c, h->stream));
     if ((r = mark(BEAMS_PHASE_MARCH, true))) return r;
     if ((r = mark(BEAMS_PHASE_COMPOSITE, false))) return r;
@@ -1165,10 +1199,11 @@ int beams_destroy(beams_renderer* h)
       if (e) (void)hipEventDestroy(e);
   h->counters.release();
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-  for (auto& br : h->bricks) br.field.release();
-  for (auto& br : h->brick_pool) br.field.release();
+  for (auto& br : h->bricks) br.release();
+  for (auto& br : h->brick_pool) br.release();
   h->bmap_local.release();
   h->bmap_final.release();
+  h->bmap_blk.release();
   h->bseeds.release();
   h->btables.release();
   h->bpartial.release();
@@ -1305,7 +1340,7 @@ int beams_set_field(beams_renderer* h, const void* values, int32_t dtype, int32_
   h->map_valid = false;
   // A new field drops the brick decomposition; its storage waits in the pool for the next
   // beams_set_bricks, which reuses it where the stored regions match.
-  for (auto& br : h->brick_pool) br.field.release();
+  for (auto& br : h->brick_pool) br.release();
   h->brick_pool.swap(h->bricks);
   h->bricks.clear();
   return BEAMS_OK;
@@ -1316,7 +1351,7 @@ int beams_set_bricks(beams_renderer* h, int32_t nbricks, const int64_t* point_ra
 {
   CHECK_HANDLE(h);
   if (nbricks == 0) {
-    for (auto& br : h->bricks) br.field.release();
+    for (auto& br : h->bricks) br.release();
     h->bricks.clear();
     h->btab_host.clear();
     return BEAMS_OK;
@@ -1354,25 +1389,31 @@ int beams_set_bricks(beams_renderer* h, int32_t nbricks, const int64_t* point_ra
   int rc = BEAMS_OK;
   for (int b = 0; b < nbricks && rc == BEAMS_OK; ++b) {
     auto& br = bricks[(size_t)b];
-    if ((size_t)b < prev.size()) std::swap(br.field, prev[(size_t)b].field);
-    const hipError_t e = br.field.ensure(esz * (size_t)(br.sdims[0] * br.sdims[1] * br.sdims[2]));
-    const hipError_t x = e != hipSuccess ? e
-                                         : launch_extract_brick(h->field.p, (int32_t)esz, h->dims, br.slo,
-                                                                br.sdims, br.field.p, h->stream);
+    if ((size_t)b < prev.size()) {
+      std::swap(br.field, prev[(size_t)b].field);
+      std::swap(br.field_blk, prev[(size_t)b].field_blk);
+    }
+    const int64_t own[3] = { br.hi[0] - br.lo[0] + 1, br.hi[1] - br.lo[1] + 1, br.hi[2] - br.lo[2] + 1 };
+    hipError_t e = br.field.ensure(esz * (size_t)(br.sdims[0] * br.sdims[1] * br.sdims[2]));
+    if (e == hipSuccess) e = br.field_blk.ensure(esz * (size_t)blocked_count(own));
+    hipError_t x = e != hipSuccess ? e
+                                   : launch_extract_brick(h->field.p, (int32_t)esz, h->dims, br.slo, br.sdims,
+                                                          br.field.p, h->stream);
+    if (x == hipSuccess) x = launch_block_extract(h->field.p, (int32_t)esz, h->dims, br.lo, own, br.field_blk.p, h->stream);
     if (x != hipSuccess) rc = h->fail(BEAMS_ERR_HIP, std::string("bricks: ") + hipGetErrorString(x));
   }
   if (rc == BEAMS_OK && hipStreamSynchronize(h->stream) != hipSuccess)
     rc = h->fail(BEAMS_ERR_HIP, "bricks: extraction failed");
-  for (auto& br : prev) br.field.release();
+  for (auto& br : prev) br.release();
   prev.clear();
   if (rc != BEAMS_OK) {
-    for (auto& br : bricks) br.field.release();
+    for (auto& br : bricks) br.release();
     h->bricks.clear();
     h->btab_host.clear();
     h->map_valid = false;
     return rc;
   }
-  for (auto& br : h->brick_pool) br.field.release();
+  for (auto& br : h->brick_pool) br.release();
   h->brick_pool.clear();
   // the brick tables are rebuilt every frame and uploaded only when their bytes change
   h->bricks.swap(bricks);
@@ -1427,6 +1468,7 @@ int beams_set_schedule(beams_renderer* h, const beams_schedule* s)
   h->map_pipeline = s->map_pipeline != 0;
   h->sweep_block = s->sweep_block != 0;
   h->march_groups_per_cu = s->march_groups_per_cu;
+  h->march_blocked = s->march_blocked != 0;
   return BEAMS_OK;
 }
 
@@ -1442,6 +1484,7 @@ int beams_get_schedule(const beams_renderer* h, beams_schedule* s)
   s->map_pipeline = h->map_pipeline;
   s->sweep_block = h->sweep_block;
   s->march_groups_per_cu = h->march_groups_per_cu;
+  s->march_blocked = h->march_blocked;
   return BEAMS_OK;
 }
 
@@ -1514,13 +1557,14 @@ int beams_autotune(beams_renderer* h, const beams_camera* cam, int32_t width, in
         best = s;
       }
     }
-    for (int knob = 0; knob < 6 && !rc; ++knob) {
+    // each switch flipped in turn, kept only for a gain above 0.5% (the timing's spread)
+    for (int knob = 0; knob < 7 && !rc; ++knob) {
       beams_schedule s = best;
-      int32_t* const v[6] = { &s.xcd_ranges, &s.light_rows, &s.march_xcd_tiles, &s.map_fast_fetch,
-                              &s.march_fast_fetch, &s.map_pipeline };
+      int32_t* const v[7] = { &s.xcd_ranges, &s.light_rows, &s.march_xcd_tiles, &s.map_fast_fetch,
+                              &s.march_fast_fetch, &s.map_pipeline, &s.march_blocked };
       *v[knob] = !*v[knob];
       const float t = time_schedule(s);
-      if (t < best_t) {
+      if (t < best_t * 0.995f) {
         best_t = t;
         best = s;
       }

@@ -455,7 +455,7 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
 }
 
 // ---- phase 4: CalcRayStart + Sampler per brick over its screen footprint ---------------------
-template <int FDT, int MDT, bool USE_MAP, bool FAST>
+template <int FDT, int MDT, bool USE_MAP, bool FAST, bool BLK>
 __global__ void __launch_bounds__(256) k_bricks_march(BrickFrame f, CameraParams cam)
 {
   // One workgroup per (brick, 16x16 screen tile) item.  Not persistent: a loop over items
@@ -475,10 +475,21 @@ __global__ void __launch_bounds__(256) k_bricks_march(BrickFrame f, CameraParams
   // scalar loads (constant address space) into scalar registers, not into every lane
   const int b = __builtin_amdgcn_readfirstlane(f.px_item_brick[item]);
   const BrickDesc* bk = f.bricks + b;
-  const GridParams g = load_uniform(&bk->g);
+  GridParams g = load_uniform(&bk->g);
   const ShadeParams sp = load_uniform(&bk->s);
   MapParams m = load_uniform(&bk->m);
   m.map = bk->map_final;
+  if (BLK) {
+    // the blocked copies: same points, the block strides in place of the row and slice strides
+    g.field = bk->field_blk;
+    g.bytes32 = bk->fblk_bytes;
+    g.field_stride[1] = bk->fblk_ybs;
+    g.field_stride[2] = bk->fblk_zbs;
+    m.map = bk->map_blk;
+    m.bytes32 = bk->mblk_bytes;
+    m.ms[1] = bk->mblk_ybs;
+    m.ms[2] = bk->mblk_zbs;
+  }
   const int fp0 = bk->fp[0], fp1 = bk->fp[1], fp2 = bk->fp[2], fp3 = bk->fp[3];
   const int64_t img_off = bk->img_off;
   stage_tf(sp, tf);
@@ -496,10 +507,47 @@ __global__ void __launch_bounds__(256) k_bricks_march(BrickFrame f, CameraParams
   float minD = cam.near_d, maxD = cam.far_d, dist = 0.0f;
   calc_ray_start(g, org, dir, minD, dist, maxD);
   float color[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
-  shade<FDT, MDT, USE_MAP, false, FAST>(g, m, sp, tf, x, org, dir, minD, maxD, color);
+  shade<FDT, MDT, USE_MAP, false, FAST, BLK>(g, m, sp, tf, x, org, dir, minD, maxD, color);
   const int64_t pix = img_off + (int64_t)ly * fp2 + lx;
   f.partial[pix] = make_float4(color[0], color[1], color[2], color[3]);
   f.partial_depth[pix] = (minD == -1.0f) ? __builtin_inff() : minD;
+}
+
+// ---- blocked copies for the march ---------------------------------------------------------
+__global__ void __launch_bounds__(256) k_block_extract(const unsigned char* src, int32_t esz, int64_t sx,
+                                                       int64_t sxy, int64_t lo0, int64_t lo1, int64_t lo2,
+                                                       int64_t n0, int64_t n1, int64_t n2, int64_t ybs, int64_t zbs,
+                                                       unsigned char* dst)
+{
+  const int64_t total = n0 * n1 * n2;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = q % n0, j = (q / n0) % n1, k = q / (n0 * n1);
+    const int64_t s = (lo0 + i) + (lo1 + j) * sx + (lo2 + k) * sxy;
+    const int64_t d = blocked_index(i, j, k, ybs, zbs);
+    if (esz == 4)
+      ((uint32_t*)dst)[d] = ((const uint32_t*)src)[s];
+    else
+      ((uint16_t*)dst)[d] = ((const uint16_t*)src)[s];
+  }
+}
+
+// One map item (256 vertices of one brick) per workgroup iteration: map_final -> map_blk.
+template <int MDT>
+__global__ void __launch_bounds__(256) k_block_maps(BrickFrame f)
+{
+  for (int item = blockIdx.x; item < f.n_map_items; item += gridDim.x) {
+    const int b = item_brick(f, item);
+    const BrickDesc* bk = f.bricks + b;
+    const int64_t id = (int64_t)(item - f.map_item_off[b]) * kMapItem + threadIdx.x;
+    if (id >= bk->nverts) continue;
+    const int64_t mp0 = bk->m.mp[0], mp1 = bk->m.mp[1];
+    const int64_t i = id % mp0, j = (id / mp0) % mp1, k = id / (mp0 * mp1);
+    const int64_t d = blocked_index(i, j, k, bk->mblk_ybs, bk->mblk_zbs);
+    if (MDT == kF32)
+      ((uint32_t*)bk->map_blk)[d] = ((const uint32_t*)bk->map_final)[id];
+    else
+      ((uint16_t*)bk->map_blk)[d] = ((const uint16_t*)bk->map_final)[id];
+  }
 }
 
 // ---- phase 5: GlobalComposite, front-to-back over in camera-distance order -------------------
@@ -614,14 +662,18 @@ hipError_t launch_bricks_march(const BrickFrame& f, int32_t field_dtype, int32_t
   if (f.march_groups_per_cu > 0) lds = std::max(lds, (size_t)(160 * 1024) / (size_t)f.march_groups_per_cu - 512);
   const dim3 grid((unsigned)f.n_px_items);
   BRICK_DISPATCH_FIELD(field_dtype, BRICK_DISPATCH_MAP(map_dtype, {
-    if (use_map && f.march_fast)
-      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, true, true>), grid, dim3(256), lds, st, f, c);
+    if (f.march_blocked && use_map)
+      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, true, true, true>), grid, dim3(256), lds, st, f, c);
+    else if (f.march_blocked)
+      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, false, true, true>), grid, dim3(256), lds, st, f, c);
+    else if (use_map && f.march_fast)
+      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, true, true, false>), grid, dim3(256), lds, st, f, c);
     else if (use_map)
-      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, true, false>), grid, dim3(256), lds, st, f, c);
+      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, true, false, false>), grid, dim3(256), lds, st, f, c);
     else if (f.march_fast)
-      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, false, true>), grid, dim3(256), lds, st, f, c);
+      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, false, true, false>), grid, dim3(256), lds, st, f, c);
     else
-      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, false, false>), grid, dim3(256), lds, st, f, c);
+      hipLaunchKernelGGL((k_bricks_march<FDT, MDT, false, false, false>), grid, dim3(256), lds, st, f, c);
   }));
   return hipGetLastError();
 }
@@ -632,6 +684,29 @@ hipError_t launch_bricks_composite(const BrickFrame& f, const CameraParams& c, c
   const int64_t npix = (int64_t)c.width * c.height;
   hipLaunchKernelGGL(k_bricks_composite, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, f,
                      c.width, c.height, o);
+  return hipGetLastError();
+}
+
+hipError_t launch_block_extract(const void* src, int32_t elem_bytes, const int64_t src_dims[3],
+                                const int64_t lo[3], const int64_t n[3], void* dst, hipStream_t st)
+{
+  int64_t ybs, zbs;
+  blocked_strides(n, ybs, zbs);
+  const int64_t total = n[0] * n[1] * n[2];
+  const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 65536)));
+  hipLaunchKernelGGL(k_block_extract, grid, dim3(256), 0, st, (const unsigned char*)src, elem_bytes, src_dims[0],
+                     src_dims[0] * src_dims[1], lo[0], lo[1], lo[2], n[0], n[1], n[2], ybs, zbs,
+                     (unsigned char*)dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_block_maps(const BrickFrame& f, int32_t map_dtype, hipStream_t st)
+{
+  const dim3 grid((unsigned)std::max(1, std::min(f.n_map_items, 4096)));
+  if (map_dtype == kF32)
+    hipLaunchKernelGGL((k_block_maps<kF32>), grid, dim3(256), 0, st, f);
+  else
+    hipLaunchKernelGGL((k_block_maps<kF16>), grid, dim3(256), 0, st, f);
   return hipGetLastError();
 }
 

@@ -16,6 +16,12 @@ struct BrickDesc {
   int64_t row_off;     // first entry of this brick's map rows in light-sheet order
   int32_t fp[4];       // screen footprint: x0, y0, width, height (pixels)
   float box32[6];      // bounds inflated for the conservative FP32 cull
+  // 4 x 4 x 4-blocked copies read by the march when march_blocked is set (BlockFetch): the
+  // owned points of the field (made with the bricks) and map_final (made after the map phases)
+  const void* field_blk;
+  void* map_blk;
+  int64_t fblk_ybs, fblk_zbs, mblk_ybs, mblk_zbs;
+  uint32_t fblk_bytes, mblk_bytes;
 };
 
 // Work-queue counters per persistent phase (one per XCD).
@@ -50,6 +56,7 @@ struct BrickFrame {
                                   // brick's field and map view below 2^32 bytes, strides below 2^24)
   int32_t map_pipeline;           // map launches: next light step's loads in flight (march_vertex_pipe)
   int32_t march_groups_per_cu;    // march launch: at most this many workgroups per CU (0: no cap)
+  int32_t march_blocked;          // march launch: reads the blocked copies (field_blk, map_blk)
 };
 
 hipError_t launch_bricks_map(const BrickFrame& f, int32_t field_dtype, int32_t map_dtype,
@@ -64,5 +71,11 @@ hipError_t launch_extract_brick(const void* src, int32_t elem_bytes, const int64
                                 const int64_t lo[3], const int64_t dims[3], void* dst,
                                 hipStream_t st);
 hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t st);
+// The owned points lo .. lo + n - 1 of a field (x fastest, dims src_dims) into a 4 x 4 x 4-blocked
+// copy (blocked_index with the strides of n).
+hipError_t launch_block_extract(const void* src, int32_t elem_bytes, const int64_t src_dims[3],
+                                const int64_t lo[3], const int64_t n[3], void* dst, hipStream_t st);
+// Every brick's map_final into its blocked copy map_blk.
+hipError_t launch_block_maps(const BrickFrame& f, int32_t map_dtype, hipStream_t st);
 
 }  // namespace beams

@@ -64,6 +64,22 @@ inline uint32_t map_bytes32(const int64_t* mp, int32_t dtype)
   return b < ((uint64_t)1 << 32) ? (uint32_t)b : 0u;
 }
 
+// Host/device: the element count of a 4 x 4 x 4-blocked copy of n points per axis, and its
+// block strides (ybs, zbs).
+BEAMS_HD int64_t blocked_count(const int64_t* n)
+{
+  return ((n[0] + 3) / 4) * ((n[1] + 3) / 4) * ((n[2] + 3) / 4) * 64;
+}
+BEAMS_HD void blocked_strides(const int64_t* n, int64_t& ybs, int64_t& zbs)
+{
+  ybs = ((n[0] + 3) / 4) * 64;
+  zbs = ybs * ((n[1] + 3) / 4);
+}
+BEAMS_HD int64_t blocked_index(int64_t i, int64_t j, int64_t k, int64_t ybs, int64_t zbs)
+{
+  return (i >> 2) * 64 + (i & 3) + (j >> 2) * ybs + (j & 3) * 4 + (k >> 2) * zbs + (k & 3) * 16;
+}
+
 struct ShadeParams {
   const float* tf;     // ntf x RGBA (device)
   int32_t ntf;

@@ -382,6 +382,60 @@ __device__ __forceinline__ int64_t walk_cell(const float* x, int64_t ncell, floa
 // The map cell of the previous lookup and its corner opacities: consecutive samples of a ray
 // usually share a map cell, whose corners need not be gathered again.  Same result as
 // map_opacity (the values are the same loads; the interpolation is recomputed each time).
+// Corner fetch from a 4 x 4 x 4-blocked copy of a grid (64 values per block, a 128-byte line
+// for FP16): point (i, j, k) sits at element
+//     X(i) + Y(j) + Z(k),  X(i) = 64 (i >> 2) + (i & 3),  Y(j) = ybs (j >> 2) + 4 (j & 3),
+//     Z(k) = zbs (k >> 2) + 16 (k & 3),
+// with ybs = 64 x-blocks and zbs = 64 x-blocks y-blocks elements.  A wave's rays sample a few
+// voxels across in y and z, which the x-fastest layout spreads over a line per (y, z) row and
+// the blocked one keeps within a few lines.  The copy is below 2^32 bytes (one buffer
+// resource, 32-bit offsets); ybs and zbs come in the stride arguments.
+template <int FDT>
+struct BlockFetch {
+  __amdgpu_buffer_rsrc_t rsrc;
+  uint32_t ybs, zbs;
+  __device__ __forceinline__ explicit BlockFetch(const void* f, uint32_t bytes32, int, int ybs_, int zbs_)
+  {
+    const uint64_t fp = (uint64_t)f;
+    const uint32_t plo = __builtin_amdgcn_readfirstlane((uint32_t)fp);
+    const uint32_t phi = __builtin_amdgcn_readfirstlane((uint32_t)(fp >> 32));
+    const void* field = (const void*)(((uint64_t)phi << 32) | plo);
+    bytes32 = __builtin_amdgcn_readfirstlane(bytes32);
+    ybs = __builtin_amdgcn_readfirstlane((uint32_t)ybs_);
+    zbs = __builtin_amdgcn_readfirstlane((uint32_t)zbs_);
+    rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(field), (short)0, (int)bytes32, 0x00020000);
+  }
+  __device__ __forceinline__ float buf(uint32_t e) const
+  {
+    const int off = (int)(e * (uint32_t)(FDT == kF32 ? 4 : 2));
+    if (FDT == kF32) return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0));
+    const uint16_t h = __builtin_amdgcn_raw_buffer_load_b16(rsrc, off, 0, 0);
+    if (FDT == kF16) {
+      float r;
+      asm("v_cvt_f32_f16 %0, %1" : "=v"(r) : "v"((uint32_t)h));
+      return r;
+    }
+    return bf16_to_f32(h);
+  }
+  __device__ __forceinline__ void load(int ci, int cj, int ck, float* v) const
+  {
+    const uint32_t i = (uint32_t)ci, j = (uint32_t)cj, k = (uint32_t)ck;
+    const uint32_t base = ((i >> 2) << 6) + (i & 3u) + __umul24(j >> 2, ybs) + ((j & 3u) << 2) +
+                          __umul24(k >> 2, zbs) + ((k & 3u) << 4);
+    const uint32_t dx = (i & 3u) == 3u ? 61u : 1u;            // the next x: the next block's first
+    const uint32_t dy = (j & 3u) == 3u ? ybs - 12u : 4u;
+    const uint32_t dz = (k & 3u) == 3u ? zbs - 48u : 16u;
+    v[0] = buf(base);
+    v[1] = buf(base + dx);
+    v[2] = buf(base + dx + dy);
+    v[3] = buf(base + dy);
+    v[4] = buf(base + dz);
+    v[5] = buf(base + dz + dx);
+    v[6] = buf(base + dz + dx + dy);
+    v[7] = buf(base + dz + dy);
+  }
+};
+
 struct MapCellCache {
   int c[3] = { -1, -1, -1 };
   float lo[3], w[3], rw[3];  // the cell's lower coordinates, widths and reciprocal widths
@@ -392,9 +446,9 @@ struct MapCellCache {
 // is not the cached one, issues the corner loads (the camera march calls it before the field's
 // own loads, so both sets are in flight together); map_cell_opacity interpolates in the cell.
 // Same result as map_opacity: the same corner values and parametric coordinates.
-template <int MDT, bool FAST = false>
-__device__ __forceinline__ void map_cell_update(const MapParams& m, const CornerFetch<MDT, FAST>& mf,
-                                                const float* p, MapCellCache& mc)
+template <int MDT, class Fetch>
+__device__ __forceinline__ void map_cell_update(const MapParams& m, const Fetch& mf, const float* p,
+                                                MapCellCache& mc)
 {
   int cell[3];
 #pragma unroll

@@ -2,6 +2,8 @@
 // kernels (march.hip) and the brick kernels (bricks.hip).
 #pragma once
 
+#include <type_traits>
+
 #include "march_core.hpp"
 
 namespace beams {
@@ -228,7 +230,9 @@ __device__ __forceinline__ float march_vertex(const GridParams& g, const MapPara
 }
 
 // ---- LightedVolumeRenderer.cxx:345-498 ----------------------------------------------------
-template <int FDT, int MDT, bool USE_MAP, bool BINS = false, bool FAST = false>
+// BLK: g.field and m.map are 4 x 4 x 4-blocked copies (BlockFetch; their block strides in
+// field_stride[1..2] and ms[1..2]).
+template <int FDT, int MDT, bool USE_MAP, bool BINS = false, bool FAST = false, bool BLK = false>
 __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapParams& m,
                                              const ShadeParams& s, const float4* tf,
                                              const float* const* x, const float* org,
@@ -259,13 +263,15 @@ __device__ __forceinline__ int64_t sample_point(const GridParams& g, const MapPa
   // 32-bit cells and strides: set_field keeps a field's xy plane below 2^31 points
   const int sx = (int)g.field_stride[0], sy = (int)g.field_stride[1], sz = (int)g.field_stride[2];
   const int rofs = coords_rofs(g);
-  const CornerFetch<FDT, FAST> corners(g.field, g.bytes32, sx, sy, sz);
-  const CornerFetch<MDT, FAST> map_corners(m.map, m.bytes32, (int)m.ms[0], (int)m.ms[1], (int)m.ms[2]);
+  typedef typename std::conditional<BLK, BlockFetch<FDT>, CornerFetch<FDT, FAST>>::type FieldFetch;
+  typedef typename std::conditional<BLK, BlockFetch<MDT>, CornerFetch<MDT, FAST>>::type MapFetch;
+  const FieldFetch corners(g.field, g.bytes32, sx, sy, sz);
+  const MapFetch map_corners(m.map, m.bytes32, (int)m.ms[0], (int)m.ms[1], (int)m.ms[2]);
   const float sd = s.sample_dist;
   MapCellCache mcache;
   while (inside(g, p) && dist < maxD && ++steps < max_steps) {
     // the shadow map's cell first: its corner loads, if any, overlap the field's
-    if (USE_MAP) map_cell_update<MDT, FAST>(m, map_corners, p, mcache);
+    if (USE_MAP) map_cell_update<MDT>(m, map_corners, p, mcache);
     const float mint = fminf(tx, fminf(ty, tz));
     const float maxt = fmaxf(tx, fmaxf(ty, tz));
     if (maxt > 1.0f || mint < 0.0f) newCell = true;
@@ -432,7 +438,7 @@ __device__ __forceinline__ void stage_tf(const ShadeParams& s, float4* tf)
     tf[i] = make_float4(s.tf[4 * i], s.tf[4 * i + 1], s.tf[4 * i + 2], s.tf[4 * i + 3]);
 }
 
-template <int FDT, int MDT, bool USE_MAP, bool BINS = false, bool FAST = false>
+template <int FDT, int MDT, bool USE_MAP, bool BINS = false, bool FAST = false, bool BLK = false>
 __device__ __forceinline__ void shade(const GridParams& g, const MapParams& m,
                                       const ShadeParams& s, const float4* tf,
                                       const float* const* x, const float* org, const float* dir,
@@ -440,7 +446,7 @@ __device__ __forceinline__ void shade(const GridParams& g, const MapParams& m,
 {
   const int64_t n = (g.assoc == 1)
                       ? sample_cell<FDT>(g, s, tf, x, org, dir, minD, maxD, color)
-                      : sample_point<FDT, MDT, USE_MAP, BINS, FAST>(g, m, s, tf, x, org, dir, minD, maxD, color);
+                      : sample_point<FDT, MDT, USE_MAP, BINS, FAST, BLK>(g, m, s, tf, x, org, dir, minD, maxD, color);
   if (s.counters) count_add(&s.counters[0], (uint32_t)n);
 }
 

@@ -135,16 +135,16 @@ def test_bricks_small_persistent_grids(oracle_mod, groups):
     assert np.array_equal(depth.reshape(-1), ref_depth)
 
 
-@pytest.mark.parametrize("knobs", [(0, 0, 0, 0, 0, 0, 0), (1, 0, 0, 1, 0, 1, 2), (0, 1, 0, 0, 1, 0, 1),
-                                   (0, 0, 1, 1, 1, 1, 0), (1, 1, 1, 0, 0, 1, 4), (1, 1, 1, 1, 1, 0, 0)])
+@pytest.mark.parametrize("knobs", [(0, 0, 0, 0, 0, 0, 0, 0), (1, 0, 0, 1, 0, 1, 2, 1), (0, 1, 0, 0, 1, 0, 1, 0),
+                                   (0, 0, 1, 1, 1, 1, 0, 1), (1, 1, 1, 0, 0, 1, 4, 0), (1, 1, 1, 1, 1, 0, 0, 1)])
 def test_bricks_schedules_give_the_same_frame(oracle_mod, knobs):
     """Every schedule of the brick launches (XCD ranges in the persistent queues, light-sheet
     row order in the map launches, XCD-contiguous march tiles, the branch-free corner fetch of
     the map and march launches) renders the oracle's local and final maps and image bit for
     bit, under a point light off every axis and a directional light along an axis (the sheet
-    grouping's degenerate case).  The map launches' pipelined light march and the cap on
-    march workgroups per CU are among them."""
-    xcd, rows, tiles, mapf, marchf, pipe, per_cu = knobs
+    grouping's degenerate case).  The map launches' pipelined light march, the cap on march
+    workgroups per CU and the march over blocked copies are among them."""
+    xcd, rows, tiles, mapf, marchf, pipe, per_cu, blocked = knobs
     for light, pos in (("point", None), ("directional", (0.0, 0.0, -1.0))):
         sc = scenes.make_scene("sched", 45, seed=23, nblobs=10, width=72, height=56, light=light)
         if pos is not None:
@@ -156,7 +156,7 @@ def test_bricks_schedules_give_the_same_frame(oracle_mod, knobs):
             r.SetBricks(ranges, ghost=1)
             want = {"persistent_groups": 96, "xcd_ranges": xcd, "light_rows": rows, "march_xcd_tiles": tiles,
                     "map_fast_fetch": mapf, "march_fast_fetch": marchf, "map_pipeline": pipe,
-                    "march_groups_per_cu": per_cu}
+                    "march_groups_per_cu": per_cu, "march_blocked": blocked}
             r.SetSchedule(**want)
             got = r.GetSchedule()
             assert {k: got[k] for k in want} == want
