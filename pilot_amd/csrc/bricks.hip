@@ -531,22 +531,26 @@ __global__ void __launch_bounds__(256) k_block_extract(const unsigned char* src,
   }
 }
 
-// One map item (256 vertices of one brick) per workgroup iteration: map_final -> map_blk.
+// map_final -> map_blk in destination order (blockIdx.y: the brick): consecutive threads write
+// consecutive elements of the blocked copy, whole lines at a time, and gather their values from
+// the x-fastest map (four consecutive x per row, rows the L2 holds for the neighbouring blocks).
 template <int MDT>
 __global__ void __launch_bounds__(256) k_block_maps(BrickFrame f)
 {
-  for (int item = blockIdx.x; item < f.n_map_items; item += gridDim.x) {
-    const int b = item_brick(f, item);
-    const BrickDesc* bk = f.bricks + b;
-    const int64_t id = (int64_t)(item - f.map_item_off[b]) * kMapItem + threadIdx.x;
-    if (id >= bk->nverts) continue;
-    const int64_t mp0 = bk->m.mp[0], mp1 = bk->m.mp[1];
-    const int64_t i = id % mp0, j = (id / mp0) % mp1, k = id / (mp0 * mp1);
-    const int64_t d = blocked_index(i, j, k, bk->mblk_ybs, bk->mblk_zbs);
+  const BrickDesc* bk = f.bricks + blockIdx.y;
+  const int64_t mp0 = bk->m.mp[0], mp1 = bk->m.mp[1], mp2 = bk->m.mp[2];
+  const int64_t nbx = (mp0 + 3) / 4, nby = (mp1 + 3) / 4;
+  const int64_t count = nbx * nby * ((mp2 + 3) / 4) * 64;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < count; e += (int64_t)gridDim.x * 256) {
+    const int64_t blk = e >> 6, w = e & 63;
+    const int64_t i = (blk % nbx) * 4 + (w & 3), j = ((blk / nbx) % nby) * 4 + ((w >> 2) & 3),
+                  k = (blk / (nbx * nby)) * 4 + (w >> 4);
+    if (i >= mp0 || j >= mp1 || k >= mp2) continue;  // padding of the last blocks
+    const int64_t id = i + mp0 * (j + mp1 * k);
     if (MDT == kF32)
-      ((uint32_t*)bk->map_blk)[d] = ((const uint32_t*)bk->map_final)[id];
+      ((uint32_t*)bk->map_blk)[e] = ((const uint32_t*)bk->map_final)[id];
     else
-      ((uint16_t*)bk->map_blk)[d] = ((const uint16_t*)bk->map_final)[id];
+      ((uint16_t*)bk->map_blk)[e] = ((const uint16_t*)bk->map_final)[id];
   }
 }
 
@@ -702,7 +706,8 @@ hipError_t launch_block_extract(const void* src, int32_t elem_bytes, const int64
 
 hipError_t launch_block_maps(const BrickFrame& f, int32_t map_dtype, hipStream_t st)
 {
-  const dim3 grid((unsigned)std::max(1, std::min(f.n_map_items, 4096)));
+  if (f.nbricks < 1) return hipSuccess;
+  const dim3 grid((unsigned)std::max(8, 4096 / f.nbricks), (unsigned)f.nbricks);
   if (map_dtype == kF32)
     hipLaunchKernelGGL((k_block_maps<kF32>), grid, dim3(256), 0, st, f);
   else
