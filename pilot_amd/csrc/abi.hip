@@ -107,6 +107,7 @@ struct beams_renderer {
   float error_budget = 2e-3f;
   int32_t check_stride = 8;
   DevBuf coarse;  // coarse extinction and optical depth, (kCoarseCells + 1)^3 each
+  DevBuf tfidx;   // the TF entry of every field point (the learned map's feature patches)
   DevBuf lerr;    // check errors as float bits
   // the last learned build, whose statistics are read back on request: 0 none, 1 single block
   // (lerr holds its two check errors), 2 bricks (lb_list holds the fallback's tile list)
@@ -508,6 +509,19 @@ void learned_built(beams_renderer* h, int32_t kind)
   h->built_budget = h->error_budget;
 }
 
+// The learned map's TF-index field (one byte per field point) when the field and colormap
+// allow it, else none (the feature patches then read the field).
+int prepare_tfidx(beams_renderer* h, const LearnedParams& p)
+{
+  if (tf_index_field_ok(p)) {
+    HIP_TRY(h, h->tfidx.ensure((size_t)(p.g.n[0] * p.g.n[1] * p.g.n[2])));
+  } else if (h->tfidx.p) {
+    HIP_TRY(h, hipStreamSynchronize(h->stream));  // a replay in flight may read it
+    h->tfidx.release();
+  }
+  return BEAMS_OK;
+}
+
 // Device buffers of the learned map build and its fallback (allocated outside any capture).
 // With the reference-march fallback, the single block is re-marched tile by tile like a brick:
 // a one-brick frame (its descriptor, queue counters and tables uploaded when they change).
@@ -515,6 +529,7 @@ int prepare_learned(beams_renderer* h, const GridParams& g, const ShadeParams& s
 {
   int rc = prepare_map(h, BEAMS_SHADOW_LEARNED);
   if (!rc) rc = prepare_map(h, h->fallback_mode);
+  if (!rc) rc = prepare_tfidx(h, learned_params(h, g, s, map_params(h, g, BEAMS_SHADOW_LEARNED)));
   if (rc) return rc;
   h->sb_tiles = h->fallback_mode == BEAMS_SHADOW_REF_MARCH;
   if (!h->sb_tiles) return BEAMS_OK;
@@ -579,6 +594,10 @@ int enqueue_learned(beams_renderer* h, const GridParams& g, const ShadeParams& s
   LearnedParams p = learned_params(h, g, s, m);
   p.verr = h->sb_tiles ? h->sb_lb.verr : nullptr;
   HIP_TRY(h, launch_coarse(p, h->stream));
+  if (h->tfidx.p) {
+    HIP_TRY(h, launch_tf_index_field(p, (uint8_t*)h->tfidx.p, h->stream));
+    p.tfidx = (const uint8_t*)h->tfidx.p;
+  }
   HIP_TRY(h, launch_learned_map(p, h->stream));
   HIP_TRY(h, launch_zero_u64((unsigned long long*)h->lerr.p, 1, h->stream));
   HIP_TRY(h, launch_learned_check(p, h->stream));
@@ -1048,7 +1067,12 @@ int enqueue_bricks_learned(beams_renderer* h, const BrickFrame& f, const Learned
   int rc;
   if ((rc = mark(BEAMS_PHASE_MAP, false))) return rc;
   HIP_TRY(h, launch_coarse(p, h->stream));
-  HIP_TRY(h, launch_bricks_learned_map(p, f, h->map_dtype, h->stream));
+  LearnedParams pm = p;
+  if (h->tfidx.p) {
+    HIP_TRY(h, launch_tf_index_field(p, (uint8_t*)h->tfidx.p, h->stream));
+    pm.tfidx = (const uint8_t*)h->tfidx.p;
+  }
+  HIP_TRY(h, launch_bricks_learned_map(pm, f, h->map_dtype, h->stream));
   HIP_TRY(h, launch_bricks_learned_check(p, f, h->map_dtype, h->lb, h->stream));
   if ((rc = mark(BEAMS_PHASE_MAP, true))) return rc;
   if ((rc = mark(BEAMS_PHASE_REMAP, false))) return rc;
@@ -1100,6 +1124,7 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
   if (learned) {
     if ((rc = prepare_bricks_learned(h, f))) return rc;
     lp = bricks_learned_params(h);
+    if ((rc = prepare_tfidx(h, lp))) return rc;
   }
   auto issue = [&](bool timed) -> int {
     int r;
@@ -1130,7 +1155,7 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
   KeyBuilder key;
   key.put(1).put(f).put(c).put(o).put(h->btab_version).put(shadows).put(learned).put(h->dtype)
      .put(h->map_dtype).put(ntf);
-  if (learned) key.put(lp).put(h->lb);
+  if (learned) key.put(lp).put(h->lb).put(h->tfidx.p);
   if ((rc = run_frame(h, key.bytes, issue))) return rc;
   if (learned) learned_built(h, 2);
   if (shadows) h->map_valid = true;
@@ -1191,6 +1216,7 @@ int beams_destroy(beams_renderer* h)
   h->sweep_slices.release();
   h->sweep_sigma.release();
   h->coarse.release();
+  h->tfidx.release();
   h->lerr.release();
   h->rays.release();
   h->image.release();
@@ -1838,7 +1864,7 @@ int beams_render_image(beams_renderer* h, const beams_camera* cam, int32_t width
   key.put(0).put(g).put(m).put(s).put(c).put(o).put(with_map).put(h->sweep_block);
   if (learned)
     key.put(h->fallback_mode).put(h->error_budget).put(h->check_stride).put(h->net ? mlp_device_params(h->net) : nullptr)
-       .put(h->sb_lb).put(h->sb_f);
+       .put(h->sb_lb).put(h->sb_f).put(h->tfidx.p);
   if ((rc = run_frame(h, key.bytes, issue))) return rc;
   if (with_map) {
     h->map_valid = true;

@@ -118,6 +118,19 @@ __device__ __forceinline__ float features(const LearnedParams& p, const float* d
     o[d][1] = c * g.field_stride[d];
     o[d][2] = hi * g.field_stride[d];
   }
+  if (p.tfidx) {
+    // the TF entries of the field points, precomputed this frame: one byte per gather
+    uint32_t b[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int t = 0; t < 3; ++t) b[d][t] = (uint32_t)o[d][t];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) f[1 + q] = d_tab[p.tfidx[b[2][q / 9] + b[1][(q / 3) % 3] + b[0][q % 3]]];
+    light_dir(p.m, v, f + 28);
+    f[31] = 0.0f;
+    return tc;
+  }
   float val[27];
   if (p.field_small) {
     // byte offsets below 4 GiB: one 32-bit add per gather from the uniform field base
@@ -152,6 +165,14 @@ __device__ __forceinline__ float features(const LearnedParams& p, const float* d
 __device__ __forceinline__ float sigma_of(float a, float inv_h)
 {
   return -log1pf(-fminf(a, 1.0f - 5.9604645e-8f)) * inv_h;
+}
+
+// ---- TF entry of every field point (the feature patches' table index), one byte each -------
+template <int FDT>
+__global__ void __launch_bounds__(kBlock) k_tf_index_field(LearnedParams p, uint8_t* out, int64_t n)
+{
+  for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlock)
+    out[q] = (uint8_t)tf_index(p.s, load_t<FDT>(p.g.field, q));
 }
 
 // ---- coarse mip: mean extinction over the field vertices nearest each coarse vertex -------
@@ -663,6 +684,19 @@ hipError_t launch_coarse(const LearnedParams& p, hipStream_t st)
   if (e != hipSuccess) return e;
   const int64_t per_block = kBlock / 64;
   hipLaunchKernelGGL(k_coarse_tau, dim3((unsigned)((nc + per_block - 1) / per_block)), dim3(kBlock), 0, st, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_tf_index_field(const LearnedParams& p, uint8_t* out, hipStream_t st)
+{
+  const int64_t n = p.g.n[0] * p.g.n[1] * p.g.n[2];
+  if (p.s.ntf > 256 || p.g.field_stride[0] != 1 || p.g.field_stride[1] != p.g.n[0] ||
+      p.g.field_stride[2] != p.g.n[0] * p.g.n[1] || n >= ((int64_t)1 << 32))
+    return hipErrorInvalidValue;
+  const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 16384)));
+  BEAMS_DISPATCH_FIELD(p.g.dtype, {
+    hipLaunchKernelGGL((k_tf_index_field<FDT>), grid, dim3(kBlock), 0, st, p, out, n);
+  });
   return hipGetLastError();
 }
 

@@ -41,12 +41,25 @@ struct LearnedParams {
   const float* params;  // network parameters (FP32 packed); null: coarse optical depth only
   unsigned int* err;    // [0] max |learned - exact|, [1] max |coarse only - exact| (float bits)
   float* verr;          // optional: |learned - exact| per check vertex (the check tiles' input)
+  const uint8_t* tfidx; // optional: the TF entry of every field point (dense field, at most 256
+                        // entries), written this frame by launch_tf_index_field
   int32_t check_stride; // check vertices: every check_stride-th map index per axis + the last
   int32_t field_small;  // the field's storage spans less than 4 GiB (32-bit byte offsets)
 };
 
 hipError_t launch_coarse(const LearnedParams& p, hipStream_t st);
 hipError_t launch_learned_map(const LearnedParams& p, hipStream_t st);
+// The TF entry of every point of p.g's field into out (one byte each): the feature patches read
+// it instead of the field.  Refused (hipErrorInvalidValue) unless the field is dense, below 2^32
+// points, and the TF has at most 256 entries.
+hipError_t launch_tf_index_field(const LearnedParams& p, uint8_t* out, hipStream_t st);
+// Whether launch_tf_index_field applies to p.
+inline bool tf_index_field_ok(const LearnedParams& p)
+{
+  const int64_t n = p.g.n[0] * p.g.n[1] * p.g.n[2];
+  return p.s.ntf <= 256 && p.g.field_stride[0] == 1 && p.g.field_stride[1] == p.g.n[0] &&
+         p.g.field_stride[2] == p.g.n[0] * p.g.n[1] && n < ((int64_t)1 << 32);
+}
 hipError_t launch_learned_check(const LearnedParams& p, hipStream_t st);
 // n features (n x kFeat) and targets at map vertices drawn uniformly (SplitMix64(seed));
 // the exact opacities come from `exact` (layout and storage type of em).
