@@ -465,12 +465,13 @@ int enqueue_map(beams_renderer* h, const GridParams& g, const MapParams& m, cons
     sp.m = m;
     sp.s = s;
     sweep_axis(h, sp.axis, sp.a1, sp.a2, sp.step);
+    sp.f = sweep_frame(m, sp.axis, sp.a1, sp.a2);
     sp.h = m.step;
     sp.inv_h = 1.0f / m.step;
     const size_t ns = (size_t)m.mp[sp.a1] * (size_t)m.mp[sp.a2];
     sp.prev = (float2*)h->sweep_slices.p;
     sp.cur = sp.prev + ns;
-    const SweepPlan plan = sweep_plan(sp, h->bounds);
+    const SweepPlan plan = sweep_plan(sp);
     HIP_TRY(h, launch_sweep(sp, plan, h->sweep_block ? (float*)h->sweep_sigma.p : nullptr, h->stream));
   } else {
     HIP_TRY(h, launch_tmap_ref(g, m, s, nullptr, h->stream));

@@ -533,15 +533,19 @@ __global__ void __launch_bounds__(256) k_block_extract(const unsigned char* src,
 
 // map_final -> map_blk in destination order (blockIdx.y: the brick): consecutive threads write
 // consecutive elements of the blocked copy, whole lines at a time, and gather their values from
-// the x-fastest map (four consecutive x per row, rows the L2 holds for the neighbouring blocks).
+// the x-fastest map (four consecutive x per row). Each workgroup copies one contiguous run of
+// blocks, so the 16 blocks that share a row's line are read by one workgroup, on one XCD, from
+// its L2 (interleaved runs put them on four XCDs, each fetching the line from HBM).
 template <int MDT>
 __global__ void __launch_bounds__(256) k_block_maps(BrickFrame f)
 {
   const BrickDesc* bk = f.bricks + blockIdx.y;
   const int64_t mp0 = bk->m.mp[0], mp1 = bk->m.mp[1], mp2 = bk->m.mp[2];
   const int64_t nbx = (mp0 + 3) / 4, nby = (mp1 + 3) / 4;
-  const int64_t count = nbx * nby * ((mp2 + 3) / 4) * 64;
-  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < count; e += (int64_t)gridDim.x * 256) {
+  const int64_t nblk = nbx * nby * ((mp2 + 3) / 4);
+  const int64_t per = (nblk + gridDim.x - 1) / gridDim.x;
+  const int64_t end = std::min<int64_t>(nblk, per * (blockIdx.x + 1)) * 64;
+  for (int64_t e = per * blockIdx.x * 64 + threadIdx.x; e < end; e += 256) {
     const int64_t blk = e >> 6, w = e & 63;
     const int64_t i = (blk % nbx) * 4 + (w & 3), j = ((blk / nbx) % nby) * 4 + ((w >> 2) & 3),
                   k = (blk / (nbx * nby)) * 4 + (w >> 4);

@@ -129,6 +129,8 @@ def test_gpu_blocked_sweep_equals_per_slice(oracle_mod, light, mdt, cells):
             print(f"[sweep blocked={blocked} {light} {cells}]", flush=True)
             img, _ = r.RenderImage(sc.camera, sc.width, sc.height)
             maps[blocked] = (r.GetMap(cells), img)
+    diff = np.abs(maps[1][0] - maps[0][0])
+    print(f"blocked vs per-slice map: {int(np.count_nonzero(diff))} vertices differ, max {diff.max():.3e}")
     assert np.array_equal(maps[1][0], maps[0][0])
     assert np.array_equal(maps[1][1], maps[0][1])
     assert maps[1][0].max() > 0.1
