@@ -535,26 +535,44 @@ __global__ void __launch_bounds__(256) k_block_extract(const unsigned char* src,
 // consecutive elements of the blocked copy, whole lines at a time, and gather their values from
 // the x-fastest map (four consecutive x per row). Each workgroup copies one contiguous run of
 // blocks, so the 16 blocks that share a row's line are read by one workgroup, on one XCD, from
-// its L2 (interleaved runs put them on four XCDs, each fetching the line from HBM).
+// its L2 (interleaved runs put them on four XCDs, each fetching the line from HBM).  A thread
+// keeps one element of every fourth block, its block coordinates counted up as it goes.
 template <int MDT>
 __global__ void __launch_bounds__(256) k_block_maps(BrickFrame f)
 {
   const BrickDesc* bk = f.bricks + blockIdx.y;
-  const int64_t mp0 = bk->m.mp[0], mp1 = bk->m.mp[1], mp2 = bk->m.mp[2];
-  const int64_t nbx = (mp0 + 3) / 4, nby = (mp1 + 3) / 4;
-  const int64_t nblk = nbx * nby * ((mp2 + 3) / 4);
+  const int mp0 = (int)bk->m.mp[0], mp1 = (int)bk->m.mp[1], mp2 = (int)bk->m.mp[2];
+  const int nbx = (mp0 + 3) / 4, nby = (mp1 + 3) / 4;
+  const int64_t nblk = (int64_t)nbx * nby * ((mp2 + 3) / 4);
   const int64_t per = (nblk + gridDim.x - 1) / gridDim.x;
-  const int64_t end = std::min<int64_t>(nblk, per * (blockIdx.x + 1)) * 64;
-  for (int64_t e = per * blockIdx.x * 64 + threadIdx.x; e < end; e += 256) {
-    const int64_t blk = e >> 6, w = e & 63;
-    const int64_t i = (blk % nbx) * 4 + (w & 3), j = ((blk / nbx) % nby) * 4 + ((w >> 2) & 3),
-                  k = (blk / (nbx * nby)) * 4 + (w >> 4);
-    if (i >= mp0 || j >= mp1 || k >= mp2) continue;  // padding of the last blocks
-    const int64_t id = i + mp0 * (j + mp1 * k);
-    if (MDT == kF32)
-      ((uint32_t*)bk->map_blk)[e] = ((const uint32_t*)bk->map_final)[id];
-    else
-      ((uint16_t*)bk->map_blk)[e] = ((const uint16_t*)bk->map_final)[id];
+  const int64_t end = std::min<int64_t>(nblk, per * (blockIdx.x + 1));
+  int64_t blk = per * blockIdx.x + (threadIdx.x >> 6);
+  const int w = (int)threadIdx.x & 63, di = w & 3, dj = (w >> 2) & 3, dk = w >> 4;
+  const int64_t q = blk / nbx;
+  int bx = (int)(blk - q * nbx), by = (int)(q % nby), bz = (int)(q / nby);
+  // four blocks' gathers in flight before their stores
+  constexpr int kInFlight = 4;
+  typedef typename std::conditional<MDT == kF32, uint32_t, uint16_t>::type Word;
+  for (; blk < end; blk += 4 * kInFlight) {
+    Word v[kInFlight];
+    bool ok[kInFlight];
+#pragma unroll
+    for (int u = 0; u < kInFlight; ++u) {
+      const int i = 4 * bx + di, j = 4 * by + dj, k = 4 * bz + dk;
+      ok[u] = blk + 4 * u < end && i < mp0 && j < mp1 && k < mp2;  // not past the run or the padding
+      v[u] = ok[u] ? ((const Word*)bk->map_final)[i + (int64_t)mp0 * (j + (int64_t)mp1 * k)] : (Word)0;
+      bx += 4;
+      while (bx >= nbx) {
+        bx -= nbx;
+        if (++by == nby) {
+          by = 0;
+          ++bz;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kInFlight; ++u)
+      if (ok[u]) ((Word*)bk->map_blk)[(blk + 4 * u) * 64 + w] = v[u];
   }
 }
 

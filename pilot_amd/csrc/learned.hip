@@ -168,10 +168,34 @@ __device__ __forceinline__ float sigma_of(float a, float inv_h)
 }
 
 // ---- TF entry of every field point (the feature patches' table index), one byte each -------
+// Four points per thread: one 8-byte (16-bit storage) or 16-byte (FP32) load and one 4-byte
+// store of their four indices (the field and the output 16-byte aligned), the last n % 4 points
+// one at a time.
 template <int FDT>
 __global__ void __launch_bounds__(kBlock) k_tf_index_field(LearnedParams p, uint8_t* out, int64_t n)
 {
-  for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlock)
+  const int64_t n4 = n / 4;
+  for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < n4; q += (int64_t)gridDim.x * kBlock) {
+    float v[4];
+    if constexpr (FDT == kF32) {
+      const float4 w = ((const float4*)p.g.field)[q];
+      v[0] = w.x;
+      v[1] = w.y;
+      v[2] = w.z;
+      v[3] = w.w;
+    } else {
+      const uint2 w = ((const uint2*)p.g.field)[q];
+      const uint16_t h[4] = { (uint16_t)(w.x & 0xffffu), (uint16_t)(w.x >> 16), (uint16_t)(w.y & 0xffffu),
+                              (uint16_t)(w.y >> 16) };
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = FDT == kF16 ? f16_to_f32(h[u]) : bf16_to_f32(h[u]);
+    }
+    uint32_t packed = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) packed |= (uint32_t)tf_index(p.s, v[u]) << (8 * u);
+    ((uint32_t*)out)[q] = packed;
+  }
+  for (int64_t q = 4 * n4 + (int64_t)blockIdx.x * kBlock + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlock)
     out[q] = (uint8_t)tf_index(p.s, load_t<FDT>(p.g.field, q));
 }
 
@@ -691,9 +715,10 @@ hipError_t launch_tf_index_field(const LearnedParams& p, uint8_t* out, hipStream
 {
   const int64_t n = p.g.n[0] * p.g.n[1] * p.g.n[2];
   if (p.s.ntf > 256 || p.g.field_stride[0] != 1 || p.g.field_stride[1] != p.g.n[0] ||
-      p.g.field_stride[2] != p.g.n[0] * p.g.n[1] || n >= ((int64_t)1 << 32))
+      p.g.field_stride[2] != p.g.n[0] * p.g.n[1] || n >= ((int64_t)1 << 32) || ((uintptr_t)p.g.field & 15) != 0 ||
+      ((uintptr_t)out & 15) != 0)
     return hipErrorInvalidValue;
-  const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 16384)));
+  const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>((n / 4 + kBlock - 1) / kBlock, 16384)));
   BEAMS_DISPATCH_FIELD(p.g.dtype, {
     hipLaunchKernelGGL((k_tf_index_field<FDT>), grid, dim3(kBlock), 0, st, p, out, n);
   });

@@ -51,14 +51,14 @@ hipError_t launch_coarse(const LearnedParams& p, hipStream_t st);
 hipError_t launch_learned_map(const LearnedParams& p, hipStream_t st);
 // The TF entry of every point of p.g's field into out (one byte each): the feature patches read
 // it instead of the field.  Refused (hipErrorInvalidValue) unless the field is dense, below 2^32
-// points, and the TF has at most 256 entries.
+// points and 16-byte aligned (as out), and the TF has at most 256 entries.
 hipError_t launch_tf_index_field(const LearnedParams& p, uint8_t* out, hipStream_t st);
-// Whether launch_tf_index_field applies to p.
+// Whether launch_tf_index_field applies to p (for a 16-byte aligned out).
 inline bool tf_index_field_ok(const LearnedParams& p)
 {
   const int64_t n = p.g.n[0] * p.g.n[1] * p.g.n[2];
   return p.s.ntf <= 256 && p.g.field_stride[0] == 1 && p.g.field_stride[1] == p.g.n[0] &&
-         p.g.field_stride[2] == p.g.n[0] * p.g.n[1] && n < ((int64_t)1 << 32);
+         p.g.field_stride[2] == p.g.n[0] * p.g.n[1] && n < ((int64_t)1 << 32) && ((uintptr_t)p.g.field & 15) == 0;
 }
 hipError_t launch_learned_check(const LearnedParams& p, hipStream_t st);
 // n features (n x kFeat) and targets at map vertices drawn uniformly (SplitMix64(seed));

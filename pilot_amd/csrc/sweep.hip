@@ -47,14 +47,18 @@ constexpr int kStage = kTile + 2 * kHalo + 1;
 
 // Staging of the TF's extinction per entry, the coordinates and their reciprocal cell widths
 // (LDS: ntf + 2 ncoord floats; xs[ncoord + i] = 1 / (x[i+1] - x[i]) within each axis).
-__device__ __forceinline__ void stage_sweep(const SweepParams& sp, float* sig_tab, float* xs)
+__device__ __forceinline__ void stage_sigma_table(const SweepParams& sp, float* sig_tab)
 {
-  const GridParams& g = sp.g;
-  const int ntf = sp.s.ntf, n0 = (int)g.n[0], n01 = (int)(g.n[0] + g.n[1]), ncoord = (int)(g.n[0] + g.n[1] + g.n[2]);
-  for (int i = threadIdx.x; i < ntf; i += blockDim.x) {
+  for (int i = threadIdx.x; i < sp.s.ntf; i += blockDim.x) {
     const float a = fminf(sp.s.tf[4 * i + 3] / sp.s.max_density, 0.99999994f);
     sig_tab[i] = -__logf(1.0f - a) * sp.inv_h;
   }
+}
+__device__ __forceinline__ void stage_sweep(const SweepParams& sp, float* sig_tab, float* xs)
+{
+  const GridParams& g = sp.g;
+  const int n0 = (int)g.n[0], n01 = (int)(g.n[0] + g.n[1]), ncoord = (int)(g.n[0] + g.n[1] + g.n[2]);
+  stage_sigma_table(sp, sig_tab);
   for (int i = threadIdx.x; i < ncoord; i += blockDim.x) {
     const int d = i < n0 ? 0 : (i < n01 ? 1 : 2), j = i - (d == 0 ? 0 : (d == 1 ? n0 : n01));
     xs[i] = g.x[d][j];
@@ -88,38 +92,42 @@ __device__ __forceinline__ void sweep_xyz(const SweepParams& sp, const float* w,
 }
 
 // The field cell of position v (x, y, z) and its parametric coordinates t: the guess from
-// uniform spacing for every axis first (their LDS reads issued together), then a walk where
-// any guess missed (v in [x[c], x[c+1]), the last cell on the max face).
+// uniform spacing for every axis first (their reads issued together), then a walk where any
+// guess missed (v in [x[c], x[c+1]), the last cell on the max face).  x[d]: axis d's
+// coordinates, rw[d]: their reciprocal cell widths (LDS or global).
 struct SweepCell {
   int c[3];
   float t[3];
 };
-__device__ __forceinline__ SweepCell sweep_cell(const GridParams& g, const float* const* x, int rofs, const float* v)
+__device__ __forceinline__ SweepCell sweep_cell(const GridParams& g, const float* const* x, const float* const* rw,
+                                                const float* v)
 {
-  AxisCell a[3];
+  int c[3];
+  float lo[3], hi[3];
   bool hit = true;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    a[d] = axis_guess(x[d], rofs, (int)g.n[d] - 1, v[d], g.gx0[d], g.ginv[d]);
-    hit = hit & (v[d] >= a[d].lo) & (v[d] < a[d].hi);
+    c[d] = (int)__builtin_amdgcn_fmed3f((v[d] - g.gx0[d]) * g.ginv[d], 0.0f, (float)(g.n[d] - 2));
+    lo[d] = x[d][c[d]];
+    hi[d] = x[d][c[d] + 1];
+    hit = hit & (v[d] >= lo[d]) & (v[d] < hi[d]);
   }
   if (__builtin_expect(!hit, 0)) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const int nc = (int)g.n[d] - 1;
-      int c = a[d].c;
-      while (c > 0 && v[d] < x[d][c]) --c;
-      while (c < nc - 1 && v[d] >= x[d][c + 1]) ++c;
-      a[d].c = c;
-      a[d].lo = x[d][c];
-      a[d].rw = x[d][rofs + c];
+      int cc = c[d];
+      while (cc > 0 && v[d] < x[d][cc]) --cc;
+      while (cc < nc - 1 && v[d] >= x[d][cc + 1]) ++cc;
+      c[d] = cc;
+      lo[d] = x[d][cc];
     }
   }
   SweepCell r;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    r.c[d] = a[d].c;
-    r.t[d] = fminf(fmaxf((v[d] - a[d].lo) * a[d].rw, 0.0f), 1.0f);
+    r.c[d] = c[d];
+    r.t[d] = fminf(fmaxf((v[d] - lo[d]) * rw[d][c[d]], 0.0f), 1.0f);
   }
   return r;
 }
@@ -144,7 +152,7 @@ __device__ __forceinline__ CornerFetch<FDT> sweep_corners(const GridParams& g)
 }
 
 // The eight corner loads of a cell for a field view below 2^32 bytes with strides below 2^24
-// (sweep_fast_fetch): full-rate 24-bit offset products, and the storage words kept at their own
+// (launch_sweep checks it): full-rate 24-bit offset products, and the storage words kept at their own
 // width until decoded (the same values as CornerFetch).
 template <int FDT>
 struct SweepFetch {
@@ -173,16 +181,21 @@ struct SweepFetch {
     const int off = (int)(e * (FDT == kF32 ? 4u : 2u));
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      if (FDT == kF32)
+      if constexpr (FDT == kF32)
         w[q] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, so[q], 0);
       else
         w[q] = __builtin_amdgcn_raw_buffer_load_b16(rsrc, off, so[q], 0);
     }
   }
-  __device__ __forceinline__ static float decode(Raw w)
+  __device__ __forceinline__ static void decode(const Raw* w, float* v)
   {
-    if (FDT == kF32) return __builtin_bit_cast(float, (uint32_t)w);
-    return FDT == kF16 ? f16_to_f32((uint16_t)w) : bf16_to_f32((uint16_t)w);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if constexpr (FDT == kF32)
+        v[q] = __builtin_bit_cast(float, (uint32_t)w[q]);
+      else
+        v[q] = FDT == kF16 ? f16_to_f32((uint16_t)w[q]) : bf16_to_f32((uint16_t)w[q]);
+    }
   }
 };
 
@@ -292,13 +305,14 @@ __global__ void __launch_bounds__(256) k_sweep_slice(SweepParams sp, int32_t k, 
     }
   __syncthreads();
   const float* x[3] = { xs, xs + g.n[0], xs + g.n[0] + g.n[1] };
+  const float* rw[3] = { x[0] + ncoord, x[1] + ncoord, x[2] + ncoord };
   const int i1 = blockIdx.x * kTile + (threadIdx.x % kTile);
   const int i2 = blockIdx.y * kTile + (threadIdx.x / kTile);
   if (i1 >= n1 || i2 >= n2) return;
   float w[3], v[3];
   sweep_w(sp, k, i1, i2, w);
   sweep_xyz(sp, w, v);
-  const SweepCell cl = sweep_cell(g, x, ncoord, v);
+  const SweepCell cl = sweep_cell(g, x, rw, v);
   float cv[8];
   sweep_corners<FDT>(g).load(cl.c[0], cl.c[1], cl.c[2], cv);
   const float sig_v = sweep_interp(sp, sig_tab, cv, cl.t);
@@ -323,16 +337,7 @@ __global__ void __launch_bounds__(256) k_sweep_slice(SweepParams sp, int32_t k, 
 // together.
 constexpr int kSigmaRuns = 4;
 
-template <int FDT, bool FAST>
-__device__ __forceinline__ auto sigma_fetch(const GridParams& g)
-{
-  if constexpr (FAST)
-    return SweepFetch<FDT>(g);
-  else
-    return sweep_corners<FDT>(g);
-}
-
-// FAST: the field suits SweepFetch (sweep_fast_fetch).
+// FAST: the field suits SweepFetch (launch_sweep checks it), otherwise CornerFetch.
 template <int FDT, bool FAST>
 __global__ void __launch_bounds__(256) k_sweep_sigma(SweepParams sp, float* sig)
 {
@@ -345,8 +350,16 @@ __global__ void __launch_bounds__(256) k_sweep_sigma(SweepParams sp, float* sig)
   const GridParams& g = sp.g;
   const int ncoord = (int)(g.n[0] + g.n[1] + g.n[2]);
   const float* x[3] = { xs, xs + g.n[0], xs + g.n[0] + g.n[1] };
-  const auto corners = sigma_fetch<FDT, FAST>(g);
+  const float* rw[3] = { x[0] + ncoord, x[1] + ncoord, x[2] + ncoord };
+  typedef typename std::conditional<FAST, SweepFetch<FDT>, CornerFetch<FDT>>::type Fetch;
   typedef typename std::conditional<FAST, typename SweepFetch<FDT>::Raw, uint32_t>::type Raw;
+  constexpr int kLoads = 8;
+  const Fetch corners = [&]() {
+    if constexpr (FAST)
+      return SweepFetch<FDT>(g);
+    else
+      return sweep_corners<FDT>(g);
+  }();
   // a slice has at most 4097^2 vertices, a sweep at most 4097 slices of 65,569 runs
   const int n0 = sp.f.n[0], n1 = sp.f.n[1], n12 = n1 * sp.f.n[2];
   const int runs = (n12 + 255) / 256, items = n0 * runs;
@@ -370,17 +383,21 @@ __global__ void __launch_bounds__(256) k_sweep_sigma(SweepParams sp, float* sig)
       float w[3], v[3];
       sweep_w(sp, sp.step > 0 ? s : n0 - 1 - s, i1, i2, w);
       sweep_xyz(sp, w, v);
-      cl[u] = sweep_cell(g, x, ncoord, v);
+      cl[u] = sweep_cell(g, x, rw, v);
       dst[u] = (int64_t)s * n12 + pl;
     }
-    Raw raw[kSigmaRuns][8];
+    Raw raw[kSigmaRuns][kLoads];
 #pragma unroll
     for (int u = 0; u < kSigmaRuns; ++u) corners.load_raw(cl[u].c[0], cl[u].c[1], cl[u].c[2], raw[u]);
 #pragma unroll
     for (int u = 0; u < kSigmaRuns; ++u) {
       float cv[8];
+      if constexpr (FAST) {
+        SweepFetch<FDT>::decode(raw[u], cv);
+      } else {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) cv[q] = FAST ? SweepFetch<FDT>::decode(raw[u][q]) : CornerFetch<FDT>::decode(raw[u][q]);
+        for (int q = 0; q < 8; ++q) cv[q] = CornerFetch<FDT>::decode(raw[u][q]);
+      }
       const float sg = sweep_interp(sp, sig_tab, cv, cl[u].t);
       if (live[u]) sig[dst[u]] = sg;
     }
@@ -445,6 +462,48 @@ __device__ __forceinline__ void tile_reach(const SweepParams& sp, const SweepSha
   }
 }
 
+// Window position of ring-order index q.  Ring 0 is the tile (slice nt - 1's region), ring m
+// the vertices of slice (nt - 1 - m)'s region outside slice (nt - m)'s: the region of slice t is
+// then the prefix q < |region t|.  b*, L*: slice t's region (b1 + r1, b2 + r2 within it).
+__device__ __forceinline__ void ring_position(int q, int tile, int nt, int lo1, int hi1, int lo2, int hi2, int t1,
+                                              int t2, int& i1, int& i2)
+{
+  if (q < tile * tile) {
+    const int r2 = q / tile;
+    i1 = t1 + q - r2 * tile;
+    i2 = t2 + r2;
+    return;
+  }
+  // the ring: outer region of slice nt - 1 - m around the inner one of slice nt - m
+  int m = 1, Lo1 = tile + lo1 + hi1, Lo2 = tile + lo2 + hi2, Li1 = tile, Li2 = tile;
+  while (m < nt - 1 && q >= Lo1 * Lo2) {
+    ++m;
+    Li1 = Lo1;
+    Li2 = Lo2;
+    Lo1 += lo1 + hi1;
+    Lo2 += lo2 + hi2;
+  }
+  const int bo1 = t1 - m * lo1, bo2 = t2 - m * lo2;  // outer region's first vertex
+  int j = q - Li1 * Li2;
+  if (j < lo2 * Lo1) {  // rows below the inner region
+    const int r = j / Lo1;
+    i1 = bo1 + j - r * Lo1;
+    i2 = bo2 + r;
+    return;
+  }
+  j -= lo2 * Lo1;
+  if (j < hi2 * Lo1) {  // rows above it
+    const int r = j / Lo1;
+    i1 = bo1 + j - r * Lo1;
+    i2 = bo2 + lo2 + Li2 + r;
+    return;
+  }
+  j -= hi2 * Lo1;  // its rows: lo1 vertices on the low side, hi1 on the high side
+  const int w = lo1 + hi1, r = j / w, c = j - r * w;
+  i2 = bo2 + lo2 + r;
+  i1 = c < lo1 ? bo1 + c : bo1 + lo1 + Li1 + (c - lo1);
+}
+
 template <int MDT>
 __global__ void __launch_bounds__(kSweepThreads) k_sweep_block(SweepParams sp, const float* sig, int32_t s0,
                                                                SweepShape sh)
@@ -496,25 +555,27 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep_block(SweepParams sp, c
       }
     }
   }
-  // sigma of this thread's vertices of slice t's region (0 outside the slice)
+  // This thread's vertices for the whole launch: ring-order indices tid + kSweepThreads u, in
+  // slice t's region while below |region t|; outside the slice they are skipped.
+  const int size0 = sweep_len(tile, nt, lo1, hi1, 0) * sweep_len(tile, nt, lo2, hi2, 0);
+  int vi1[kSweepPer], vi2[kSweepPer];
+  bool vin[kSweepPer];
+#pragma unroll
+  for (int u = 0; u < kSweepPer; ++u) {
+    const int q = tid + kSweepThreads * u;
+    int i1 = t1, i2 = t2;
+    if (q < size0) ring_position(q, tile, nt, lo1, hi1, lo2, hi2, t1, t2, i1, i2);
+    vi1[u] = i1;
+    vi2[u] = i2;
+    vin[u] = q < size0 && (unsigned)i1 < (unsigned)n1 && (unsigned)i2 < (unsigned)n2;
+  }
+  const int ntile = tile * tile;
   auto load_sigma = [&](int t, float* out) {
-    const int L1 = sweep_len(tile, nt, lo1, hi1, t), L2 = sweep_len(tile, nt, lo2, hi2, t);
-    const int b1 = t1 - (nt - 1 - t) * lo1, b2 = t2 - (nt - 1 - t) * lo2;
-    const int d2 = kSweepThreads / L1, d1 = kSweepThreads - d2 * L1;
-    int r2 = tid / L1, r1 = tid - r2 * L1;
+    const int size = sweep_len(tile, nt, lo1, hi1, t) * sweep_len(tile, nt, lo2, hi2, t);
     const float* ss = sig + (int64_t)(s0 + t) * n12;
 #pragma unroll
-    for (int u = 0; u < kSweepPer; ++u) {
-      if (r2 >= L2) break;
-      const int i1 = b1 + r1, i2 = b2 + r2;
-      out[u] = ((unsigned)i1 < (unsigned)n1 && (unsigned)i2 < (unsigned)n2) ? ss[i2 * n1 + i1] : 0.0f;
-      r1 += d1;
-      r2 += d2;
-      if (r1 >= L1) {
-        r1 -= L1;
-        ++r2;
-      }
-    }
+    for (int u = 0; u < kSweepPer; ++u)
+      if (tid + kSweepThreads * u < size && vin[u]) out[u] = ss[vi2[u] * n1 + vi1[u]];
   };
   float sg_next[kSweepPer];
   load_sigma(0, sg_next);
@@ -526,35 +587,26 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep_block(SweepParams sp, c
     if (t + 1 < nts) load_sigma(t + 1, sg_next);
     const int s = s0 + t, k = sp.step > 0 ? s : n0 - 1 - s;
     const bool last = t == nts - 1;
-    const int L1 = sweep_len(tile, nt, lo1, hi1, t), L2 = sweep_len(tile, nt, lo2, hi2, t);
-    const int b1 = t1 - (nt - 1 - t) * lo1, b2 = t2 - (nt - 1 - t) * lo2;
-    const int d2 = kSweepThreads / L1, d1 = kSweepThreads - d2 * L1;
+    const int size = sweep_len(tile, nt, lo1, hi1, t) * sweep_len(tile, nt, lo2, hi2, t);
     const int64_t kbase = (int64_t)k * n12;
     const SweepSlice geo = sweep_slice(sp, k);
-    int r2 = tid / L1, r1 = tid - r2 * L1;
 #pragma unroll
     for (int u = 0; u < kSweepPer; ++u) {
-      if (r2 >= L2) break;
-      const int i1 = b1 + r1, i2 = b2 + r2;
-      if ((unsigned)i1 < (unsigned)n1 && (unsigned)i2 < (unsigned)n2) {
+      const int q = tid + kSweepThreads * u;
+      if (q < size && vin[u]) {
+        const int i1 = vi1[u], i2 = vi2[u];
         const float sig_v = sg[u];
         float tau = 0.0f;
         if (s > 0) {
-          auto q = [&](int c1, int c2) -> float2 { return prev[(c2 - o2) * W1 + (c1 - o1)]; };
-          tau = sweep_tau(sp, geo, i1, i2, sig_v, q);
+          auto qv = [&](int c1, int c2) -> float2 { return prev[(c2 - o2) * W1 + (c1 - o1)]; };
+          tau = sweep_tau(sp, geo, i1, i2, sig_v, qv);
         }
         cur[(i2 - o2) * W1 + (i1 - o1)] = make_float2(tau, sig_v);
-        if (i1 >= t1 && i2 >= t2 && i1 < t1 + tile && i2 < t2 + tile) {
+        if (q < ntile) {  // the tile
           const int p = i2 * n1 + i1;
           store_t<MDT>(sp.m.map, kbase + p, sweep_opacity(sp, tau, sig_v));
           if (last) sp.cur[p] = make_float2(tau, sig_v);
         }
-      }
-      r1 += d1;
-      r2 += d2;
-      if (r1 >= L1) {
-        r1 -= L1;
-        ++r2;
       }
     }
     __syncthreads();
@@ -671,11 +723,11 @@ hipError_t launch_sweep(SweepParams sp, const SweepPlan& plan, float* sig, hipSt
   if (plan.blocked && sig) {
     const int64_t items = (int64_t)n0 * (((int64_t)n1 * n2 + 255) / 256);
     const size_t lds_a = sizeof(float) * (size_t)(sp.s.ntf + 2 * ncoord);
-    const dim3 ga((unsigned)std::max<int64_t>(1, std::min<int64_t>((items + kSigmaRuns - 1) / kSigmaRuns, 2048)));
     // 24-bit offset products: a view below 2^32 bytes, strides and cells below 2^24
-    const bool fast = sp.g.bytes32 != 0 && sp.g.field_stride[2] < (1 << 24) && sp.g.field_stride[1] < (1 << 24) &&
-                      sp.g.field_stride[0] < (1 << 24) && sp.g.n[0] < (1 << 24) && sp.g.n[1] < (1 << 24) &&
+    const bool fast = sp.g.bytes32 != 0 && sp.g.field_stride[0] < (1 << 24) && sp.g.field_stride[2] < (1 << 24) &&
+                      sp.g.field_stride[1] < (1 << 24) && sp.g.n[0] < (1 << 24) && sp.g.n[1] < (1 << 24) &&
                       sp.g.n[2] < (1 << 24);
+    const dim3 ga((unsigned)std::max<int64_t>(1, std::min<int64_t>((items + kSigmaRuns - 1) / kSigmaRuns, 2048)));
     if (fast) {
       BEAMS_DISPATCH_FIELD(sp.g.dtype, { hipLaunchKernelGGL((k_sweep_sigma<FDT, true>), ga, dim3(256), lds_a, st, sp, sig); });
     } else {

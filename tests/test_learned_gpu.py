@@ -352,6 +352,49 @@ def test_bricks_learned_fallback_per_tile(oracle_mod):
           f"max {err.max():.2e}; learned vs its fallback: max {np.abs(img_l - img_x).max():.2e}")
 
 
+class _FieldView:
+    """A device field starting `offset` bytes into a DeviceArray."""
+
+    is_cuda = True
+
+    def __init__(self, arr, offset):
+        self.arr, self.offset = arr, offset
+
+    def data_ptr(self):
+        return self.arr.data_ptr() + self.offset
+
+
+@pytest.mark.parametrize("dtype", [B.F32, B.F16])
+def test_learned_map_from_tf_index_field_equals_field_patches(dtype):
+    """The learned map's feature patches read a per-frame TF-index field (one byte per field
+    point) where the field is 16-byte aligned, else the field itself: a field placed one element
+    past an aligned address takes the second path, and both give the same map bit for bit (no
+    fallback: the budget is never exceeded)."""
+    from pilot_amd.devmem import DeviceArray, H2D, _ok, hip
+
+    sc = _scene(69, light="point", field_dtype=dtype)
+    host = np.ascontiguousarray(sc.field_store)
+    esz = host.dtype.itemsize
+    buf = DeviceArray((host.size + 1,), host.dtype)
+    _ok(hip().hipMemcpy(buf.data_ptr() + esz, host.ctypes.data, host.nbytes, H2D), "hipMemcpy H2D")
+    maps = []
+    try:
+        with B.ShadowMLP(seed=6) as net:
+            net.set_params(_random_params(70) * 0.2)
+            for values in (host, _FieldView(buf, esz)):
+                with B.LightedVolumeRenderer() as r:
+                    scenes.configure(r, sc)
+                    r.SetData(values, sc.dims, coords=sc.coords, scalar_range=sc.scalar_range, dtype=dtype)
+                    r.SetShadow(True, sc.map_cells, B.F32, mode=B.SHADOW_LEARNED)
+                    r.SetLearned(net, fallback=B.SHADOW_REF_MARCH, error_budget=1.0, check_stride=8)
+                    r.RenderImage(sc.camera, sc.width, sc.height)
+                    maps.append(r.GetMap(sc.map_cells))
+    finally:
+        buf.free()
+    assert maps[0].max() > 0.05
+    assert np.array_equal(maps[0], maps[1])
+
+
 @pytest.mark.parametrize("budget", [1.0, -1.0])
 def test_learned_frame_is_one_graph(oracle_mod, budget):
     """Single block: the learned map, its check, the tile list and the fallback are part of the
