@@ -93,8 +93,9 @@ __device__ __forceinline__ void sweep_xyz(const SweepParams& sp, const float* w,
 
 // The field cell of position v (x, y, z) and its parametric coordinates t: the guess from
 // uniform spacing for every axis first (their reads issued together), then a walk where any
-// guess missed (v in [x[c], x[c+1]), the last cell on the max face).  x[d]: axis d's
-// coordinates, rw[d]: their reciprocal cell widths (LDS or global).
+// guess missed (v in [x[c], x[c+1]), the last cell also holding the max face, which the
+// guess's clamp already lands on).  x[d]: axis d's coordinates, rw[d]: their reciprocal cell
+// widths (LDS or global).
 struct SweepCell {
   int c[3];
   float t[3];
@@ -110,7 +111,7 @@ __device__ __forceinline__ SweepCell sweep_cell(const GridParams& g, const float
     c[d] = (int)__builtin_amdgcn_fmed3f((v[d] - g.gx0[d]) * g.ginv[d], 0.0f, (float)(g.n[d] - 2));
     lo[d] = x[d][c[d]];
     hi[d] = x[d][c[d] + 1];
-    hit = hit & (v[d] >= lo[d]) & (v[d] < hi[d]);
+    hit = hit & (v[d] >= lo[d]) & ((v[d] < hi[d]) | (c[d] == (int)g.n[d] - 2));
   }
   if (__builtin_expect(!hit, 0)) {
 #pragma unroll
@@ -335,7 +336,7 @@ __global__ void __launch_bounds__(256) k_sweep_slice(SweepParams sp, int32_t k, 
 // a1 fastest: sig[s * n1 n2 + i2 n1 + i1].  Workgroups take kSigmaRuns runs of 256 vertices at
 // a time, one per thread each, with every run's LDS reads and then all corner loads in flight
 // together.
-constexpr int kSigmaRuns = 4;
+constexpr int kSigmaRuns = 2;
 
 // FAST: the field suits SweepFetch (launch_sweep checks it), otherwise CornerFetch.
 template <int FDT, bool FAST>
@@ -364,15 +365,23 @@ __global__ void __launch_bounds__(256) k_sweep_sigma(SweepParams sp, float* sig)
   const int n0 = sp.f.n[0], n1 = sp.f.n[1], n12 = n1 * sp.f.n[2];
   const int runs = (n12 + 255) / 256, items = n0 * runs;
   const float rn1 = 1.0f / (float)n1;
-  for (int w0 = blockIdx.x * kSigmaRuns; w0 < items; w0 += gridDim.x * kSigmaRuns) {
+  // item w0 + u is run r + u of slice s (carried into the next slices); the stride between a
+  // workgroup's iterations is ds slices and dr runs
+  const int stride = (int)gridDim.x * kSigmaRuns, ds = stride / runs, dr = stride - ds * runs;
+  int s0 = (int)blockIdx.x * kSigmaRuns / runs, r0 = (int)blockIdx.x * kSigmaRuns - s0 * runs;
+  for (int w0 = blockIdx.x * kSigmaRuns; w0 < items; w0 += stride) {
     SweepCell cl[kSigmaRuns];
     int64_t dst[kSigmaRuns];
     bool live[kSigmaRuns];
+    int s = s0, r = r0;
 #pragma unroll
     for (int u = 0; u < kSigmaRuns; ++u) {
       const int it = w0 + u;
-      const int s = it < items ? it / runs : 0;
-      const int p = (it - s * runs) * 256 + (int)threadIdx.x;
+      if (u > 0 && ++r == runs) {
+        r = 0;
+        ++s;
+      }
+      const int p = r * 256 + (int)threadIdx.x;
       live[u] = it < items && p < n12;
       const int pl = live[u] ? p : 0;
       // pl / n1: the FP32 quotient is within one of it, corrected either way
@@ -380,11 +389,18 @@ __global__ void __launch_bounds__(256) k_sweep_sigma(SweepParams sp, float* sig)
       i2 -= i2 * n1 > pl ? 1 : 0;
       i2 += (i2 + 1) * n1 <= pl ? 1 : 0;
       const int i1 = pl - i2 * n1;
+      const int sl = min(s, n0 - 1);
       float w[3], v[3];
-      sweep_w(sp, sp.step > 0 ? s : n0 - 1 - s, i1, i2, w);
+      sweep_w(sp, sp.step > 0 ? sl : n0 - 1 - sl, i1, i2, w);
       sweep_xyz(sp, w, v);
       cl[u] = sweep_cell(g, x, rw, v);
-      dst[u] = (int64_t)s * n12 + pl;
+      dst[u] = (int64_t)sl * n12 + pl;
+    }
+    s0 += ds;
+    r0 += dr;
+    if (r0 >= runs) {
+      r0 -= runs;
+      ++s0;
     }
     Raw raw[kSigmaRuns][kLoads];
 #pragma unroll
@@ -729,9 +745,13 @@ hipError_t launch_sweep(SweepParams sp, const SweepPlan& plan, float* sig, hipSt
                       sp.g.n[2] < (1 << 24);
     const dim3 ga((unsigned)std::max<int64_t>(1, std::min<int64_t>((items + kSigmaRuns - 1) / kSigmaRuns, 2048)));
     if (fast) {
-      BEAMS_DISPATCH_FIELD(sp.g.dtype, { hipLaunchKernelGGL((k_sweep_sigma<FDT, true>), ga, dim3(256), lds_a, st, sp, sig); });
+      BEAMS_DISPATCH_FIELD(sp.g.dtype, {
+        hipLaunchKernelGGL((k_sweep_sigma<FDT, true>), ga, dim3(256), lds_a, st, sp, sig);
+      });
     } else {
-      BEAMS_DISPATCH_FIELD(sp.g.dtype, { hipLaunchKernelGGL((k_sweep_sigma<FDT, false>), ga, dim3(256), lds_a, st, sp, sig); });
+      BEAMS_DISPATCH_FIELD(sp.g.dtype, {
+        hipLaunchKernelGGL((k_sweep_sigma<FDT, false>), ga, dim3(256), lds_a, st, sp, sig);
+      });
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -190,3 +190,30 @@ def test_sweep_converges_to_the_continuum_depth(kind, light):
     # loses up to half a cell where rays switch from a side face to the near face
     assert np.median(err) < 5e-3
     assert np.quantile(err, 0.99) < 0.1
+
+
+@pytest.mark.gpu
+def test_gpu_sweep_graph_replays_equal_eager():
+    """The blocked sweep (a sigma pass, then several slices per launch) captured in the
+    frame's hipGraph: every replay equals the eager frame (map and image), and the eager frame
+    equals the one-launch-per-slice shape."""
+    sc = scenes.make_scene("sweepgraph", 41, seed=43, nblobs=12, noise_octaves=3, noise_amp=0.2,
+                           light="point", width=64, height=48, map_cells=(40, 40, 40))
+    out = {}
+    for graph, blocked in ((False, 0), (False, 1), (True, 1)):
+        with B.LightedVolumeRenderer() as r:
+            scenes.configure(r, sc)
+            r.SetShadow(True, sc.map_cells, B.F32, mode=B.SHADOW_SWEEP)
+            r.SetSchedule(sweep_block=blocked)
+            r.SetGraph(graph)
+            print(f"[sweep graph={graph} blocked={blocked}]", flush=True)
+            imgs = [r.RenderImage(sc.camera, sc.width, sc.height)[0] for _ in range(3 if graph else 1)]
+            out[(graph, blocked)] = (imgs, r.GetMap(sc.map_cells))
+    e_imgs, e_map = out[(False, 1)]
+    assert np.array_equal(e_map, out[(False, 0)][0 + 1])
+    assert np.array_equal(e_imgs[0], out[(False, 0)][0][0])
+    g_imgs, g_map = out[(True, 1)]
+    assert np.array_equal(g_map, e_map)
+    for img in g_imgs:
+        assert np.array_equal(img, e_imgs[0])
+    assert e_map.max() > 0.1
