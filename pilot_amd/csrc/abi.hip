@@ -148,6 +148,7 @@ struct beams_renderer {
   std::vector<Brick> brick_pool;  // storage of the decomposition a new field replaced
   int32_t ghost = 0;
   DevBuf bmap_local, bmap_final, bseeds, btables, bpartial, bqueue;
+  DevBuf brm_list;  // phase 3's list of the vertices to re-march (one uint32 per map vertex at most)
   DevBuf bmap_blk;  // 4 x 4 x 4-blocked copies of the final maps (the blocked march's)
   // learned bricks: check-slot and tile offsets (host copy and device table), per-check-vertex
   // errors, and the fallback's tile list
@@ -950,6 +951,9 @@ int prepare_bricks(beams_renderer* h, const CameraParams& c, BrickFrame& f, int6
   f.depth_order = (const int32_t*)(base + o_ord);
   f.queue = (unsigned int*)h->bqueue.p;
   f.seeds = (float*)h->bseeds.p;
+  f.rm_count = f.queue + 5 * kQueueSlots;
+  f.rm_item_off = (int32_t*)(f.rm_count + nb);
+  f.rm_list = (uint32_t*)h->brm_list.p;
   f.partial = (float4*)h->bpartial.p;
   f.partial_depth = (float*)((float4*)h->bpartial.p + partial_pixels);
   f.max_coords = max_coords;
@@ -1101,7 +1105,10 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
   }
   HIP_TRY(h, h->bmap_blk.ensure((size_t)blk_total * msz));
   HIP_TRY(h, h->bseeds.ensure((size_t)nv_total * sizeof(float)));
-  HIP_TRY(h, h->bqueue.ensure(5 * kQueueSlots * sizeof(unsigned int)));
+  HIP_TRY(h, h->brm_list.ensure((size_t)nv_total * sizeof(uint32_t)));
+  // the queue counters, then per brick phase 3's list count and item offsets (and the total)
+  const size_t nbr = h->bricks.size();
+  HIP_TRY(h, h->bqueue.ensure((5 * kQueueSlots + 2 * nbr + 1) * sizeof(unsigned int)));
   if ((rc = frame_begin(h))) return rc;
   BrickFrame f;
   int64_t total_verts = 0;
@@ -1132,7 +1139,7 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
     const std::function<int(int, bool)> mark = [&](int p, bool end) {
       return timed ? phase_mark(h, p, end) : BEAMS_OK;
     };
-    HIP_TRY(h, launch_zero_u32(f.queue, 5 * kQueueSlots, h->stream));
+    HIP_TRY(h, launch_zero_u32(f.queue, 5 * kQueueSlots + f.nbricks, h->stream));
     if (learned && (r = enqueue_bricks_learned(h, f, lp, mark))) return r;
     if (shadows && !learned) {
       if ((r = mark(BEAMS_PHASE_MAP, false))) return r;
@@ -1142,6 +1149,7 @@ int render_bricks(beams_renderer* h, const CameraParams& c, const float* backgro
       HIP_TRY(h, launch_bricks_exchange(f, h->map_dtype, total_verts, h->stream));
       if ((r = mark(BEAMS_PHASE_EXCHANGE, true))) return r;
       if ((r = mark(BEAMS_PHASE_REMAP, false))) return r;
+      HIP_TRY(h, launch_bricks_finalize(f, h->map_dtype, h->stream));
       HIP_TRY(h, launch_bricks_map(f, h->dtype, h->map_dtype, ntf, true, h->stream));
       if ((r = mark(BEAMS_PHASE_REMAP, true))) return r;
     }
@@ -1235,6 +1243,7 @@ int beams_destroy(beams_renderer* h)
   h->btables.release();
   h->bpartial.release();
   h->bqueue.release();
+  h->brm_list.release();
   h->brows.release();
   h->lb_tab.release();
   h->lb_err.release();

@@ -5,7 +5,9 @@
 //   phase 2  k_bricks_exchange                       per map vertex: every other brick the
 //            light segment crosses (SegmentAABB), that brick's local opacity at the segment's
 //            exit, over-composited in exit-distance order -> seed opacity
-//   phase 3  k_bricks_map (seeded re-march)          persistent, all bricks
+//   phase 3  k_bricks_finalize (copies, constants,   per map vertex
+//            and the list of the rest), then
+//            k_bricks_map (seeded re-march)          persistent, the listed vertices
 //   phase 4  k_bricks_march (footprint tiles)        one workgroup per (brick, tile) item
 //   phase 5  k_bricks_composite (depth-ordered over) per pixel
 // The reference routes 32-byte hit records through rank 0 with MPI; here the hits of a
@@ -22,6 +24,8 @@ namespace beams {
 namespace {
 
 // ---- phases 1 and 3: per-brick TransmittanceMapGenerator -------------------------------------
+// Phase 1 marches every map vertex of every brick; phase 3 (seeded) re-marches the vertices
+// k_bricks_finalize listed, from their seeds, densely packed into work items.
 template <int FDT, int MDT, bool FAST, bool PIPE>
 __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
 {
@@ -31,15 +35,17 @@ __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
   float* a_tab = lds;
   float* xs = lds + ntf;
   unsigned int* counters = f.queue + (seeded ? 2 : 0) * kQueueSlots;
-  // the seeded re-march has many cheap items (vertices already opaque): fetched four at a time
-  const int chunk = seeded ? 4 : 1;
+  // the listed vertices' items (k_rm_items wrote their count), or every vertex's
+  typedef const __attribute__((address_space(4))) int32_t ConstInt;
+  const int nitems = seeded ? __builtin_amdgcn_readfirstlane(((ConstInt*)(const void*)f.rm_item_off)[f.nbricks])
+                            : f.n_map_items;
   queue_init(&queue);
   int staged = -1;
   for (;;) {
-    const int item = next_item(counters, &queue, f.n_map_items, chunk, f.xcd_ranges);
-    if (item >= f.n_map_items) break;
+    const int item = next_item(counters, &queue, nitems, 1, f.xcd_ranges);
+    if (item >= nitems) break;
     // the brick's parameters in scalar registers (the same brick for the whole workgroup)
-    const int b = item_brick(f, item);
+    const int b = seeded ? item_brick_in(f.rm_item_off, f.nbricks, item) : item_brick(f, item);
     const BrickDesc* bk = f.bricks + b;
     const GridParams g = load_uniform(&bk->g);
     const MapParams m = load_uniform(&bk->m);
@@ -50,45 +56,122 @@ __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
     staged = b;
     __syncthreads();
     const float* x[3] = { xs, xs + g.n[0], xs + g.n[0] + g.n[1] };
-    // the item's ordinal of this lane's vertex: x fastest, or along the brick's rows in
-    // light-sheet order (rows whose light paths share a sheet follow one another, so the
-    // workgroups in flight read a few sheets of the field instead of a slab of the brick)
     // (a brick's map holds fewer than 2^31 vertices: 32-bit divisions)
-    const int64_t q = (int64_t)(item - f.map_item_off[b]) * kMapItem + threadIdx.x;
-    if (q < bk->nverts) {
-      const uint32_t mp0 = (uint32_t)m.mp[0], mp1 = (uint32_t)m.mp[1];
-      const uint32_t r = (uint32_t)q / mp0, i = (uint32_t)q - r * mp0;
-      uint32_t j, k;
+    const uint32_t mp0 = (uint32_t)m.mp[0], mp1 = (uint32_t)m.mp[1];
+    uint32_t i, j, k;
+    int64_t v;
+    bool live;
+    if (seeded) {
+      // the item's share of the brick's list
+      const int first = (item - ((ConstInt*)(const void*)f.rm_item_off)[b]) * kMapItem;
+      const int e = first + (int)threadIdx.x;
+      live = e < (int)__builtin_amdgcn_readfirstlane(f.rm_count[b]);
+      const uint32_t vv = live ? f.rm_list[bk->seed_off + e] : 0u;
+      k = vv / (mp0 * mp1);
+      const uint32_t rr = vv - k * (mp0 * mp1);
+      j = rr / mp0;
+      i = rr - j * mp0;
+      v = (int64_t)vv;
+    } else {
+      // the item's ordinal of this lane's vertex: x fastest, or along the brick's rows in
+      // light-sheet order (rows whose light paths share a sheet follow one another, so the
+      // workgroups in flight read a few sheets of the field instead of a slab of the brick)
+      const int64_t q = (int64_t)(item - f.map_item_off[b]) * kMapItem + threadIdx.x;
+      live = q < bk->nverts;
+      const uint32_t r = (uint32_t)q / mp0;
+      i = (uint32_t)q - r * mp0;
       if (f.light_rows) {
-        const uint32_t jk = f.rows[bk->row_off + r];
+        const uint32_t jk = live ? f.rows[bk->row_off + r] : 0u;
         j = jk & 0xffffu;
         k = jk >> 16;
       } else {
         k = r / mp1;
         j = r - k * mp1;
       }
-      const int64_t v = (int64_t)i + (int64_t)mp0 * ((int64_t)j + (int64_t)mp1 * (int64_t)k);
+      v = (int64_t)i + (int64_t)mp0 * ((int64_t)j + (int64_t)mp1 * (int64_t)k);
+    }
+    if (live) {
       // ArrayPortalUniformPointCoordinates::Get, as map_vertex computes it
       const float pos[3] = { m.morg[0] + m.msp[0] * (float)i, m.morg[1] + m.msp[1] * (float)j,
                              m.morg[2] + m.msp[2] * (float)k };
       int64_t taken = 0;
       const float seed = seeded ? f.seeds[bk->seed_off + v] : 0.0f;
-      // A march seeded at opacity 1 returns 1 (each step adds (1 - 1) a = 0 for finite a), and
-      // one seeded at 0 repeats the phase-1 march, whose stored value is copied; both skipped
-      // unless steps are being counted.
-      const bool opaque = seed == 1.0f && sp.alpha_finite && !sp.counters;
-      const bool unseeded = seeded && seed == 0.0f && !sp.counters;
-      if (unseeded) {
-        copy_t<MDT>(bk->map_final, m.map, v);
-      } else {
-        const float op = opaque ? 1.0f
-                       : PIPE ? march_vertex_pipe<FDT, FAST>(g, m, sp, a_tab, x, pos, seed, &taken)
-                              : march_vertex_at<FDT, false, FAST>(g, m, sp, a_tab, x, pos, seed, &taken);
-        store_t<MDT>(seeded ? bk->map_final : m.map, v, op);
-      }
+      const float op = PIPE ? march_vertex_pipe<FDT, FAST>(g, m, sp, a_tab, x, pos, seed, &taken)
+                            : march_vertex_at<FDT, false, FAST>(g, m, sp, a_tab, x, pos, seed, &taken);
+      store_t<MDT>(seeded ? bk->map_final : m.map, v, op);
       if (sp.counters) count_add(&sp.counters[1], (uint32_t)taken);
     }
   }
+}
+
+// Phase 3's bulk pass (grid (G, nbricks), one contiguous run of vertices per workgroup): a
+// vertex seeded at 0 would repeat its phase-1 march, so its local value is copied; one seeded at
+// 1 keeps opacity 1 (each step adds (1 - 1) a = 0 for finite a); the others (all of them while
+// steps are counted) go to the brick's list, in vertex order within each wave.  A workgroup
+// gathers them in LDS and reserves list space once per kFinalBuf - 256 of them (one counter per
+// brick takes every reservation, so per-wave reservations would queue on it), writing each
+// flush as one contiguous run.
+constexpr int kFinalBuf = 2048;
+
+template <int MDT>
+__global__ void __launch_bounds__(256) k_bricks_finalize(BrickFrame f)
+{
+  __shared__ uint32_t buf[kFinalBuf];
+  __shared__ unsigned int nbuf, base;
+  const int b = blockIdx.y;
+  const BrickDesc* bk = f.bricks + b;
+  const int64_t nv = bk->nverts;
+  const bool counting = bk->s.counters != nullptr, finite = bk->s.alpha_finite != 0;
+  const int64_t per = ((nv + gridDim.x - 1) / gridDim.x + 255) / 256 * 256;
+  const int64_t end = std::min<int64_t>(nv, per * (blockIdx.x + 1));
+  const int lane = (int)__lane_id();
+  if (threadIdx.x == 0) nbuf = 0;
+  __syncthreads();
+  auto flush = [&]() {
+    const unsigned int n = nbuf;  // every thread's appends are in (a barrier precedes)
+    if (threadIdx.x == 0) base = n ? atomicAdd(&f.rm_count[b], n) : 0u;
+    __syncthreads();
+    for (unsigned int i = threadIdx.x; i < n; i += 256) f.rm_list[bk->seed_off + base + i] = buf[i];
+    __syncthreads();
+    if (threadIdx.x == 0) nbuf = 0;
+    __syncthreads();
+  };
+  for (int64_t v0 = per * blockIdx.x; v0 < end; v0 += 256) {
+    const int64_t v = v0 + threadIdx.x;
+    bool listed = false;
+    if (v < end) {
+      const float seed = f.seeds[bk->seed_off + v];
+      if (!counting && seed == 0.0f)
+        copy_t<MDT>(bk->map_final, bk->m.map, v);
+      else if (!counting && seed == 1.0f && finite)
+        store_t<MDT>(bk->map_final, v, 1.0f);
+      else
+        listed = true;
+    }
+    const unsigned long long bal = __ballot(listed);
+    if (bal) {
+      const int leader = __builtin_ctzll(bal);
+      unsigned int at = 0;
+      if (lane == leader) at = atomicAdd(&nbuf, (unsigned int)__popcll(bal));
+      at = (unsigned int)__shfl((int)at, leader);
+      if (listed) buf[at + __popcll(bal & ((1ull << lane) - 1ull))] = (uint32_t)v;
+    }
+    __syncthreads();
+    if (nbuf > kFinalBuf - 256) flush();  // room for the next round's 256 at most
+  }
+  flush();
+}
+
+// The listed vertices' map items, kMapItem per item: prefix sums over the bricks.
+__global__ void k_rm_items(BrickFrame f)
+{
+  if (threadIdx.x != 0) return;
+  int total = 0;
+  for (int b = 0; b < f.nbricks; ++b) {
+    f.rm_item_off[b] = total;
+    total += (int)((f.rm_count[b] + kMapItem - 1) / kMapItem);
+  }
+  f.rm_item_off[f.nbricks] = total;
 }
 
 // Bounding sphere (centre c, radius R) of the map vertices [first, last] of a brick: the
@@ -628,7 +711,8 @@ __global__ void __launch_bounds__(256) k_extract(const unsigned char* src, int32
 
 __global__ void k_zero_u32(unsigned int* p, int n)
 {
-  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) p[i] = 0u;
 }
 
 }  // namespace
@@ -663,6 +747,17 @@ hipError_t launch_bricks_map(const BrickFrame& f, int32_t field_dtype, int32_t m
     else
       hipLaunchKernelGGL((k_bricks_map<FDT, MDT, false, false>), grid, dim3(256), lds, st, f, seeded ? 1 : 0);
   }));
+  return hipGetLastError();
+}
+
+hipError_t launch_bricks_finalize(const BrickFrame& f, int32_t map_dtype, hipStream_t st)
+{
+  if (f.nbricks < 1) return hipSuccess;
+  const dim3 grid((unsigned)std::max(8, 4096 / f.nbricks), (unsigned)f.nbricks);
+  BRICK_DISPATCH_MAP(map_dtype, { hipLaunchKernelGGL((k_bricks_finalize<MDT>), grid, dim3(256), 0, st, f); });
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_rm_items, dim3(1), dim3(64), 0, st, f);
   return hipGetLastError();
 }
 
@@ -749,7 +844,7 @@ hipError_t launch_extract_brick(const void* src, int32_t elem_bytes, const int64
 
 hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, p, n);
+  hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)std::max(1, (n + 255) / 256)), dim3(256), 0, st, p, n);
   return hipGetLastError();
 }
 

@@ -44,6 +44,10 @@ struct BrickFrame {
   const int32_t* depth_order;     // bricks by ascending camera-to-centre distance
   unsigned int* queue;            // work-queue counters, kQueueSlots per phase: map-1, exchange, map-3, march, fallback
   float* seeds;                   // non-local opacity per map vertex (phase 2 -> 3)
+  // phase 3: the vertices whose seeded re-march is not a copy or a constant 1
+  unsigned int* rm_count;         // per brick: how many k_bricks_finalize listed
+  int32_t* rm_item_off;           // nbricks + 1 prefix sums of their map items (k_rm_items)
+  uint32_t* rm_list;              // per brick, at its seed offset: their x-fastest map indices
   float4* partial;                // per-brick footprint RGBA
   float* partial_depth;           // per-brick footprint entry distance
   int32_t max_coords;             // the largest brick's n0 + n1 + n2 (LDS: coordinates + reciprocal widths)
@@ -63,6 +67,10 @@ hipError_t launch_bricks_map(const BrickFrame& f, int32_t field_dtype, int32_t m
                              int32_t ntf, bool seeded, hipStream_t st);
 hipError_t launch_bricks_exchange(const BrickFrame& f, int32_t map_dtype, int64_t total_verts,
                                   hipStream_t st);
+// Phase 3's first part: every final map value that needs no march (a copy of the local map where
+// the seed is 0, 1 where it is 1), and the list of the other vertices with its map items; then
+// launch_bricks_map(seeded) re-marches the listed vertices only.  f.rm_count starts at zero.
+hipError_t launch_bricks_finalize(const BrickFrame& f, int32_t map_dtype, hipStream_t st);
 hipError_t launch_bricks_march(const BrickFrame& f, int32_t field_dtype, int32_t map_dtype,
                                int32_t ntf, bool use_map, const CameraParams& c, hipStream_t st);
 hipError_t launch_bricks_composite(const BrickFrame& f, const CameraParams& c, const ImageOut& o,

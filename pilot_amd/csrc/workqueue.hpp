@@ -68,6 +68,24 @@ __device__ __forceinline__ int next_item(unsigned int* counters, ItemQueue* q, i
 }
 
 // The brick of map work item `item` (uniform): the last b with map_item_off[b] <= item.
+// The last b with offs[b] <= item (uniform) for prefix sums offs[0 .. nbricks] that no kernel
+// writes while this one runs.
+__device__ __forceinline__ int item_brick_in(const int32_t* offs, int nbricks, int item)
+{
+  typedef const __attribute__((address_space(4))) int32_t ConstInt;
+  ConstInt* off = (ConstInt*)(const void*)offs;
+  item = __builtin_amdgcn_readfirstlane(item);
+  int lo = 0, hi = nbricks - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= item)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
 template <class Frame>
 __device__ __forceinline__ int item_brick(const Frame& f, int item)
 {
