@@ -38,6 +38,15 @@ __device__ __forceinline__ int64_t nearest_index(const float* x, int64_t n, floa
   return (v - x[i] <= x[i + 1] - v) ? i : i + 1;
 }
 
+// nearest_index in 32 bits (an axis has fewer than 2^31 points): the same index.
+__device__ __forceinline__ int nearest_index32(const float* x, int n, float v, float x0, float ginv)
+{
+  int i = (int)fminf(fmaxf(floorf((v - x0) * ginv), 0.0f), (float)(n - 2));
+  while (i > 0 && x[i] > v) --i;
+  while (i < n - 2 && x[i + 1] < v) ++i;
+  return (v - x[i] <= x[i + 1] - v) ? i : i + 1;
+}
+
 // The TF entry of a scalar as the reference indexes it (truncate, then clamp to the table);
 // clamping in float first gives the same index for every finite value and keeps it 32-bit.
 __device__ __forceinline__ int tf_index(const ShadeParams& s, float val)
@@ -108,6 +117,39 @@ __device__ __forceinline__ float features(const LearnedParams& p, const float* d
   const GridParams& g = p.g;
   const float tc = fminf(coarse_lerp(p, p.coarse_tau, v), kTauMax);
   f[0] = tc * 0.25f;
+  if (p.tfidx) {
+    // the TF entries of the field points, precomputed this frame: one byte per gather, through
+    // a buffer resource over the index field (fewer than 2^32 points: 32-bit offsets, no
+    // 64-bit address per gather); the clamped neighbour offsets per axis
+    uint32_t b[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int n = (int)g.n[d];
+      const int c = nearest_index32(g.x[d], n, v[d], g.gx0[d], g.ginv[d]);
+      const uint32_t st = (uint32_t)g.field_stride[d], oc = (uint32_t)c * st;
+      b[d][0] = c > 0 ? oc - st : oc;
+      b[d][1] = oc;
+      b[d][2] = c < n - 1 ? oc + st : oc;
+    }
+    // readfirstlane returns int: each half goes through uint32_t so the low one is not
+    // sign-extended into the high one when the OR forms the 64-bit base
+    const uint64_t tp = (uint64_t)p.tfidx;
+    const uint32_t tlo = __builtin_amdgcn_readfirstlane((uint32_t)tp);
+    const uint32_t thi = __builtin_amdgcn_readfirstlane((uint32_t)(tp >> 32));
+    void* tbase = (void*)(((uint64_t)thi << 32) | tlo);
+    const uint32_t npts = (uint32_t)(g.n[0] * g.n[1] * g.n[2]);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(tbase, (short)0, (int)npts, 0x00020000);
+#pragma unroll
+    for (int zy = 0; zy < 9; ++zy) {
+      const uint32_t row = b[2][zy / 3] + b[1][zy % 3];
+#pragma unroll
+      for (int t = 0; t < 3; ++t)
+        f[1 + 3 * zy + t] = d_tab[__builtin_amdgcn_raw_buffer_load_b8(rs, (int)(row + b[0][t]), 0, 0)];
+    }
+    light_dir(p.m, v, f + 28);
+    f[31] = 0.0f;
+    return tc;
+  }
   // element offsets of the clamped neighbour indices per axis
   int64_t o[3][3];
 #pragma unroll
@@ -117,19 +159,6 @@ __device__ __forceinline__ float features(const LearnedParams& p, const float* d
     o[d][0] = lo * g.field_stride[d];
     o[d][1] = c * g.field_stride[d];
     o[d][2] = hi * g.field_stride[d];
-  }
-  if (p.tfidx) {
-    // the TF entries of the field points, precomputed this frame: one byte per gather
-    uint32_t b[3][3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int t = 0; t < 3; ++t) b[d][t] = (uint32_t)o[d][t];
-#pragma unroll
-    for (int q = 0; q < 27; ++q) f[1 + q] = d_tab[p.tfidx[b[2][q / 9] + b[1][(q / 3) % 3] + b[0][q % 3]]];
-    light_dir(p.m, v, f + 28);
-    f[31] = 0.0f;
-    return tc;
   }
   float val[27];
   if (p.field_small) {
