@@ -117,7 +117,8 @@ int beams_set_field(beams_renderer* h, const void* values, int32_t dtype, int32_
  * renders one MPI rank's block: point_ranges holds 6 inclusive point-index bounds per brick
  * (i0,i1,j0,j1,k0,k1; neighbours share their boundary plane), each brick is stored with
  * `ghost` extra point layers (clipped at the field's edge; outside the brick's bounds), and
- * map_cells gives 3 map cells per brick (NULL: the brick's own cell counts).  A frame then
+ * map_cells gives 3 map cells per brick (NULL: the brick's own cell counts; fewer than 2^31
+ * map vertices per brick, else BEAMS_ERR_UNSUPPORTED).  A frame then
  * runs, for all bricks at once: local maps (phase 1), the non-local hit exchange and
  * composite (TransmittanceMap.h:342-513, LightedVolumeRenderer.cxx:968-1235, phase 2), the
  * seeded re-march (phase 3), the per-brick march (phase 4) and the depth-ordered over

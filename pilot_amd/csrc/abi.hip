@@ -1414,6 +1414,9 @@ int beams_set_bricks(beams_renderer* h, int32_t nbricks, const int64_t* point_ra
         return h->fail(BEAMS_ERR_INVALID, "bricks: map cells must be in [1, 4096]");
     }
     br.nverts = (br.cells[0] + 1) * (br.cells[1] + 1) * (br.cells[2] + 1);
+    // the brick kernels index a brick's map vertices in 32 bits
+    if (br.nverts > INT32_MAX)
+      return h->fail(BEAMS_ERR_UNSUPPORTED, "bricks: a brick's map must have fewer than 2^31 vertices");
     br.seed_off = off;
     off += br.nverts;
   }

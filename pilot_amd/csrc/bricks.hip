@@ -298,8 +298,10 @@ __device__ __forceinline__ void row_of(const BrickFrame& f, const BrickDesc& bd,
     j = jk & 0xffffu;
     k = jk >> 16;
   } else {
-    j = r % bd.m.mp[1];
-    k = r / bd.m.mp[1];
+    // (a brick's map holds fewer than 2^31 vertices: 32-bit divisions)
+    const uint32_t mp1 = (uint32_t)bd.m.mp[1], q = (uint32_t)r / mp1;
+    j = (int64_t)((uint32_t)r - q * mp1);
+    k = (int64_t)q;
   }
 }
 
@@ -449,8 +451,9 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
     for (int sub = 0; sub < 4; ++sub) {
       const int64_t qv = item_first + sub * 64 + lane;
       if (qv >= bd.nverts) continue;
+      // (a brick's map holds fewer than 2^31 vertices: a 32-bit division)
       const int64_t mp0 = bd.m.mp[0];
-      const int64_t r = qv / mp0, i = qv - r * mp0;
+      const int64_t r = (int64_t)((uint32_t)qv / (uint32_t)mp0), i = qv - r * mp0;
       int64_t rj, rk;
       row_of(f, bd, r, rj, rk);
       const int64_t v = i + mp0 * (rj + bd.m.mp[1] * rk);

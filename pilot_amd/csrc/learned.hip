@@ -492,7 +492,11 @@ __global__ void __launch_bounds__(kBlock) k_bricks_learned_map(LearnedParams p, 
     const int64_t id = (int64_t)(item - f.map_item_off[b]) * kMapItem + wave * 64 + lane;
     const bool valid = id < nv;
     float v[3], feat[kFeat];
-    map_vertex(d.m, valid ? id : nv - 1, v);
+    // (a brick's map holds fewer than 2^31 vertices; the one-block frame's map may not)
+    if (nv <= INT32_MAX)
+      map_vertex32(d.m, (uint32_t)(valid ? id : nv - 1), v);
+    else
+      map_vertex(d.m, valid ? id : nv - 1, v);
     const float tc = features<FDT>(p, d_tab, v, feat);
     const float y = p.params ? mlp_forward_fast(fr, s.w, s.buf[wave], feat) : 0.0f;
     if (valid) store_t<MDT>(d.map_final, id, 1.0f - expf(-fmaxf(tc - y, 0.0f)));

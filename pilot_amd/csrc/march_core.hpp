@@ -572,6 +572,18 @@ __device__ __forceinline__ void map_vertex(const MapParams& m, int64_t id, float
   v[2] = m.morg[2] + m.msp[2] * (float)k;
 }
 
+// map_vertex for a vertex id below 2^31: the same indices from 32-bit divisions (the 64-bit
+// ones are long emulated sequences), so the same position.
+__device__ __forceinline__ void map_vertex32(const MapParams& m, uint32_t id, float* v)
+{
+  const uint32_t mp0 = (uint32_t)m.mp[0], mp1 = (uint32_t)m.mp[1];
+  const uint32_t row = id / mp0, i = id - row * mp0;
+  const uint32_t k = row / mp1, j = row - k * mp1;
+  v[0] = m.morg[0] + m.msp[0] * (float)i;
+  v[1] = m.morg[1] + m.msp[1] * (float)j;
+  v[2] = m.morg[2] + m.msp[2] * (float)k;
+}
+
 __device__ __forceinline__ void light_origin(const MapParams& m, const float* v, float* o)
 {
   if (m.light_kind == 1) {

@@ -202,3 +202,20 @@ def test_bricks_kd_composite_order_matches_oracle(oracle_mod):
         r.SetCompositeOrder(B.ORDER_KD_FRONT_TO_BACK)
         with pytest.raises(B.BeamsError, match="kd decomposition"):
             r.RenderImage(sc2.camera, 32, 32)
+
+
+def test_bricks_refuse_maps_of_2_31_vertices():
+    """A brick map of 2^31 vertices or more is refused before anything is allocated (the brick
+    kernels index a brick's map vertices in 32 bits), and the renderer then takes and renders
+    an ordinary decomposition."""
+    sc = scenes.make_scene("bricks", 33, seed=5, nblobs=4, width=32, height=32, light="point")
+    ranges = B.kd_bricks(sc.dims, 2, 5, 8, 32)
+    with B.LightedVolumeRenderer() as r:
+        scenes.configure(r, sc)
+        big = np.tile(np.array([[4096, 4096, 127]], np.int64), (len(ranges), 1))
+        assert 4097 * 4097 * 128 >= 2 ** 31  # map vertices per brick
+        with pytest.raises(B.BeamsError, match="2\\^31"):
+            r.SetBricks(ranges, ghost=1, map_cells=big)
+        r.SetBricks(ranges, ghost=1)
+        img, _ = r.RenderImage(sc.camera, 32, 32)
+        assert np.isfinite(img).all()
