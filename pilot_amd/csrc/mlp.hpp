@@ -88,6 +88,20 @@ __device__ __forceinline__ float relu(float x)
   return __builtin_bit_cast(float, max(__builtin_bit_cast(int32_t, x), 0));
 }
 
+// ReLU of a and b rounded to BF16, packed in one word (a low): both are rounded first (one
+// v_cvt_pk_bf16_f32), then a packed 16-bit integer max takes the ReLU on their bits.  Rounding
+// keeps the sign (a negative value or -0 gives a BF16 with the sign bit set, NaN keeps its
+// sign bit), so the bits equal bf16_hw(relu(a)) and bf16_hw(relu(b)).
+__device__ __forceinline__ uint32_t relu_bf16x2(float a, float b)
+{
+  typedef short s16x2 __attribute__((ext_vector_type(2)));
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  typedef __bf16 b16x2 __attribute__((ext_vector_type(2)));
+  const f32x2 v = { a, b };
+  const s16x2 w = __builtin_bit_cast(s16x2, __builtin_convertvector(v, b16x2));
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(w, (s16x2)0));
+}
+
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
 {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -253,11 +267,8 @@ __device__ __forceinline__ float mlp_forward_fast(const MlpFrags& fr, const MlpB
 #pragma unroll
     for (int vt = 0; vt < 4; ++vt) {
       const f32x4 acc = mfma(a, xb[vt], bias);
-      typedef __attribute__((__vector_size__(4 * sizeof(uint16_t)))) uint16_t u16x4;
-      u16x4 h;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) h[i] = bf16_hw(relu(acc[i]));
-      *(u16x4*)(buf + (16 * vt + r) * kLd + 16 * nt + 4 * g) = h;
+      const uint2 h = make_uint2(relu_bf16x2(acc[0], acc[1]), relu_bf16x2(acc[2], acc[3]));
+      *(uint2*)(buf + (16 * vt + r) * kLd + 16 * nt + 4 * g) = h;
     }
   }
   wave_lds_sync();
