@@ -573,6 +573,9 @@ int prepare_learned(beams_renderer* h, const GridParams& g, const ShadeParams& s
   f.queue = (unsigned int*)h->sb_queue.p;
   f.persistent_groups = h->persistent_groups;
   f.xcd_ranges = h->xcd_ranges;
+  // the fallback tiles' march: the schedule's switches (the launch checks the field's size)
+  f.map_fast = h->map_fast_fetch;
+  f.map_pipeline = h->map_pipeline;
   LearnedBricks& lb = h->sb_lb;
   lb.chk_off = (const int32_t*)h->sb_tab.p;
   lb.tile_off = lb.chk_off + 2;

@@ -626,7 +626,9 @@ __global__ void __launch_bounds__(kBlock) k_learned_tiles(BrickFrame f, LearnedB
 
 // Fallback: the whole volume's light march for every vertex of the listed tiles (one tile per
 // work item).  The list was written by earlier kernels, so its words go through the scalar cache.
-template <int FDT, int MDT>
+// FAST and PIPE as in the phase-1 map launches: the branch-free corner fetch (the whole field
+// below 4 GiB, strides below 2^24) and the next step's loads in flight; the same opacities.
+template <int FDT, int MDT, bool FAST, bool PIPE>
 __global__ void __launch_bounds__(kBlock) k_bricks_exact_tiles(LearnedParams p, BrickFrame f, LearnedBricks lb)
 {
   typedef const __attribute__((address_space(4))) int32_t ConstInt;
@@ -671,8 +673,13 @@ __global__ void __launch_bounds__(kBlock) k_bricks_exact_tiles(LearnedParams p, 
     for (int v = threadIdx.x; v < c0 * c1 * c2; v += kBlock) {
       const int i = i0 + v % c0, j = j0 + (v / c0) % c1, k = k0 + v / (c0 * c1);
       const int64_t id = i + (int64_t)mp0 * (j + (int64_t)mp1 * k);
+      // map_vertex's position of vertex (i, j, k)
+      const float pos[3] = { m.morg[0] + m.msp[0] * (float)i, m.morg[1] + m.msp[1] * (float)j,
+                             m.morg[2] + m.msp[2] * (float)k };
       int64_t taken = 0;
-      store_t<MDT>(bk->map_final, id, march_vertex<FDT>(p.g, m, p.s, a_tab, x, id, 0.0f, &taken));
+      const float op = PIPE ? march_vertex_pipe<FDT, FAST>(p.g, m, p.s, a_tab, x, pos, 0.0f, &taken)
+                            : march_vertex_at<FDT, false, FAST>(p.g, m, p.s, a_tab, x, pos, 0.0f, &taken);
+      store_t<MDT>(bk->map_final, id, op);
     }
   }
 }
@@ -716,8 +723,17 @@ hipError_t launch_bricks_exact_tiles(const LearnedParams& p, const BrickFrame& f
 {
   const size_t lds = sizeof(float) * (size_t)(p.s.ntf + 2 * (p.g.n[0] + p.g.n[1] + p.g.n[2]));
   const dim3 grid((unsigned)std::max(1, std::min(lb.n_tiles, f.persistent_groups)));
+  const bool fast = f.map_fast && p.g.bytes32 != 0 && p.g.field_stride[1] < (1 << 24) &&
+                    p.g.field_stride[2] < (1 << 24);
   BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(map_dtype, {
-    hipLaunchKernelGGL((k_bricks_exact_tiles<FDT, MDT>), grid, dim3(kBlock), lds, st, p, f, lb);
+    if (fast && f.map_pipeline)
+      hipLaunchKernelGGL((k_bricks_exact_tiles<FDT, MDT, true, true>), grid, dim3(kBlock), lds, st, p, f, lb);
+    else if (fast)
+      hipLaunchKernelGGL((k_bricks_exact_tiles<FDT, MDT, true, false>), grid, dim3(kBlock), lds, st, p, f, lb);
+    else if (f.map_pipeline)
+      hipLaunchKernelGGL((k_bricks_exact_tiles<FDT, MDT, false, true>), grid, dim3(kBlock), lds, st, p, f, lb);
+    else
+      hipLaunchKernelGGL((k_bricks_exact_tiles<FDT, MDT, false, false>), grid, dim3(kBlock), lds, st, p, f, lb);
   }));
   return hipGetLastError();
 }
