@@ -254,13 +254,18 @@ __global__ void k_coarse_ranges(LearnedParams p)
 // once, coalesced along x, and spreads each column's sum over the (one or two) coarse x boxes
 // holding that column.  A field coarser than the mip (a column in three or more boxes) sums
 // each box directly instead.  Field x extent up to kCoarseMaxX points.
+// Rows of 16-byte aligned vectors (x contiguous, the row extent and strides multiples of the
+// vector): each thread reads 16 bytes per row (8 FP16/BF16 or 4 FP32 columns), sums its
+// columns over its share of the rows in registers, and each box then sums its columns.
 constexpr int kCoarseMaxX = 4097;
 
 template <int FDT>
 __global__ void __launch_bounds__(kBlock) k_coarse_sigma(LearnedParams p)
 {
+  constexpr int V = FDT == kF32 ? 4 : 8;  // columns per 16-byte vector
   __shared__ float sig_tab[1024];
   __shared__ int16_t first_box[kCoarseMaxX], second_box[kCoarseMaxX];
+  __shared__ float colsum[kCoarseMaxX];
   __shared__ float sums[64];
   __shared__ int crowded;
   const GridParams& g = p.g;
@@ -270,6 +275,7 @@ __global__ void __launch_bounds__(kBlock) k_coarse_sigma(LearnedParams p)
   for (int i = threadIdx.x; i < n0; i += blockDim.x) {
     first_box[i] = -1;
     second_box[i] = -1;
+    colsum[i] = 0.0f;
   }
   for (int i = threadIdx.x; i < cn; i += blockDim.x) sums[i] = 0.0f;
   __syncthreads();
@@ -303,7 +309,50 @@ __global__ void __launch_bounds__(kBlock) k_coarse_sigma(LearnedParams p)
       atomicAdd(&sums[cx], acc);
     }
   }
-  for (int i = threadIdx.x; i < (crowded ? 0 : n0); i += blockDim.x) {
+  const bool vec = !crowded && g.field_stride[0] == 1 && n0 % V == 0 && g.field_stride[1] % V == 0 &&
+                   g.field_stride[2] % V == 0 && ((uintptr_t)g.field & 15) == 0;
+  if (vec) {
+    // vector column vc of the row, row phase rp of P (consecutive threads: consecutive vectors)
+    const int ncv = n0 / V, cw = ncv < kBlock ? ncv : kBlock, P = kBlock / cw;
+    const int rp = (int)threadIdx.x / cw, rows = cnt1 * cnt2;
+    for (int vc = (int)threadIdx.x % cw; rp < P && vc < ncv; vc += cw) {
+      float acc[V];
+#pragma unroll
+      for (int c = 0; c < V; ++c) acc[c] = 0.0f;
+      int jj = rp % cnt1, kk = rp / cnt1;
+      for (int row = rp; row < rows; row += P) {
+        const int64_t off = (int64_t)(lo1 + jj) * g.field_stride[1] + (int64_t)(lo2 + kk) * g.field_stride[2] + vc * V;
+        const uint4 w = *(const uint4*)((const char*)g.field + off * (FDT == kF32 ? 4 : 2));
+        const uint32_t ws[4] = { w.x, w.y, w.z, w.w };
+#pragma unroll
+        for (int c = 0; c < V; ++c) {
+          float val;
+          if constexpr (FDT == kF32)
+            val = __uint_as_float(ws[c]);
+          else if constexpr (FDT == kF16)
+            val = f16_to_f32((uint16_t)(ws[c / 2] >> (16 * (c & 1))));
+          else
+            val = bf16_to_f32((uint16_t)(ws[c / 2] >> (16 * (c & 1))));
+          acc[c] += sig_tab[tf_index(p.s, val)];
+        }
+        jj += P;
+        while (jj >= cnt1) {
+          jj -= cnt1;
+          ++kk;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < V; ++c) atomicAdd(&colsum[vc * V + c], acc[c]);
+    }
+    __syncthreads();
+    for (int cx = threadIdx.x; cx < cn; cx += blockDim.x) {
+      const int lo0 = p.coarse_ranges[2 * cx], cnt0 = p.coarse_ranges[2 * cx + 1];
+      float acc = 0.0f;
+      for (int i = lo0; i < lo0 + cnt0; ++i) acc += colsum[i];
+      sums[cx] = acc;
+    }
+  }
+  for (int i = threadIdx.x; i < (crowded || vec ? 0 : n0); i += blockDim.x) {
     float acc = 0.0f;
     for (int row = 0; row < cnt1 * cnt2; ++row) {
       const int64_t j = lo1 + row % cnt1, k = lo2 + row / cnt1;
