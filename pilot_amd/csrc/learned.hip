@@ -28,23 +28,49 @@ __device__ __forceinline__ int64_t lower_index(const float* x, int64_t n, float 
   return lo;
 }
 
-// Index of the grid coordinate nearest v (ties to the lower one), v clamped to the grid.
+// Index of the grid coordinate nearest v (ties to the lower one), v clamped to the grid: from
+// the uniform-spacing guess i, walk down while x[i] > v, then up while x[i+1] < v.  The guess's
+// two coordinates are read first, without waiting on each other, and the walks run only when
+// one of their tests would move i (otherwise neither loop would step), so the index is the
+// walk's.
 __device__ __forceinline__ int64_t nearest_index(const float* x, int64_t n, float v, float x0, float ginv)
 {
   int64_t i = (int64_t)floorf((v - x0) * ginv);
   i = i < 0 ? 0 : (i > n - 2 ? n - 2 : i);
-  while (i > 0 && x[i] > v) --i;
-  while (i < n - 2 && x[i + 1] < v) ++i;
-  return (v - x[i] <= x[i + 1] - v) ? i : i + 1;
+  float a = x[i], b = x[i + 1];
+  if (((int)(i > 0) & (int)(a > v)) | ((int)(i < n - 2) & (int)(b < v))) {
+    while (i > 0 && x[i] > v) --i;
+    while (i < n - 2 && x[i + 1] < v) ++i;
+    a = x[i];
+    b = x[i + 1];
+  }
+  return (v - a <= b - v) ? i : i + 1;
 }
 
-// nearest_index in 32 bits (an axis has fewer than 2^31 points): the same index.
-__device__ __forceinline__ int nearest_index32(const float* x, int n, float v, float x0, float ginv)
+// nearest_index on the three axes of v in 32 bits (an axis has fewer than 2^31 points): the
+// same indices, with the six guess coordinates read together before any test.
+__device__ __forceinline__ void nearest_index3(const GridParams& g, const float* v, int* c)
 {
-  int i = (int)fminf(fmaxf(floorf((v - x0) * ginv), 0.0f), (float)(n - 2));
-  while (i > 0 && x[i] > v) --i;
-  while (i < n - 2 && x[i + 1] < v) ++i;
-  return (v - x[i] <= x[i + 1] - v) ? i : i + 1;
+  int i[3], n[3];
+  float a[3], b[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    n[d] = (int)g.n[d];
+    i[d] = (int)fminf(fmaxf(floorf((v[d] - g.gx0[d]) * g.ginv[d]), 0.0f), (float)(n[d] - 2));
+    a[d] = g.x[d][i[d]];
+    b[d] = g.x[d][i[d] + 1];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float* x = g.x[d];
+    if (((int)(i[d] > 0) & (int)(a[d] > v[d])) | ((int)(i[d] < n[d] - 2) & (int)(b[d] < v[d]))) {
+      while (i[d] > 0 && x[i[d]] > v[d]) --i[d];
+      while (i[d] < n[d] - 2 && x[i[d] + 1] < v[d]) ++i[d];
+      a[d] = x[i[d]];
+      b[d] = x[i[d] + 1];
+    }
+    c[d] = (v[d] - a[d] <= b[d] - v[d]) ? i[d] : i[d] + 1;
+  }
 }
 
 // The TF entry of a scalar as the reference indexes it (truncate, then clamp to the table);
@@ -122,10 +148,11 @@ __device__ __forceinline__ float features(const LearnedParams& p, const float* d
     // a buffer resource over the index field (fewer than 2^32 points: 32-bit offsets, no
     // 64-bit address per gather); the clamped neighbour offsets per axis
     uint32_t b[3][3];
+    int cc[3];
+    nearest_index3(g, v, cc);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const int n = (int)g.n[d];
-      const int c = nearest_index32(g.x[d], n, v[d], g.gx0[d], g.ginv[d]);
+      const int n = (int)g.n[d], c = cc[d];
       const uint32_t st = (uint32_t)g.field_stride[d], oc = (uint32_t)c * st;
       b[d][0] = c > 0 ? oc - st : oc;
       b[d][1] = oc;
@@ -534,21 +561,36 @@ __global__ void __launch_bounds__(kBlock) k_bricks_learned_map(LearnedParams p, 
   stage_patch(p.s, d_tab);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // The current brick's item range, vertex count, map geometry and output, in scalar registers:
+  // a workgroup's items are gridDim.x apart, so it stays on a brick for many items, and the
+  // brick is looked up (and its words loaded) again only when an item leaves the range.
+  typedef const __attribute__((address_space(4))) int32_t ConstInt;
+  ConstInt* item_off = (ConstInt*)(const void*)f.map_item_off;
+  int b_lo = 0, b_hi = 0;
+  int64_t nv = 0;
+  MapParams dm;
+  void* map_final = nullptr;
   for (int item = blockIdx.x; item < f.n_map_items; item += gridDim.x) {
-    const int b = item_brick(f, item);
-    const BrickDesc& d = f.bricks[b];
-    const int64_t nv = d.nverts;
-    const int64_t id = (int64_t)(item - f.map_item_off[b]) * kMapItem + wave * 64 + lane;
+    if (item < b_lo || item >= b_hi) {
+      const int b = item_brick(f, item);
+      const BrickDesc* bk = f.bricks + b;
+      b_lo = item_off[b];
+      b_hi = item_off[b + 1];
+      nv = load_uniform(&bk->nverts);
+      dm = load_uniform(&bk->m);
+      map_final = load_uniform(&bk->map_final);
+    }
+    const int64_t id = (int64_t)(item - b_lo) * kMapItem + wave * 64 + lane;
     const bool valid = id < nv;
     float v[3], feat[kFeat];
     // (a brick's map holds fewer than 2^31 vertices; the one-block frame's map may not)
     if (nv <= INT32_MAX)
-      map_vertex32(d.m, (uint32_t)(valid ? id : nv - 1), v);
+      map_vertex32(dm, (uint32_t)(valid ? id : nv - 1), v);
     else
-      map_vertex(d.m, valid ? id : nv - 1, v);
+      map_vertex(dm, valid ? id : nv - 1, v);
     const float tc = features<FDT>(p, d_tab, v, feat);
     const float y = p.params ? mlp_forward_fast(fr, s.w, s.buf[wave], feat) : 0.0f;
-    if (valid) store_t<MDT>(d.map_final, id, 1.0f - expf(-fmaxf(tc - y, 0.0f)));
+    if (valid) store_t<MDT>(map_final, id, 1.0f - expf(-fmaxf(tc - y, 0.0f)));
   }
 }
 
