@@ -365,10 +365,12 @@ __device__ __forceinline__ bool exchange_hit(const BrickFrame& f, int j, int b, 
   return true;
 }
 
-// Hits kept sorted per vertex in a lane-private LDS column (slot k of lane l at
-// hits[k][l]); a path crossing more bricks is composited by repeated selection of the next
-// key over the candidates (same order, no storage).  The item's candidates are ordered by
-// their distance along the light first, so hits mostly arrive in order and are appended.
+// Hits kept sorted per vertex in a lane-private LDS column (slot k of lane l: the key's exit
+// distance bits at hit_t[k][l], its brick at hit_b[k][l]); a path crossing more bricks is
+// composited by repeated selection of the next key over the candidates (same order, no
+// storage).  The item's candidates are ordered by their distance along the light first, so
+// hits mostly arrive in order and are appended.  Candidate and hit bricks are 16-bit (at most
+// 4096 bricks): the exchange's LDS per wave (8.3 KB) sets how many waves a CU holds.
 constexpr int kRegHits = 16;
 
 // Ordering key of a candidate brick: distance of its centre from the point light, or its
@@ -385,8 +387,9 @@ template <int MDT>
 __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
 {
   __shared__ ItemQueue queue;
-  __shared__ int cand[kMaxCandidates];
-  __shared__ unsigned long long hits[kRegHits][64];
+  __shared__ uint16_t cand[kMaxCandidates];
+  __shared__ uint32_t hit_t[kRegHits][64];
+  __shared__ uint16_t hit_b[kRegHits][64];
   const int lane = threadIdx.x;
   const int chunk = 1;
   queue_init(&queue);
@@ -428,7 +431,7 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
         }
         const unsigned long long mask = __ballot(keep);
         const int pos = ncand + __popcll(mask & ((1ull << lane) - 1ull));
-        if (keep && pos < kMaxCandidates) cand[pos] = j;
+        if (keep && pos < kMaxCandidates) cand[pos] = (uint16_t)j;
         ncand += __popcll(mask);
       }
       __syncthreads();
@@ -442,7 +445,7 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
           rank += (kk < key || (kk == key && k < lane)) ? 1 : 0;
         }
         __syncthreads();
-        if (lane < ncand) cand[rank] = j;
+        if (lane < ncand) cand[rank] = (uint16_t)j;
         __syncthreads();
       }
     }
@@ -488,7 +491,15 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
           local = map_opacity<MDT>(f.bricks[j].m, q);
         return op + (1.0f - op) * local;
       };
-      unsigned long long top = 0;  // the largest key so far (hits[nh-1][lane])
+      // slot k of this lane's column: a key (exit distance bits << 32 | brick) split in two
+      auto put = [&](int k, unsigned long long key) {
+        hit_t[k][lane] = (uint32_t)(key >> 32);
+        hit_b[k][lane] = (uint16_t)key;
+      };
+      auto get = [&](int k) {
+        return ((unsigned long long)hit_t[k][lane] << 32) | (unsigned long long)hit_b[k][lane];
+      };
+      unsigned long long top = 0;  // the largest key so far (slot nh - 1)
       int nh = 0;
       bool spill = false;
       for (int q = 0; q < nloop; ++q) {
@@ -501,23 +512,23 @@ __global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
           break;
         }
         if (nh == 0 || key > top) {
-          hits[nh][lane] = key;  // in order: append
+          put(nh, key);  // in order: append
           top = key;
         } else {
           // insertion: the larger keys move up one slot
           int k = nh;
           for (; k > 0; --k) {
-            const unsigned long long prev = hits[k - 1][lane];
+            const unsigned long long prev = get(k - 1);
             if (prev < key) break;
-            hits[k][lane] = prev;
+            put(k, prev);
           }
-          hits[k][lane] = key;
+          put(k, key);
         }
         ++nh;
       }
       float op = 0.0f;
       if (!spill) {
-        for (int k = 0; k < nh; ++k) op = composite(op, hits[k][lane]);
+        for (int k = 0; k < nh; ++k) op = composite(op, get(k));
       } else {
         bool started = false;
         unsigned long long last = 0;
