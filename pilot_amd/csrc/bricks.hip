@@ -326,7 +326,7 @@ __device__ __forceinline__ bool run_may_cross(const MapParams& m, int64_t j, int
 // (LightedVolumeRenderer.cxx:1214-1235), one thread per map vertex ------------------------------
 // Per work item the wave first keeps the bricks whose bounding sphere may meet a light segment
 // of the item (conservative; the per-vertex FP32 slab cull and the exact FP64 test follow).
-constexpr int kMaxCandidates = 1024;
+constexpr int kMaxCandidates = 1000;
 
 // One light segment (o -> p) against brick j: the conservative FP32 slab cull on the brick's
 // inflated box, then the exact test (FP64 slab terms); key = (exit distance bits, brick).
@@ -384,7 +384,7 @@ __device__ __forceinline__ float cand_order(const MapParams& m, const float* bx)
 }
 
 template <int MDT>
-__global__ void __launch_bounds__(64) k_bricks_exchange(BrickFrame f)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_bricks_exchange(BrickFrame f)
 {
   __shared__ ItemQueue queue;
   __shared__ uint16_t cand[kMaxCandidates];
