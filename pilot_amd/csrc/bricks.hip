@@ -370,7 +370,8 @@ __device__ __forceinline__ bool exchange_hit(const BrickFrame& f, int j, int b, 
 // composited by repeated selection of the next key over the candidates (same order, no
 // storage).  The item's candidates are ordered by their distance along the light first, so
 // hits mostly arrive in order and are appended.  Candidate and hit bricks are 16-bit (at most
-// 4096 bricks): the exchange's LDS per wave (8.3 KB) sets how many waves a CU holds.
+// 4096 bricks): with a list of 1000 candidates a wave's LDS is 8160 B, so a CU holds 20 waves,
+// as many as 96 VGPRs allow (five per SIMD).
 constexpr int kRegHits = 16;
 
 // Ordering key of a candidate brick: distance of its centre from the point light, or its
