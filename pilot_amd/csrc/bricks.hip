@@ -41,11 +41,19 @@ __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
                             : f.n_map_items;
   queue_init(&queue);
   int staged = -1;
+  // the current brick and its item range: a workgroup's items mostly follow one another in a
+  // brick, so the brick is searched for only when an item leaves the range
+  ConstInt* offs = (ConstInt*)(const void*)(seeded ? f.rm_item_off : f.map_item_off);
+  int b = 0, b_lo = 0, b_hi = 0;
   for (;;) {
     const int item = next_item(counters, &queue, nitems, 1, f.xcd_ranges);
     if (item >= nitems) break;
+    if (item < b_lo || item >= b_hi) {
+      b = seeded ? item_brick_in(f.rm_item_off, f.nbricks, item) : item_brick(f, item);
+      b_lo = offs[b];
+      b_hi = offs[b + 1];
+    }
     // the brick's parameters in scalar registers (the same brick for the whole workgroup)
-    const int b = seeded ? item_brick_in(f.rm_item_off, f.nbricks, item) : item_brick(f, item);
     const BrickDesc* bk = f.bricks + b;
     const GridParams g = load_uniform(&bk->g);
     const MapParams m = load_uniform(&bk->m);
@@ -63,7 +71,7 @@ __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
     bool live;
     if (seeded) {
       // the item's share of the brick's list
-      const int first = (item - ((ConstInt*)(const void*)f.rm_item_off)[b]) * kMapItem;
+      const int first = (item - b_lo) * kMapItem;
       const int e = first + (int)threadIdx.x;
       live = e < (int)__builtin_amdgcn_readfirstlane(f.rm_count[b]);
       const uint32_t vv = live ? f.rm_list[bk->seed_off + e] : 0u;
@@ -76,7 +84,7 @@ __global__ void __launch_bounds__(256) k_bricks_map(BrickFrame f, int seeded)
       // the item's ordinal of this lane's vertex: x fastest, or along the brick's rows in
       // light-sheet order (rows whose light paths share a sheet follow one another, so the
       // workgroups in flight read a few sheets of the field instead of a slab of the brick)
-      const int64_t q = (int64_t)(item - f.map_item_off[b]) * kMapItem + threadIdx.x;
+      const int64_t q = (int64_t)(item - b_lo) * kMapItem + threadIdx.x;
       live = q < bk->nverts;
       const uint32_t r = (uint32_t)q / mp0;
       i = (uint32_t)q - r * mp0;
