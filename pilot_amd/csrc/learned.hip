@@ -621,7 +621,9 @@ __device__ __forceinline__ int brick_of(const int32_t* off, int nb, int v)
 // (the single-block reference march) to the brick's vertex.  One item = 256 check slots of one
 // brick (each brick's slots are padded to whole items); the largest error over all bricks goes
 // to list[1] (non-negative floats order as their bits, a NaN's bits above every number).
-template <int FDT, int MDT>
+// FAST and PIPE as in the phase-1 map launches; check slots and brick map indices are 32-bit
+// (a brick's map holds fewer than 2^31 vertices).
+template <int FDT, int MDT, bool FAST, bool PIPE>
 __global__ void __launch_bounds__(kBlock) k_bricks_learned_check(LearnedParams p, BrickFrame f, LearnedBricks lb)
 {
   extern __shared__ float lds[];
@@ -648,20 +650,25 @@ __global__ void __launch_bounds__(kBlock) k_bricks_learned_check(LearnedParams p
     const int b = __builtin_amdgcn_readfirstlane(lo);
     const BrickDesc* bk = f.bricks + b;
     const MapParams m = whole_volume_march(load_uniform(&bk->m), p.m);
-    const int64_t n0 = check_count(m.mp[0] - 1, lb.stride), n1 = check_count(m.mp[1] - 1, lb.stride);
-    const int64_t n2 = check_count(m.mp[2] - 1, lb.stride);
-    const int64_t q = (int64_t)(slot0 - coff[b]) + threadIdx.x;
+    const int n0 = (int)check_count(m.mp[0] - 1, lb.stride), n1 = (int)check_count(m.mp[1] - 1, lb.stride);
+    const int n2 = (int)check_count(m.mp[2] - 1, lb.stride);
+    const int q = slot0 - coff[b] + (int)threadIdx.x;
     if (q < n0 * n1 * n2) {
-      const int64_t qi[3] = { q % n0, (q / n0) % n1, q / (n0 * n1) };
-      int64_t vi[3];
+      const int q1 = q / n0, q2 = q1 / n1;
+      const int qi[3] = { q - q1 * n0, q1 - q2 * n1, q2 };
+      int vi[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const int64_t i = qi[a] * lb.stride;
-        vi[a] = i < m.mp[a] - 1 ? i : m.mp[a] - 1;
+        const int i = qi[a] * lb.stride, last = (int)m.mp[a] - 1;
+        vi[a] = i < last ? i : last;
       }
       const int64_t id = vi[0] + m.mp[0] * (vi[1] + m.mp[1] * vi[2]);
+      // map_vertex's position of vertex id
+      const float pos[3] = { m.morg[0] + m.msp[0] * (float)vi[0], m.morg[1] + m.msp[1] * (float)vi[1],
+                             m.morg[2] + m.msp[2] * (float)vi[2] };
       int64_t taken = 0;
-      const float exact = march_vertex<FDT>(p.g, m, p.s, a_tab, x, id, 0.0f, &taken);
+      const float exact = PIPE ? march_vertex_pipe<FDT, FAST>(p.g, m, p.s, a_tab, x, pos, 0.0f, &taken)
+                               : march_vertex_at<FDT, false, FAST>(p.g, m, p.s, a_tab, x, pos, 0.0f, &taken);
       const float e = fabsf(load_t<MDT>(bk->map_final, id) - exact);
       lb.verr[slot0 + threadIdx.x] = e;
       worst = max(worst, f32_bits(e));
@@ -777,6 +784,14 @@ __global__ void __launch_bounds__(kBlock) k_bricks_exact_tiles(LearnedParams p, 
 
 }  // namespace
 
+// Whether the whole volume's light march takes the branch-free corner fetch: the schedule's
+// switch (and the bricks' own conditions) in f.map_fast, the field below 4 GiB, strides below
+// 2^24.
+static bool whole_field_fast(const LearnedParams& p, const BrickFrame& f)
+{
+  return f.map_fast && p.g.bytes32 != 0 && p.g.field_stride[1] < (1 << 24) && p.g.field_stride[2] < (1 << 24);
+}
+
 hipError_t launch_bricks_learned_map(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
                                      hipStream_t st)
 {
@@ -795,8 +810,16 @@ hipError_t launch_bricks_learned_check(const LearnedParams& p, const BrickFrame&
   if (e != hipSuccess) return e;
   const size_t lds = sizeof(float) * (size_t)(p.s.ntf + 2 * (p.g.n[0] + p.g.n[1] + p.g.n[2]));
   const dim3 grid((unsigned)std::max(1, std::min(lb.n_chk_items, 2048)));
+  const bool fast = whole_field_fast(p, f);
   BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(map_dtype, {
-    hipLaunchKernelGGL((k_bricks_learned_check<FDT, MDT>), grid, dim3(kBlock), lds, st, p, f, lb);
+    if (fast && f.map_pipeline)
+      hipLaunchKernelGGL((k_bricks_learned_check<FDT, MDT, true, true>), grid, dim3(kBlock), lds, st, p, f, lb);
+    else if (fast)
+      hipLaunchKernelGGL((k_bricks_learned_check<FDT, MDT, true, false>), grid, dim3(kBlock), lds, st, p, f, lb);
+    else if (f.map_pipeline)
+      hipLaunchKernelGGL((k_bricks_learned_check<FDT, MDT, false, true>), grid, dim3(kBlock), lds, st, p, f, lb);
+    else
+      hipLaunchKernelGGL((k_bricks_learned_check<FDT, MDT, false, false>), grid, dim3(kBlock), lds, st, p, f, lb);
   }));
   if ((e = hipGetLastError()) != hipSuccess) return e;
   return launch_learned_tiles(f, lb, st);
@@ -814,8 +837,7 @@ hipError_t launch_bricks_exact_tiles(const LearnedParams& p, const BrickFrame& f
 {
   const size_t lds = sizeof(float) * (size_t)(p.s.ntf + 2 * (p.g.n[0] + p.g.n[1] + p.g.n[2]));
   const dim3 grid((unsigned)std::max(1, std::min(lb.n_tiles, f.persistent_groups)));
-  const bool fast = f.map_fast && p.g.bytes32 != 0 && p.g.field_stride[1] < (1 << 24) &&
-                    p.g.field_stride[2] < (1 << 24);
+  const bool fast = whole_field_fast(p, f);
   BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(map_dtype, {
     if (fast && f.map_pipeline)
       hipLaunchKernelGGL((k_bricks_exact_tiles<FDT, MDT, true, true>), grid, dim3(kBlock), lds, st, p, f, lb);
