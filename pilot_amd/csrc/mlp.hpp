@@ -264,10 +264,13 @@ __device__ __forceinline__ float mlp_forward_fast(const MlpFrags& fr, const MlpB
     const bf16x8 a = fr.w1[nt];
     // the bias enters as the accumulator input (rounded once, after the products)
     const f32x4 bias = *(const f32x4*)&w.b1[16 * nt + 4 * g];
+    // the four row tiles' products first, then their epilogues (no wait on each product)
+    f32x4 acc[4];
+#pragma unroll
+    for (int vt = 0; vt < 4; ++vt) acc[vt] = mfma(a, xb[vt], bias);
 #pragma unroll
     for (int vt = 0; vt < 4; ++vt) {
-      const f32x4 acc = mfma(a, xb[vt], bias);
-      const uint2 h = make_uint2(relu_bf16x2(acc[0], acc[1]), relu_bf16x2(acc[2], acc[3]));
+      const uint2 h = make_uint2(relu_bf16x2(acc[vt][0], acc[vt][1]), relu_bf16x2(acc[vt][2], acc[vt][3]));
       *(uint2*)(buf + (16 * vt + r) * kLd + 16 * nt + 4 * g) = h;
     }
   }
@@ -285,13 +288,16 @@ __device__ __forceinline__ float mlp_forward_fast(const MlpFrags& fr, const MlpB
     const bf16x8 a1 = fr.w2[nt][1];
     const f32x4 bias = *(const f32x4*)&w.b2[16 * nt + 4 * g];
     const f32x4 w3 = *(const f32x4*)&w.w3[16 * nt + 4 * g];
+    // the four row tiles' first products, then their second, then the epilogues
+    f32x4 acc[4];
 #pragma unroll
-    for (int vt = 0; vt < 4; ++vt) {
-      f32x4 acc = mfma(a0, hb[vt][0], bias);
-      acc = mfma(a1, hb[vt][1], acc);
+    for (int vt = 0; vt < 4; ++vt) acc[vt] = mfma(a0, hb[vt][0], bias);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) part[vt] += relu(acc[i]) * w3[i];
-    }
+    for (int vt = 0; vt < 4; ++vt) acc[vt] = mfma(a1, hb[vt][1], acc[vt]);
+#pragma unroll
+    for (int vt = 0; vt < 4; ++vt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) part[vt] += relu(acc[vt][i]) * w3[i];
   }
   // row 16 vt + r's output: sum over the four lane groups; lane 16 g + r keeps row tile g
 #pragma unroll
