@@ -153,7 +153,7 @@ struct beams_renderer {
   // learned bricks: check-slot and tile offsets (host copy and device table), per-check-vertex
   // errors, and the fallback's tile list
   std::vector<int32_t> lb_host;
-  DevBuf lb_tab, lb_err, lb_list;
+  DevBuf lb_tab, lb_err, lb_list, lb_fail;
   LearnedBricks lb{};
   // learned single block: its fallback tiles through the brick kernels, as a frame of one brick
   // whose map is the whole volume's (device descriptor, queue, tables, errors, tile list)
@@ -581,6 +581,7 @@ int prepare_learned(beams_renderer* h, const GridParams& g, const ShadeParams& s
   lb.tile_off = lb.chk_off + 2;
   lb.verr = (float*)h->sb_err.p;
   lb.list = (int32_t*)h->sb_list.p;
+  lb.tile_fail = nullptr;
   lb.n_chk_items = (int32_t)(slots / 256);
   lb.n_tiles = (int32_t)tiles;
   lb.stride = h->check_stride;
@@ -1047,11 +1048,13 @@ int prepare_bricks_learned(beams_renderer* h, const BrickFrame& f)
   }
   HIP_TRY(h, h->lb_err.ensure(sizeof(float) * (size_t)slots));
   HIP_TRY(h, h->lb_list.ensure(sizeof(int32_t) * (2 + (size_t)tiles)));
+  HIP_TRY(h, h->lb_fail.ensure((size_t)tiles + 1));
   LearnedBricks& lb = h->lb;
   lb.chk_off = (const int32_t*)h->lb_tab.p;
   lb.tile_off = lb.chk_off + nb + 1;
   lb.verr = (float*)h->lb_err.p;
   lb.list = (int32_t*)h->lb_list.p;
+  lb.tile_fail = (uint8_t*)h->lb_fail.p;
   lb.n_chk_items = (int32_t)(slots / 256);
   lb.n_tiles = (int32_t)tiles;
   lb.stride = h->check_stride;
@@ -1080,8 +1083,11 @@ int enqueue_bricks_learned(beams_renderer* h, const BrickFrame& f, const Learned
     HIP_TRY(h, launch_tf_index_field(p, (uint8_t*)h->tfidx.p, h->stream));
     pm.tfidx = (const uint8_t*)h->tfidx.p;
   }
-  HIP_TRY(h, launch_bricks_learned_map(pm, f, h->map_dtype, h->stream));
+  // the network at the check vertices, the check and the tile list, then the network at the
+  // other vertices (waves wholly inside listed tiles skipped), then the fallback
+  HIP_TRY(h, launch_bricks_learned_map(pm, f, h->map_dtype, &h->lb, true, h->stream));
   HIP_TRY(h, launch_bricks_learned_check(p, f, h->map_dtype, h->lb, h->stream));
+  HIP_TRY(h, launch_bricks_learned_map(pm, f, h->map_dtype, &h->lb, false, h->stream));
   if ((rc = mark(BEAMS_PHASE_MAP, true))) return rc;
   if ((rc = mark(BEAMS_PHASE_REMAP, false))) return rc;
   HIP_TRY(h, launch_bricks_exact_tiles(p, f, h->map_dtype, h->lb, h->stream));
@@ -1251,6 +1257,7 @@ int beams_destroy(beams_renderer* h)
   h->lb_tab.release();
   h->lb_err.release();
   h->lb_list.release();
+  h->lb_fail.release();
   h->sb_desc.release();
   h->sb_queue.release();
   h->sb_tab.release();

@@ -547,9 +547,27 @@ __global__ void __launch_bounds__(kBlock) k_learned_samples(LearnedParams p, Map
 }
 
 // ---- bricks: every brick's map from the whole volume's coarse optical depth --------------
+// The brick of slot `v` in a table of nb + 1 ascending offsets: the last b with off[b] <= v.
+__device__ __forceinline__ int brick_of(const int32_t* off, int nb, int v)
+{
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= v)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 // One map work item (256 vertices of one brick) per workgroup iteration, 64 per wave.
-template <int FDT, int MDT>
-__global__ void __launch_bounds__(kBlock) k_bricks_learned_map(LearnedParams p, BrickFrame f)
+// AT_CHECKS: the items are the check slots (256 per item, lb.chk_off) and each lane's vertex
+// is its check vertex.  Else, with lb.tile_fail, a wave whose vertices all lie in listed tiles
+// (every vertex belongs to one tile: tile t along an axis holds [t s, (t+1) s), the last one
+// up to the end) is skipped, since the fallback writes every vertex of those tiles.
+template <int FDT, int MDT, bool AT_CHECKS>
+__global__ void __launch_bounds__(kBlock) k_bricks_learned_map(LearnedParams p, BrickFrame f, LearnedBricks lb)
 {
   __shared__ MapLds s;
   extern __shared__ float d_tab[];
@@ -565,29 +583,67 @@ __global__ void __launch_bounds__(kBlock) k_bricks_learned_map(LearnedParams p, 
   // a workgroup's items are gridDim.x apart, so it stays on a brick for many items, and the
   // brick is looked up (and its words loaded) again only when an item leaves the range.
   typedef const __attribute__((address_space(4))) int32_t ConstInt;
-  ConstInt* item_off = (ConstInt*)(const void*)f.map_item_off;
-  int b_lo = 0, b_hi = 0;
+  ConstInt* item_off = (ConstInt*)(const void*)(AT_CHECKS ? lb.chk_off : f.map_item_off);
+  ConstInt* tile_off = (ConstInt*)(const void*)lb.tile_off;
+  const int nitems = AT_CHECKS ? lb.n_chk_items : f.n_map_items;
+  const int s_chk = lb.stride;
+  int b = 0, b_lo = 0, b_hi = 0, n0 = 0, n1 = 0, n2 = 0;
   int64_t nv = 0;
   MapParams dm;
   void* map_final = nullptr;
-  for (int item = blockIdx.x; item < f.n_map_items; item += gridDim.x) {
+  for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
     if (item < b_lo || item >= b_hi) {
-      const int b = item_brick(f, item);
+      if (AT_CHECKS)
+        b = brick_of((const int32_t*)lb.chk_off, f.nbricks, item * kBlock);
+      else
+        b = item_brick(f, item);
+      b = __builtin_amdgcn_readfirstlane(b);
       const BrickDesc* bk = f.bricks + b;
-      b_lo = item_off[b];
-      b_hi = item_off[b + 1];
+      b_lo = AT_CHECKS ? item_off[b] / kBlock : item_off[b];
+      b_hi = AT_CHECKS ? item_off[b + 1] / kBlock : item_off[b + 1];
       nv = load_uniform(&bk->nverts);
       dm = load_uniform(&bk->m);
       map_final = load_uniform(&bk->map_final);
+      if (AT_CHECKS || lb.tile_fail) {
+        n0 = (int)check_count(dm.mp[0] - 1, s_chk);
+        n1 = (int)check_count(dm.mp[1] - 1, s_chk);
+        n2 = (int)check_count(dm.mp[2] - 1, s_chk);
+      }
     }
-    const int64_t id = (int64_t)(item - b_lo) * kMapItem + wave * 64 + lane;
-    const bool valid = id < nv;
+    int64_t id;
+    bool valid;
+    if (AT_CHECKS) {
+      // check slot q of the brick: check vertex (every s-th per axis and the last)
+      const int q = (item - b_lo) * kBlock + (int)threadIdx.x;
+      valid = q < n0 * n1 * n2;
+      const int qq = valid ? q : 0, q1 = qq / n0, q2 = q1 / n1;
+      const int qi[3] = { qq - q1 * n0, q1 - q2 * n1, q2 };
+      int vi[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int i = qi[a] * s_chk, last = (int)dm.mp[a] - 1;
+        vi[a] = i < last ? i : last;
+      }
+      id = vi[0] + dm.mp[0] * (vi[1] + dm.mp[1] * vi[2]);
+    } else {
+      id = (int64_t)(item - b_lo) * kMapItem + wave * 64 + lane;
+      valid = id < nv;
+      if (lb.tile_fail && nv <= INT32_MAX) {
+        // the tile of this lane's vertex; the wave is skipped when every lane's tile is listed
+        const uint32_t u = (uint32_t)(valid ? id : nv - 1), mp0 = (uint32_t)dm.mp[0], mp1 = (uint32_t)dm.mp[1];
+        const uint32_t row = u / mp0, i = u - row * mp0, k = row / mp1, j = row - k * mp1;
+        const int ti = min((int)(i / (uint32_t)s_chk), n0 - 2), tj = min((int)(j / (uint32_t)s_chk), n1 - 2);
+        const int tk = min((int)(k / (uint32_t)s_chk), n2 - 2);
+        const bool listed = !valid || lb.tile_fail[tile_off[b] + ti + (n0 - 1) * (tj + (n1 - 1) * tk)] != 0;
+        if (__ballot(!listed) == 0) continue;
+      }
+    }
     float v[3], feat[kFeat];
     // (a brick's map holds fewer than 2^31 vertices; the one-block frame's map may not)
     if (nv <= INT32_MAX)
-      map_vertex32(dm, (uint32_t)(valid ? id : nv - 1), v);
+      map_vertex32(dm, (uint32_t)(valid ? id : (AT_CHECKS ? 0 : nv - 1)), v);
     else
-      map_vertex(dm, valid ? id : nv - 1, v);
+      map_vertex(dm, valid ? id : (AT_CHECKS ? 0 : nv - 1), v);
     const float tc = features<FDT>(p, d_tab, v, feat);
     const float y = p.params ? mlp_forward_fast(fr, s.w, s.buf[wave], feat) : 0.0f;
     if (valid) store_t<MDT>(map_final, id, 1.0f - expf(-fmaxf(tc - y, 0.0f)));
@@ -601,20 +657,6 @@ __device__ __forceinline__ MapParams whole_volume_march(const MapParams& brick_m
   m.step = whole.step;
   m.dir_far = whole.dir_far;
   return m;
-}
-
-// The brick of slot `v` in a table of nb + 1 ascending offsets: the last b with off[b] <= v.
-__device__ __forceinline__ int brick_of(const int32_t* off, int nb, int v)
-{
-  int lo = 0, hi = nb - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (off[mid] <= v)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
 }
 
 // Per check vertex: |learned - exact|, the exact value being the whole volume's light march
@@ -709,6 +751,7 @@ __global__ void __launch_bounds__(kBlock) k_learned_tiles(BrickFrame f, LearnedB
       w = max(w, f32_bits(e[ci + n0 * (cj + n1 * ck)]));
     }
     fail = !(bits_f32(w) <= lb.budget);
+    if (lb.tile_fail) lb.tile_fail[g] = fail ? 1 : 0;
   }
   const unsigned long long mask = __ballot(fail);
   if (mask) {
@@ -793,12 +836,19 @@ static bool whole_field_fast(const LearnedParams& p, const BrickFrame& f)
 }
 
 hipError_t launch_bricks_learned_map(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
-                                     hipStream_t st)
+                                     const LearnedBricks* lb, bool at_checks, hipStream_t st)
 {
-  if (p.s.ntf > 1024) return hipErrorInvalidValue;
-  const dim3 grid((unsigned)std::max(1, std::min(f.n_map_items, f.persistent_groups)));
+  if (p.s.ntf > 1024 || (at_checks && !lb)) return hipErrorInvalidValue;
+  LearnedBricks l{};
+  if (lb) l = *lb;
+  const int nitems = at_checks ? l.n_chk_items : f.n_map_items;
+  if (nitems == 0) return hipSuccess;
+  const dim3 grid((unsigned)std::max(1, std::min(nitems, f.persistent_groups)));
   BEAMS_DISPATCH_FIELD(p.g.dtype, BEAMS_DISPATCH_MAP(map_dtype, {
-    hipLaunchKernelGGL((k_bricks_learned_map<FDT, MDT>), grid, dim3(kBlock), sizeof(float) * p.s.ntf, st, p, f);
+    if (at_checks)
+      hipLaunchKernelGGL((k_bricks_learned_map<FDT, MDT, true>), grid, dim3(kBlock), sizeof(float) * p.s.ntf, st, p, f, l);
+    else
+      hipLaunchKernelGGL((k_bricks_learned_map<FDT, MDT, false>), grid, dim3(kBlock), sizeof(float) * p.s.ntf, st, p, f, l);
   }));
   return hipGetLastError();
 }

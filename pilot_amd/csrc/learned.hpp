@@ -80,6 +80,7 @@ struct LearnedBricks {
   const int32_t* tile_off;  // nbricks + 1: first tile of each brick
   float* verr;              // per check slot: |learned - exact|
   int32_t* list;            // [0] tiles listed, [1] largest check error (float bits), [2, ...) the tiles
+  uint8_t* tile_fail;       // optional, per tile: 1 when the tile is listed (re-marched), else 0
   int32_t n_chk_items;      // chk_off[nbricks] / 256
   int32_t n_tiles;          // tile_off[nbricks]
   int32_t stride;
@@ -90,8 +91,11 @@ BEAMS_HD int64_t check_count(int64_t cells, int32_t stride)
 {
   return cells / stride + 1 + (cells % stride != 0 ? 1 : 0);
 }
+// at_checks: the learned value at every check vertex of lb only (before the check); else at
+// every map vertex, skipping the waves whose vertices all lie in tiles lb.tile_fail marks
+// (the fallback re-marches them), when lb and its flags are given.
 hipError_t launch_bricks_learned_map(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
-                                     hipStream_t st);
+                                     const LearnedBricks* lb, bool at_checks, hipStream_t st);
 // Zeroes the list's two counters, then the check and the tile listing.
 hipError_t launch_bricks_learned_check(const LearnedParams& p, const BrickFrame& f, int32_t map_dtype,
                                        const LearnedBricks& lb, hipStream_t st);
