@@ -144,11 +144,17 @@ __global__ void __launch_bounds__(256) k_bricks_finalize(BrickFrame f)
     if (threadIdx.x == 0) nbuf = 0;
     __syncthreads();
   };
-  for (int64_t v0 = per * blockIdx.x; v0 < end; v0 += 256) {
+  // each iteration's seeds are read one iteration ahead, so their latency overlaps the
+  // previous iteration's copies and barrier
+  const float* seeds = f.seeds + bk->seed_off;
+  int64_t v0 = per * blockIdx.x;
+  float next = v0 + threadIdx.x < end ? seeds[v0 + threadIdx.x] : 0.0f;
+  for (; v0 < end; v0 += 256) {
     const int64_t v = v0 + threadIdx.x;
+    const float seed = next;
+    if (v + 256 < end) next = seeds[v + 256];
     bool listed = false;
     if (v < end) {
-      const float seed = f.seeds[bk->seed_off + v];
       if (!counting && seed == 0.0f)
         copy_t<MDT>(bk->map_final, bk->m.map, v);
       else if (!counting && seed == 1.0f && finite)
