@@ -144,19 +144,31 @@ __global__ void __launch_bounds__(256) k_bricks_finalize(BrickFrame f)
     if (threadIdx.x == 0) nbuf = 0;
     __syncthreads();
   };
-  // each iteration's seeds are read one iteration ahead, so their latency overlaps the
-  // previous iteration's copies and barrier
+  // each iteration's seeds and local map words are read one iteration ahead, so their latency
+  // overlaps the previous iteration's stores and barrier (the local word of every vertex is
+  // read, most of them being copied)
+  typedef typename std::conditional<MDT == kF32, uint32_t, uint16_t>::type Word;
   const float* seeds = f.seeds + bk->seed_off;
+  const Word* local = (const Word*)bk->m.map;
   int64_t v0 = per * blockIdx.x;
-  float next = v0 + threadIdx.x < end ? seeds[v0 + threadIdx.x] : 0.0f;
+  float next = 0.0f;
+  Word next_w = 0;
+  if (v0 + threadIdx.x < end) {
+    next = seeds[v0 + threadIdx.x];
+    next_w = local[v0 + threadIdx.x];
+  }
   for (; v0 < end; v0 += 256) {
     const int64_t v = v0 + threadIdx.x;
     const float seed = next;
-    if (v + 256 < end) next = seeds[v + 256];
+    const Word w = next_w;
+    if (v + 256 < end) {
+      next = seeds[v + 256];
+      next_w = local[v + 256];
+    }
     bool listed = false;
     if (v < end) {
       if (!counting && seed == 0.0f)
-        copy_t<MDT>(bk->map_final, bk->m.map, v);
+        ((Word*)bk->map_final)[v] = w;  // the local value, bit for bit
       else if (!counting && seed == 1.0f && finite)
         store_t<MDT>(bk->map_final, v, 1.0f);
       else
