@@ -674,8 +674,8 @@ __global__ void __launch_bounds__(256) k_block_maps(BrickFrame f)
   const int w = (int)threadIdx.x & 63, di = w & 3, dj = (w >> 2) & 3, dk = w >> 4;
   const int64_t q = blk / nbx;
   int bx = (int)(blk - q * nbx), by = (int)(q % nby), bz = (int)(q / nby);
-  // eight blocks' gathers in flight before their stores
-  constexpr int kInFlight = 8;
+  // sixteen blocks' gathers in flight before their stores
+  constexpr int kInFlight = 16;
   typedef typename std::conditional<MDT == kF32, uint32_t, uint16_t>::type Word;
   for (; blk < end; blk += 4 * kInFlight) {
     Word v[kInFlight];
