@@ -42,15 +42,6 @@ template <> __device__ __forceinline__ void store_t<kF16>(void* p, int64_t i, fl
   ((uint16_t*)p)[i] = f32_to_f16(v);
 }
 
-// Element i of one stored map copied to another of the same storage type, bit for bit.
-template <int DT> __device__ __forceinline__ void copy_t(void* dst, const void* src, int64_t i)
-{
-  if (DT == kF32)
-    ((uint32_t*)dst)[i] = ((const uint32_t*)src)[i];
-  else
-    ((uint16_t*)dst)[i] = ((const uint16_t*)src)[i];
-}
-
 // The 8 corner scalars of a field cell (hex order 0:(i,j,k) 1:(i+1,j,k) 2:(i+1,j+1,k)
 // 3:(i,j+1,k), 4-7 the same at k+1).  A field view of fewer than 2^32 bytes is read through a
 // raw buffer resource: the cell's byte offset is the only per-lane address and the corners'
